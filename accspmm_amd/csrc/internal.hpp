@@ -1,0 +1,119 @@
+// internal.hpp — shared internals of libaccspmm (handle layout, HIP checks).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/accspmm.h"
+
+#define ACC_HIP(expr)                                        \
+  do {                                                       \
+    hipError_t _e = (expr);                                  \
+    if (_e != hipSuccess) {                                  \
+      (void)hipGetLastError();                               \
+      return _e == hipErrorOutOfMemory ? ACCSPMM_ERR_OUT_OF_MEMORY \
+                                       : ACCSPMM_ERR_HIP;    \
+    }                                                        \
+  } while (0)
+
+#define ACC_TRY(expr)                          \
+  do {                                         \
+    accspmm_status _s = (expr);                \
+    if (_s != ACCSPMM_OK) return _s;           \
+  } while (0)
+
+namespace accspmm {
+
+constexpr uint32_t kColWindow = 8;
+constexpr uint32_t kPad = 0xFFFFFFFFu;
+
+// Device buffer owned by RAII.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  accspmm_status alloc(size_t n) {
+    release();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return ACCSPMM_ERR_OUT_OF_MEMORY;
+    }
+    bytes = n;
+    return ACCSPMM_OK;
+  }
+  // grows (never shrinks) to at least n bytes; contents not preserved
+  accspmm_status reserve(size_t n) {
+    if (p && bytes >= n) return ACCSPMM_OK;
+    return alloc(n);
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Work unit of the SpMM kernels: a contiguous tile range of one row window.
+// `slot` >= 0 marks a piece of a split window (partials combined in a fixed
+// order by the last piece to finish); -1 writes C directly.
+struct alignas(16) WorkUnit {
+  uint32_t window;
+  uint32_t tile_begin;
+  uint32_t tile_end;
+  int32_t slot;  // split-piece slot (global piece index), or -1
+};
+
+struct Schedule {
+  uint32_t split_tiles = 0xFFFFFFFFu;
+  uint32_t num_units = 0;
+  uint32_t num_split_windows = 0;
+  uint32_t num_pieces = 0;  // units with slot >= 0
+  DevBuf units;             // WorkUnit[num_units]
+  DevBuf piece_first;       // per split window: first piece slot, count
+  DevBuf counters;          // per split window arrival counter (self-resetting)
+};
+
+}  // namespace accspmm
+
+struct accspmm_metcf {
+  int device = 0;
+  uint32_t rows = 0, cols = 0, nnz = 0;
+  uint32_t rw = 8, words = 1;
+  uint32_t W = 0, T = 0, G = 0;
+  uint32_t ibd = 0, balanced = 0, max_window_tiles = 0;
+  accspmm_build_opts opts{};
+  accspmm::DevBuf rwo, tco, bits, a2b, data, grp, rix;
+  std::vector<uint32_t> h_rwo;  // host copy of rowWindowOffset
+  // lazily built state (guarded by mu; handles are otherwise immutable)
+  mutable std::mutex mu;
+  mutable std::map<uint32_t, std::unique_ptr<accspmm::Schedule>> schedules;
+  mutable accspmm::DevBuf bconv;    // converted-B workspace (BF16 forms)
+  mutable accspmm::DevBuf partials; // split-window partial sums
+  uint64_t device_bytes() const {
+    return rwo.bytes + tco.bytes + bits.bytes + a2b.bytes + data.bytes +
+           grp.bytes + rix.bytes;
+  }
+};
+
+namespace accspmm {
+// scan.hip
+accspmm_status exclusive_scan_u32(const uint32_t* in, uint32_t* out,
+                                  uint64_t n, hipStream_t s);
+accspmm_status inclusive_scan_u32(const uint32_t* in, uint32_t* out,
+                                  uint64_t n, hipStream_t s);
+// spmm.hip
+accspmm_status get_schedule(const accspmm_metcf* h, uint32_t split_tiles,
+                            hipStream_t s, const Schedule** out);
+}  // namespace accspmm

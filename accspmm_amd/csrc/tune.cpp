@@ -1,0 +1,216 @@
+// tune.cpp — tile-shape autotuner for the METCF SpMM kernels.
+//
+// The reference hard-codes its launch shape: warps per block K/32 or K/64
+// (include/tf32_comp.hpp:19-25), one block per row window or AdpBME group
+// (:27-31).  Here the shape (features per lane NB, waves per block, window
+// splitting threshold, XCD feature slicing) is chosen per (degree histogram,
+// K, form): candidates are timed with HIP events on the caller's stream and
+// the fastest is cached in memory and, optionally, in a JSON file
+// (also read at first use from $ACCSPMM_TUNE_CACHE).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace accspmm {
+void default_tile(const accspmm_metcf* h, uint32_t K, int form,
+                  accspmm_tile* t);
+accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
+                           float* C, uint32_t ldc, uint32_t K, int form,
+                           const accspmm_tile& tile, hipStream_t s,
+                           int phases);
+}  // namespace accspmm
+
+using namespace accspmm;
+
+namespace {
+
+std::mutex g_mu;
+std::map<std::string, accspmm_tile> g_cache;
+bool g_env_loaded = false;
+
+// Log2 buckets of tiles per window, as percentages rounded to 10%, plus the
+// log2 of the window count; graphs with the same key get the same tile.
+std::string make_key(const accspmm_metcf* h, uint32_t K, int form) {
+  uint32_t buckets[24] = {0};
+  for (uint32_t w = 0; w < h->W; ++w) {
+    uint32_t t = h->h_rwo[w + 1] - h->h_rwo[w];
+    uint32_t b = 0;
+    while (b < 23 && (1u << b) <= t) ++b;
+    buckets[b]++;
+  }
+  std::ostringstream os;
+  os << "f" << form << "-k" << K << "-rw" << h->rw << "-w"
+     << (h->W ? (int)std::log2((double)h->W) : 0) << "-h";
+  for (int b = 0; b < 24; ++b) {
+    int pct = h->W ? (int)std::lround(100.0 * buckets[b] / h->W / 10.0) : 0;
+    os << (b ? "." : "") << pct;
+  }
+  return os.str();
+}
+
+void load_file(const char* path) {
+  std::ifstream in(path);
+  if (!in) return;
+  std::string line;
+  // one entry per line: "key": [nb, wpb, split, xcd],
+  while (std::getline(in, line)) {
+    size_t a = line.find('"'), b = line.find('"', a + 1);
+    size_t l = line.find('['), r = line.find(']');
+    if (a == std::string::npos || b == std::string::npos ||
+        l == std::string::npos || r == std::string::npos)
+      continue;
+    std::string key = line.substr(a + 1, b - a - 1);
+    unsigned v[4];
+    if (std::sscanf(line.c_str() + l + 1, "%u , %u , %u , %u", &v[0], &v[1],
+                    &v[2], &v[3]) == 4)
+      g_cache[key] = accspmm_tile{v[0], v[1], v[2], v[3]};
+  }
+}
+
+void save_file(const char* path) {
+  std::ofstream out(path);
+  if (!out) return;
+  out << "{\n";
+  size_t i = 0;
+  for (auto& kv : g_cache) {
+    out << "  \"" << kv.first << "\": [" << kv.second.feat_per_lane << ", "
+        << kv.second.waves_per_block << ", " << kv.second.split_tiles << ", "
+        << kv.second.xcd_slice << "]" << (++i < g_cache.size() ? "," : "")
+        << "\n";
+  }
+  out << "}\n";
+}
+
+void ensure_env() {
+  if (g_env_loaded) return;
+  g_env_loaded = true;
+  const char* p = std::getenv("ACCSPMM_TUNE_CACHE");
+  if (p && *p) load_file(p);
+}
+
+}  // namespace
+
+extern "C" accspmm_status accspmm_tune_key(const accspmm_metcf* h, uint32_t K,
+                                           accspmm_form form, char* buf,
+                                           size_t cap) {
+  if (!h || !buf || cap == 0) return ACCSPMM_ERR_INVALID_ARG;
+  std::string k = make_key(h, K, (int)form);
+  std::snprintf(buf, cap, "%s", k.c_str());
+  return ACCSPMM_OK;
+}
+
+extern "C" accspmm_status accspmm_tile_select(const accspmm_metcf* h,
+                                              uint32_t K, accspmm_form form,
+                                              accspmm_tile* out) {
+  if (!h || !out || K == 0) return ACCSPMM_ERR_INVALID_ARG;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    ensure_env();
+    auto it = g_cache.find(make_key(h, K, (int)form));
+    if (it != g_cache.end()) {
+      *out = it->second;
+      return ACCSPMM_OK;
+    }
+  }
+  default_tile(h, K, (int)form, out);
+  return ACCSPMM_OK;
+}
+
+extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
+                                           const float* B, uint32_t ldb,
+                                           float* C, uint32_t ldc, uint32_t K,
+                                           accspmm_form form,
+                                           const char* cache_path,
+                                           accspmm_tile* best,
+                                           float* best_ms, void* stream) {
+  if (!h || !B || !C || K == 0) return ACCSPMM_ERR_INVALID_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<accspmm_tile> cands;
+  const uint32_t mean = h->W ? std::max<uint32_t>(1, h->T / h->W) : 1;
+  std::vector<uint32_t> splits = {0};
+  for (uint32_t f : {2u, 4u, 8u}) {
+    uint32_t sp = std::max<uint32_t>(32, mean * f);
+    if (h->max_window_tiles > sp) splits.push_back(sp);
+  }
+  for (uint32_t nb : {1u, 2u, 4u, 8u}) {
+    if (nb > 1 && 16 * nb > 2 * ((K + 15) / 16 * 16)) continue;
+    const uint32_t n_kc = (K + 16 * nb - 1) / (16 * nb);
+    for (uint32_t wpb : {2u, 4u, 8u})
+      for (uint32_t sp : splits)
+        for (uint32_t xcd : {0u, 1u}) {
+          if (xcd && (n_kc == 1 || (8 % n_kc) != 0)) continue;
+          cands.push_back(accspmm_tile{nb, wpb, sp, xcd});
+        }
+  }
+  hipEvent_t e0, e1;
+  ACC_HIP(hipEventCreate(&e0));
+  ACC_HIP(hipEventCreate(&e1));
+  float bms = 1e30f;
+  accspmm_tile bt{};
+  for (const auto& c : cands) {
+    accspmm_status st = spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3);
+    if (st != ACCSPMM_OK) continue;  // e.g. alignment not met
+    for (int i = 0; i < 2; ++i) spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3);
+    std::vector<float> t;
+    for (int i = 0; i < 5; ++i) {
+      (void)hipEventRecord(e0, s);
+      spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3);
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      t.push_back(ms);
+    }
+    std::sort(t.begin(), t.end());
+    if (t[2] < bms) {
+      bms = t[2];
+      bt = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (bms >= 1e30f) return ACCSPMM_ERR_UNSUPPORTED;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    ensure_env();
+    g_cache[make_key(h, K, (int)form)] = bt;
+    if (cache_path && *cache_path) save_file(cache_path);
+  }
+  if (best) *best = bt;
+  if (best_ms) *best_ms = bms;
+  return ACCSPMM_OK;
+}
+
+extern "C" const char* accspmm_status_string(accspmm_status s) {
+  switch (s) {
+    case ACCSPMM_OK: return "ok";
+    case ACCSPMM_ERR_INVALID_ARG: return "invalid argument";
+    case ACCSPMM_ERR_NOT_CANONICAL: return "CSR row not strictly increasing in column";
+    case ACCSPMM_ERR_HIP: return "HIP runtime error";
+    case ACCSPMM_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case ACCSPMM_ERR_UNSUPPORTED: return "unsupported option";
+    case ACCSPMM_ERR_BUDGET: return "HBM budget exhausted";
+    case ACCSPMM_ERR_NOT_FOUND: return "not found";
+    case ACCSPMM_ERR_QUEUE_FULL: return "request queue full";
+    case ACCSPMM_ERR_MTX: return "Matrix Market parse error";
+    case ACCSPMM_ERR_NOT_READY: return "not ready";
+    case ACCSPMM_ERR_SHAPE: return "operand shapes do not agree";
+  }
+  return "unknown status";
+}
+
+extern "C" const char* accspmm_version(void) {
+  return "accspmm-mi355x 0.1 gfx950";
+}

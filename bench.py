@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""METCF SpMM benchmark on one MI355X.
+
+Workload (default): the 232,965-node symmetric Chung-Lu power-law graph
+(exponent 2.1, mean degree 492, self loops, seeded relabel, ~114.8M nnz,
+values 1/deg of the row) times a dense 232,965 x 128 FP32 B, U[-1,1].
+A step is one SpMM C = A*B with A resident on the GPU in METCF form and B
+resident in HBM: the B conversion pass (BF16 forms) plus the SpMM kernel.
+
+Prints ONE JSON line on stdout: GFLOP/s (2*nnz*K/t) of the headline form
+(split-BF16), the roofline of the SpMM kernel against HBM, the CPU baseline
+(the oracle's OpenMP CSR SpMM on the host cores) and per-form numbers with
+their max-rel / Frobenius errors against the FP64 oracle.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # 232,965-node Chung-Lu, exponent 2.1, mean degree 492, 1/deg values
+    "chung_lu_233k": dict(kind="chung_lu", n=232965, deg=492.0, gamma=2.1, seed=11,
+                          values=1, bseed=12),
+    # 100^3 27-point stencil, U[-1,1] values
+    "stencil27_100": dict(kind="stencil", g=100, seed=13, bseed=14),
+    # 16,384-node Chung-Lu, exponent 2.1, mean degree 16, U(0,1] symmetric values
+    "chung_lu_16k": dict(kind="chung_lu", n=16384, deg=16.0, gamma=2.1, seed=1,
+                         values=0, bseed=2),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_graph(spec):
+    import accspmm_amd as acc
+    if spec["kind"] == "chung_lu":
+        return acc.gen_chung_lu(spec["n"], spec["deg"], spec["gamma"], spec["seed"],
+                                spec["values"])
+    return acc.gen_stencil27(spec["g"], spec["seed"])
+
+
+def alg_bytes(M, N, nnz, K, form):
+    s = 2 if form == 2 else 4
+    return 4 * (M + 1) + nnz * (4 + s) + N * K * s + M * K * 4
+
+
+def time_form(g, B, Cd, form, tile, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        g.prepare_b(B, form, tile)
+        g.spmm_prepared(B, Cd, form, tile)
+    torch.cuda.synchronize()
+    e0 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    e1 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    e2 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        e0[i].record()
+        g.prepare_b(B, form, tile)
+        e1[i].record()
+        g.spmm_prepared(B, Cd, form, tile)
+        e2[i].record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    conv = [a.elapsed_time(b) for a, b in zip(e0, e1)]
+    kern = [a.elapsed_time(b) for a, b in zip(e1, e2)]
+    return {
+        "ms_per_step": (t1 - t0) * 1e3 / steps,
+        "kernel_ms_mean": statistics.fmean(kern),
+        "kernel_ms_median": statistics.median(kern),
+        "convert_ms_mean": statistics.fmean(conv),
+    }
+
+
+def cpu_baseline(csr, B, K, min_seconds=10.0, max_runs=10):
+    """The oracle's OpenMP FP32 CSR SpMM on the host cores (a baseline only)."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    out, used = O.spmm_f32_omp(csr.row_ptr, csr.col_idx, csr.data, B, threads=threads)
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < max_runs and (time.perf_counter() - t_start < min_seconds
+                                     or len(times) < 3):
+        t0 = time.perf_counter()
+        O.spmm_f32_omp(csr.row_ptr, csr.col_idx, csr.data, B, out=out, threads=threads)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": 2.0 * csr.nnz * K / med / 1e9, "unit": "GFLOP/s", "cores": used,
+            "kind": "port", "runs": len(times), "seconds_per_run": med}
+
+
+def check_errors(csr, B, results):
+    """Max-rel (componentwise vs |A||B|) and Frobenius errors vs the FP64
+    oracle, for every form's C."""
+    from oracle import oracle as O
+    C64 = O.spmm_f64(csr.row_ptr, csr.col_idx, csr.data, B)
+    ab = O.spmm_abs_f64(csr.row_ptr, csr.col_idx, csr.data, B)
+    errs = {}
+    for name, Cm in results.items():
+        mx, fr = O.errors(Cm, C64, ab)
+        errs[name] = (mx, fr)
+    return errs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="chung_lu_233k", choices=sorted(WORKLOADS))
+    ap.add_argument("--K", type=int, default=128)
+    ap.add_argument("--forms", default="split_bf16,fp32,bf16")
+    ap.add_argument("--tile", default="", help="nb,wpb,split,xcd (default: library pick)")
+    ap.add_argument("--tune", action="store_true", help="autotune each form first")
+    ap.add_argument("--no-check", action="store_true", help="skip errors and CPU baseline")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(row-window sharding is described in DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import accspmm_amd as acc
+
+    name_to_form = {"fp32": acc.FORM_FP32, "split_bf16": acc.FORM_SPLIT_BF16,
+                    "bf16": acc.FORM_BF16}
+    forms = [f for f in args.forms.split(",") if f]
+    spec = WORKLOADS[args.workload]
+    K = args.K
+    log(f"bench: generating {args.workload} ...")
+    t0 = time.perf_counter()
+    csr = make_graph(spec)
+    gen_s = time.perf_counter() - t0
+    log(f"bench: {csr.rows} rows, {csr.nnz} nnz, generated in {gen_s:.1f}s")
+    Bh = acc.gen_dense(spec["bseed"] + K, csr.cols, K)
+    torch.cuda.init()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = acc.DeviceGraph(csr)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    inf = g.info
+    log(f"bench: METCF built on GPU in {build_ms:.1f} ms: {inf.num_windows} windows, "
+        f"{inf.num_tiles} tiles, {inf.num_groups} AdpBME groups, ibd={inf.ibd}")
+    B = torch.from_numpy(Bh).cuda()
+    Cd = torch.empty((csr.rows, K), dtype=torch.float32, device="cuda")
+
+    per_form = {}
+    outputs = {}
+    for fname in forms:
+        form = name_to_form[fname]
+        if args.tile:
+            tile = acc.Tile(*[int(x) for x in args.tile.split(",")])
+        elif args.tune:
+            tile, _ = g.autotune(B, Cd, form)
+        else:
+            tile = g.tile_select(K, form)
+        log(f"bench: form={fname} tile={tile.as_tuple()}")
+        r = time_form(g, B, Cd, form, tile, args.steps, args.warmup)
+        flops = 2.0 * csr.nnz * K
+        r["gflops"] = flops / (r["ms_per_step"] * 1e-3) / 1e9
+        r["kernel_gflops"] = flops / (r["kernel_ms_mean"] * 1e-3) / 1e9
+        ab = alg_bytes(csr.rows, csr.cols, csr.nnz, K, form)
+        r["hbm_frac_kernel"] = ab / (r["kernel_ms_mean"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+        r["tile"] = list(tile.as_tuple())
+        per_form[fname] = r
+        log(f"bench: {fname}: {r['ms_per_step']:.3f} ms/step, kernel "
+            f"{r['kernel_ms_mean']:.3f} ms, convert {r['convert_ms_mean']:.3f} ms, "
+            f"{r['gflops']:.1f} GFLOP/s, {100 * r['hbm_frac_kernel']:.1f}% HBM roof")
+        if not args.no_check:
+            torch.cuda.synchronize()
+            outputs[fname] = Cd.cpu().numpy()
+
+    cpu = None
+    if not args.no_check:
+        log("bench: FP64 oracle check and CPU baseline ...")
+        errs = check_errors(csr, Bh, outputs)
+        for fname, (mx, fr) in errs.items():
+            per_form[fname]["max_rel"] = mx
+            per_form[fname]["fro_rel"] = fr
+            log(f"bench: {fname}: max_rel={mx:.3e} fro_rel={fr:.3e}")
+        cpu = cpu_baseline(csr, Bh, K)
+        cpu["sample"] = (f"full {args.workload} matrix ({csr.nnz} nnz), K={K}, "
+                         f"median of {cpu.pop('runs')} runs")
+        log(f"bench: CPU baseline {cpu['value']:.1f} GFLOP/s on {cpu['cores']} threads")
+
+    head = forms[0]
+    hr = per_form[head]
+    head_form = name_to_form[head]
+    ab = alg_bytes(csr.rows, csr.cols, csr.nnz, K, head_form)
+    achieved = ab / (hr["kernel_ms_mean"] * 1e-3) / 1e9
+    traffic = None
+    tfile = REPO / "profiles" / f"traffic_{args.workload}_k{K}_{head}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+    out = {
+        "metric": "SpMM GFLOP/s (2*nnz*K/t)",
+        "value": round(hr["gflops"], 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(hr["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if head_form != acc.FORM_FP32 else "f32",
+        "data": "synthetic (seeded generators, A and B resident in HBM)",
+        "config": {"workload": args.workload, "rows": csr.rows, "nnz": csr.nnz, "K": K,
+                   "form": head, "row_window": inf.row_window,
+                   "tiles": inf.num_tiles, "tile": hr["tile"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "spmm_kernel", "kernel_ms": round(hr["kernel_ms_mean"], 4),
+                     "alg_bytes_per_launch": ab},
+        "cpu_baseline": cpu,
+        "build_ms": round(build_ms, 1),
+        "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv)
+                      for kk, vv in v.items()} for k, v in per_form.items()},
+    }
+    if "fp32" in per_form and head != "fp32":
+        out["vs_fp32_form"] = round(hr["gflops"] / per_form["fp32"]["gflops"], 3)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,295 @@
+/*
+ * accspmm.h — C ABI of the MI355X (gfx950) METCF SpMM library `libaccspmm.so`.
+ *
+ * C = A * B with A sparse (CSR, converted on the GPU to the METCF bitmap-tile
+ * format) and B dense row-major N x K.  Every entry point returns an
+ * accspmm_status and never aborts or prints.  Pointers documented as "device"
+ * are HIP device pointers; `stream` is a hipStream_t passed as void* (NULL is
+ * the null stream).
+ *
+ * The reference (AI4SClab/AccSpMM) has no library or FFI layer; its seams are
+ * the `./mma` CLI, the format.hpp construction API and the TF32Compute
+ * launcher.  Each declaration below names the reference interface it replaces.
+ */
+#ifndef ACCSPMM_H_
+#define ACCSPMM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference prints and continues (include/utils.hpp:6-13)
+ * and leaves launches unchecked (include/tf32_comp.hpp:36,42). */
+typedef enum accspmm_status {
+  ACCSPMM_OK = 0,
+  ACCSPMM_ERR_INVALID_ARG = 1,    /* null pointer, K == 0, bad option value   */
+  ACCSPMM_ERR_NOT_CANONICAL = 2,  /* CSR row not strictly increasing in col   */
+  ACCSPMM_ERR_HIP = 3,            /* a HIP runtime call failed                */
+  ACCSPMM_ERR_OUT_OF_MEMORY = 4,  /* device allocation failed                 */
+  ACCSPMM_ERR_UNSUPPORTED = 5,    /* option outside the supported set         */
+  ACCSPMM_ERR_BUDGET = 6,         /* serving: HBM budget exhausted            */
+  ACCSPMM_ERR_NOT_FOUND = 7,      /* serving: unknown graph or request id     */
+  ACCSPMM_ERR_QUEUE_FULL = 8,     /* serving: request queue full              */
+  ACCSPMM_ERR_MTX = 9,            /* Matrix Market parse error                */
+  ACCSPMM_ERR_NOT_READY = 10,     /* serving: request still in flight         */
+  ACCSPMM_ERR_SHAPE = 11          /* operand shapes do not agree              */
+} accspmm_status;
+
+const char* accspmm_status_string(accspmm_status s);
+/* Library version string, e.g. "accspmm-mi355x 0.1 gfx950". */
+const char* accspmm_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Inputs                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* CSR view (include/format.hpp:96-142: row_ptr/col_idx/data, 32-bit). */
+typedef struct accspmm_csr_view {
+  uint32_t rows, cols, nnz;
+  const uint32_t* row_ptr; /* rows + 1 */
+  const uint32_t* col_idx; /* nnz      */
+  const float* val;        /* nnz      */
+} accspmm_csr_view;
+
+/* Build options.  Defaults are the reference's compile-time constants:
+ * ROW_WINDOW 8, COL_WINDOW 8 (include/common.hpp:46,48), TARGET_NUM 16
+ * (:55), IMBALANCE_THRESHOLD 8 (:62) and the driver's CSR2AdpBME(csr,3,1,2)
+ * cost model (src/mma_tf32.cu:222). */
+typedef struct accspmm_build_opts {
+  uint32_t row_window;          /* 8 or 16 (16 => 128-bit tiles)            */
+  uint32_t col_window;          /* must be 8                                */
+  uint32_t t_load, t_comp, t_write, target_num; /* AdpBME cost model        */
+  uint32_t imbalance_threshold; /* balanceStrategy threshold                */
+  uint32_t build_adpbme;        /* 1: also build AdpBME groups              */
+} accspmm_build_opts;
+
+void accspmm_build_opts_default(accspmm_build_opts* o);
+
+/* Opaque device-resident METCF(+AdpBME) graph.  Immutable after build, so it
+ * may be shared read-only across streams. */
+typedef struct accspmm_metcf accspmm_metcf;
+
+typedef struct accspmm_metcf_info {
+  uint32_t rows, cols, nnz;
+  uint32_t row_window;
+  uint32_t num_windows;   /* rowWindowOffset.size() - 1               */
+  uint32_t num_tiles;     /* tcOffset.size() - 1                      */
+  uint32_t num_groups;    /* AdpBME groupOffset.size() - 1 (0 if none) */
+  uint32_t ibd;           /* imbalance degree (balanceStrategy)       */
+  uint32_t balanced;      /* ibd > imbalance_threshold                */
+  uint32_t max_window_tiles;
+  uint64_t device_bytes;  /* device memory held by the handle         */
+} accspmm_metcf_info;
+
+/* GPU METCF builder.  Replaces METCFBit::convertFromCSR
+ * (include/format.hpp:215-274), AdpBME::CSR2AdpBME (:340-429) and
+ * balanceStrategy (include/utils.hpp:75-85).  `csr` holds DEVICE pointers to
+ * a canonical CSR (each row strictly increasing in column); a
+ * non-canonical row yields ACCSPMM_ERR_NOT_CANONICAL.  Blocks until the
+ * handle is complete.  The handle owns its device arrays. */
+accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
+                                   const accspmm_build_opts* opts,
+                                   accspmm_metcf** out, void* stream);
+accspmm_status accspmm_metcf_destroy(accspmm_metcf* h);
+accspmm_status accspmm_metcf_get_info(const accspmm_metcf* h,
+                                      accspmm_metcf_info* info);
+
+/* Byte-exact export of the format arrays to HOST buffers laid out as the
+ * public std::vector members of METCFBit (include/format.hpp:209-211,
+ * :175-176) and AdpBME (:333-336).  Any pointer may be NULL (skipped).
+ * Sizes: rowWindowOffset W+1, tcOffset T+1, tcLocalBit T*(row_window/8),
+ * sparseA2B 8T, data nnz, groupOffset G+1, rowIndices T. */
+accspmm_status accspmm_metcf_export(const accspmm_metcf* h,
+                                    uint32_t* rowWindowOffset,
+                                    uint32_t* tcOffset, uint64_t* tcLocalBit,
+                                    uint32_t* sparseA2B, float* data,
+                                    uint32_t* groupOffset,
+                                    uint32_t* rowIndices);
+
+/* ------------------------------------------------------------------------ */
+/* SpMM                                                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef enum accspmm_form {
+  ACCSPMM_FORM_FP32 = 0,       /* FP32 MFMA (v_mfma_f32_16x16x4_f32)           */
+  ACCSPMM_FORM_SPLIT_BF16 = 1, /* A,B split hi+lo BF16, 4 products, FP32 acc  */
+  ACCSPMM_FORM_BF16 = 2        /* A,B rounded to BF16, FP32 accumulate        */
+} accspmm_form;
+
+/* Kernel tile configuration (all zero => library default / tuned). */
+typedef struct accspmm_tile {
+  uint32_t feat_per_lane;  /* NB: features per lane per B row: 1,2,4,8; KC=16*NB */
+  uint32_t waves_per_block;/* 1..8                                           */
+  uint32_t split_tiles;    /* max tiles per work unit (0 = no splitting)      */
+  uint32_t xcd_slice;      /* 1: pin feature chunks to XCDs                   */
+} accspmm_tile;
+
+/* C[rows x K] = A * B[cols x K].  B (ldb >= K) and C (ldc >= K) are
+ * caller-owned DEVICE FP32 buffers; C is written exactly (rows x K, no
+ * padding rows).  Replaces TF32Compute::compute / adpBalanceCompute
+ * (include/tf32_comp.hpp:52-113) and the four kernels of
+ * include/mma_tf32.cuh.  The BF16 forms convert B into a workspace owned by
+ * the handle (allocated on first use for a given K, reused after), so the
+ * conversion is part of each call.  Asynchronous on `stream`. */
+accspmm_status accspmm_spmm(const accspmm_metcf* h, const float* B,
+                            uint32_t ldb, float* C, uint32_t ldc, uint32_t K,
+                            accspmm_form form, const accspmm_tile* tile,
+                            void* stream);
+
+/* The two halves of accspmm_spmm, for callers that time or overlap them:
+ * accspmm_spmm_prepare_b converts B into the handle's workspace for `form`
+ * (a no-op for FP32), accspmm_spmm_prepared runs the SpMM kernel on it.
+ * Both must be given the same B, K, form and tile. */
+accspmm_status accspmm_spmm_prepare_b(const accspmm_metcf* h, const float* B,
+                                      uint32_t ldb, uint32_t K,
+                                      accspmm_form form,
+                                      const accspmm_tile* tile, void* stream);
+accspmm_status accspmm_spmm_prepared(const accspmm_metcf* h, const float* B,
+                                     uint32_t ldb, float* C, uint32_t ldc,
+                                     uint32_t K, accspmm_form form,
+                                     const accspmm_tile* tile, void* stream);
+
+/* Chooses the tile configuration the library would use for (h, K, form):
+ * the autotuner's cached pick if one exists for this degree histogram and
+ * K, else the heuristic default. */
+accspmm_status accspmm_tile_select(const accspmm_metcf* h, uint32_t K,
+                                   accspmm_form form, accspmm_tile* out);
+
+/* Autotuner: times each candidate tile configuration on (h, B, C, K, form)
+ * on `stream`, stores the fastest under the handle's degree-histogram key
+ * and K (in memory, and in the JSON file `cache_path` if non-NULL), and
+ * returns it.  B and C are scratch operands of the right shape. */
+accspmm_status accspmm_autotune(const accspmm_metcf* h, const float* B,
+                                uint32_t ldb, float* C, uint32_t ldc,
+                                uint32_t K, accspmm_form form,
+                                const char* cache_path, accspmm_tile* best,
+                                float* best_ms, void* stream);
+/* Degree-histogram key of a handle (log2 buckets of tiles per window),
+ * written as a NUL-terminated string of at most `cap` bytes. */
+accspmm_status accspmm_tune_key(const accspmm_metcf* h, uint32_t K,
+                                accspmm_form form, char* buf, size_t cap);
+
+/* ------------------------------------------------------------------------ */
+/* Batched: many (graph, B, C, K) problems in one persistent launch.        */
+/* No reference counterpart (the reference runs one matrix per process).    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct accspmm_problem {
+  const accspmm_metcf* graph;
+  const float* B; uint32_t ldb;
+  float* C; uint32_t ldc;
+  uint32_t K;
+} accspmm_problem;
+
+typedef struct accspmm_batch accspmm_batch;
+
+/* Plans a batch (work-unit queue over all problems, one form); allocates its
+ * device task list and workspaces.  `problems` may be re-bound later with
+ * accspmm_batch_rebind as long as graphs and K stay the same. */
+accspmm_status accspmm_batch_create(const accspmm_problem* problems,
+                                    uint32_t count, accspmm_form form,
+                                    accspmm_batch** out, void* stream);
+accspmm_status accspmm_batch_rebind(accspmm_batch* b,
+                                    const accspmm_problem* problems,
+                                    uint32_t count, void* stream);
+/* One persistent launch over the whole queue (plus the B conversion pass
+ * for the BF16 forms).  Asynchronous on `stream`. */
+accspmm_status accspmm_batch_run(accspmm_batch* b, void* stream);
+/* Captures accspmm_batch_run into a hipGraph once and replays it. */
+accspmm_status accspmm_batch_run_graph(accspmm_batch* b, void* stream);
+accspmm_status accspmm_batch_destroy(accspmm_batch* b);
+accspmm_status accspmm_batch_num_tasks(const accspmm_batch* b,
+                                       uint64_t* tasks);
+
+/* ------------------------------------------------------------------------ */
+/* Host utilities (no GPU): Matrix Market ingest, CSR, generators.          */
+/* ------------------------------------------------------------------------ */
+
+/* Host CSR owned by the library (free with accspmm_host_csr_free). */
+typedef struct accspmm_host_csr {
+  uint32_t rows, cols, nnz;
+  uint32_t* row_ptr;
+  uint32_t* col_idx;
+  float* val;
+} accspmm_host_csr;
+
+void accspmm_host_csr_free(accspmm_host_csr* c);
+
+/* Reads a Matrix Market file with the semantics of read_from_mtx
+ * (include/mmio.hpp:171-210: banner tokens, symmetric/Hermitian expansion
+ * (r,c) then (c,r), pattern => 1.0) and builds the CSR by stable counting
+ * sort (include/format.hpp:110-142).  If `canonicalize` is non-zero each row
+ * is then sorted by column and duplicates are summed.  `mtx_code` receives
+ * the reference's read_from_mtx code (0, -1 open failure, -2 banner). */
+accspmm_status accspmm_read_mtx(const char* path, int canonicalize,
+                                accspmm_host_csr* out, int* mtx_code);
+
+/* 1 if every row is strictly increasing in column and all columns < cols. */
+int accspmm_csr_is_canonical(const accspmm_host_csr* c);
+
+/* Deterministic generators (xoshiro256** seeded by splitmix64). */
+/* Symmetric Chung-Lu power-law graph (Miller-Hagberg skip sampling), a self
+ * loop on every node, seeded random relabel, canonical rows.  value_mode:
+ * 0 = symmetric U(0,1] per undirected edge, 1 = 1/deg(row), 2 = U[-1,1]. */
+accspmm_status accspmm_gen_chung_lu(uint32_t n, double mean_degree,
+                                    double exponent, uint64_t seed,
+                                    int value_mode, accspmm_host_csr* out);
+/* 27-point 3-D stencil on an n^3 grid (id = x + n*y + n*n*z), values
+ * U[-1,1] in CSR order. */
+accspmm_status accspmm_gen_stencil27(uint32_t n, uint64_t seed,
+                                     accspmm_host_csr* out);
+/* Dense row-major rows x cols, U[-1,1]. */
+void accspmm_gen_dense(uint64_t seed, uint64_t count, float* out);
+
+/* ------------------------------------------------------------------------ */
+/* Serving: resident graphs, HBM-budget admission, continuous batching.     */
+/* No reference counterpart.                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct accspmm_server accspmm_server;
+
+typedef struct accspmm_server_opts {
+  uint64_t hbm_budget_bytes;  /* resident graphs + in-flight workspaces     */
+  uint32_t max_batch;         /* requests per persistent launch             */
+  uint32_t max_queue;         /* pending requests before QUEUE_FULL         */
+  uint32_t max_inflight;      /* launches in flight (streams)               */
+  uint32_t form;              /* accspmm_form                               */
+  uint32_t one_per_launch;    /* 1: baseline mode, one launch per request   */
+} accspmm_server_opts;
+
+void accspmm_server_opts_default(accspmm_server_opts* o);
+accspmm_status accspmm_server_create(const accspmm_server_opts* opts,
+                                     accspmm_server** out);
+/* Registers a resident graph built from a device CSR; charges its bytes
+ * against the budget (ACCSPMM_ERR_BUDGET if it does not fit). */
+accspmm_status accspmm_server_register(accspmm_server* s,
+                                       const accspmm_csr_view* csr,
+                                       uint32_t* graph_id);
+accspmm_status accspmm_server_unregister(accspmm_server* s, uint32_t graph_id);
+/* Submits C = A_g * B (device buffers, caller-owned until completion). */
+accspmm_status accspmm_server_submit(accspmm_server* s, uint32_t graph_id,
+                                     const float* B, uint32_t ldb, float* C,
+                                     uint32_t ldc, uint32_t K,
+                                     uint64_t* request_id);
+/* Non-blocking: ACCSPMM_OK when done (latency in seconds written, from
+ * submit to completion), ACCSPMM_ERR_NOT_READY otherwise. */
+accspmm_status accspmm_server_poll(accspmm_server* s, uint64_t request_id,
+                                   double* latency_s);
+/* Blocks until every submitted request has completed. */
+accspmm_status accspmm_server_drain(accspmm_server* s);
+typedef struct accspmm_server_stats {
+  uint64_t submitted, completed, launches, rejected_budget;
+  uint64_t resident_bytes, peak_inflight_bytes;
+  double busy_s;
+} accspmm_server_stats;
+accspmm_status accspmm_server_get_stats(accspmm_server* s,
+                                        accspmm_server_stats* st);
+accspmm_status accspmm_server_destroy(accspmm_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ACCSPMM_H_ */

@@ -1,0 +1,141 @@
+"""GPU SpMM per form against the host FP64 oracle.
+
+Tolerances (componentwise max-rel against |A||B|, FP64 oracle):
+  FP32-MFMA: 1e-5;  single-BF16: 1e-2;
+  split-BF16: 1e-5 on the high-degree matrices of the precision study (the
+  27-point stencil; the 232,965-node graph in test_gpu_large); on arbitrary
+  graphs its proven bound 2^-15 (+1e-6 accumulation): a two-term BF16 split
+  leaves at most 2^-16 |x| of each operand, so one product can be off by
+  2*2^-16 relative, which rows with one or two nonzeros expose.
+"""
+import numpy as np
+import pytest
+
+import accspmm_amd as acc
+from helpers import random_csr
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {acc.FORM_FP32: 1e-5, acc.FORM_SPLIT_BF16: 2.0 ** -15 + 1e-6, acc.FORM_BF16: 1e-2}
+TOL_HIGH_DEGREE = {acc.FORM_FP32: 1e-5, acc.FORM_SPLIT_BF16: 1e-5, acc.FORM_BF16: 1e-2}
+FORMS = [acc.FORM_FP32, acc.FORM_SPLIT_BF16, acc.FORM_BF16]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def run_case(csr: acc.CSR, K: int, form: int, tile=None, rw: int = 8, seed: int = 0,
+             graph=None, B=None):
+    torch = _torch()
+    g = graph or acc.DeviceGraph(csr, acc.default_build_opts(rw))
+    if B is None:
+        B = acc.gen_dense(seed, csr.cols, K)
+    Bd = torch.from_numpy(B).cuda()
+    Cd = torch.full((csr.rows, K), float("nan"), device="cuda")
+    g.spmm(Bd, Cd, form, tile)
+    torch.cuda.synchronize()
+    Cg = Cd.cpu().numpy()
+    C64 = O.spmm_f64(csr.row_ptr, csr.col_idx, csr.data, B)
+    ab = O.spmm_abs_f64(csr.row_ptr, csr.col_idx, csr.data, B)
+    mx, fr = O.errors(Cg, C64, ab)
+    return mx, fr, Cg
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+@pytest.mark.parametrize("K", [1, 16, 17, 32, 64, 100, 128, 256])
+def test_forms_random(form, K):
+    rng = np.random.default_rng(K)
+    rows, cols = 1000, 1200
+    rp, ci, va = random_csr(rng, rows, cols, 0.01, empty_rows=0.1, heavy_rows=3)
+    csr = acc.CSR(rows, cols, rp, ci, va)
+    mx, fr, _ = run_case(csr, K, form, seed=K)
+    assert mx <= TOL[form], (mx, fr)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+@pytest.mark.parametrize("nb", [1, 2, 4, 8])
+@pytest.mark.parametrize("split", [0, 4])
+@pytest.mark.parametrize("xcd", [0, 1])
+def test_tile_variants(form, nb, split, xcd):
+    rng = np.random.default_rng(nb * 10 + split)
+    rows, cols = 777, 900
+    rp, ci, va = random_csr(rng, rows, cols, 0.02, empty_rows=0.05, heavy_rows=4)
+    csr = acc.CSR(rows, cols, rp, ci, va)
+    tile = acc.Tile(nb, 4, split, xcd)
+    mx, fr, _ = run_case(csr, 128, form, tile=tile, seed=3)
+    assert mx <= TOL[form], (mx, fr)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+def test_row_window_16(form):
+    rng = np.random.default_rng(16)
+    rp, ci, va = random_csr(rng, 501, 640, 0.02, empty_rows=0.1, heavy_rows=2)
+    csr = acc.CSR(501, 640, rp, ci, va)
+    for tile in (None, acc.Tile(4, 2, 3, 0)):
+        mx, fr, _ = run_case(csr, 64, form, tile=tile, rw=16, seed=5)
+        assert mx <= TOL[form], (mx, fr)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+def test_chung_lu_c0(form):
+    # 16,384-node Chung-Lu (exponent 2.1, mean degree 16), values U(0,1], K=32
+    csr = acc.gen_chung_lu(16384, 16.0, 2.1, 1, 0)
+    mx, fr, _ = run_case(csr, 32, form, seed=2)
+    assert mx <= TOL[form], (mx, fr)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+def test_stencil(form):
+    csr = acc.gen_stencil27(24, 13)
+    mx, fr, _ = run_case(csr, 128, form, seed=14)
+    assert mx <= TOL_HIGH_DEGREE[form], (mx, fr)
+
+
+def test_rectangular_and_strided():
+    torch = _torch()
+    rng = np.random.default_rng(9)
+    rp, ci, va = random_csr(rng, 300, 1700, 0.01)
+    csr = acc.CSR(300, 1700, rp, ci, va)
+    g = acc.DeviceGraph(csr)
+    B = acc.gen_dense(4, 1700, 48)
+    Bp = np.zeros((1700, 52), np.float32)
+    Bp[:, :48] = B
+    Bd = torch.from_numpy(Bp).cuda()[:, :48]
+    Cd = torch.full((300, 56), 7.0, device="cuda")
+    g.spmm(Bd, Cd[:, :48], acc.FORM_FP32)
+    torch.cuda.synchronize()
+    out = Cd.cpu().numpy()
+    assert np.all(out[:, 48:] == 7.0)  # padding columns untouched
+    C64 = O.spmm_f64(rp, ci, va, B)
+    ab = O.spmm_abs_f64(rp, ci, va, B)
+    assert O.errors(out[:, :48], C64, ab)[0] <= 1e-5
+
+
+def test_deterministic_and_split_consistent():
+    torch = _torch()
+    csr = acc.gen_chung_lu(8000, 40.0, 2.0, 21, 0)
+    g = acc.DeviceGraph(csr)
+    B = torch.from_numpy(acc.gen_dense(5, csr.cols, 64)).cuda()
+    outs = []
+    for _ in range(3):
+        outs.append(g.spmm(B, None, acc.FORM_SPLIT_BF16, acc.Tile(4, 4, 8, 0)).cpu())
+    assert all(torch.equal(outs[0], o) for o in outs[1:])
+
+
+def test_invalid_arguments():
+    torch = _torch()
+    rng = np.random.default_rng(1)
+    rp, ci, va = random_csr(rng, 50, 50, 0.1)
+    g = acc.DeviceGraph(acc.CSR(50, 50, rp, ci, va))
+    B = torch.zeros((50, 8), device="cuda")
+    Cm = torch.zeros((50, 8), device="cuda")
+    L = acc.lib()
+    assert L.accspmm_spmm(g.handle, B.data_ptr(), 8, Cm.data_ptr(), 8, 0, 0, None,
+                          None) == acc.Status.ERR_INVALID_ARG
+    assert L.accspmm_spmm(g.handle, B.data_ptr(), 4, Cm.data_ptr(), 8, 8, 0, None,
+                          None) == acc.Status.ERR_SHAPE
+    assert L.accspmm_spmm(g.handle, B.data_ptr(), 8, Cm.data_ptr(), 8, 8, 7, None,
+                          None) == acc.Status.ERR_INVALID_ARG
