@@ -28,7 +28,7 @@ import numpy as np
 __all__ = [
     "lib", "Status", "AccSpMMError", "FORM_FP32", "FORM_SPLIT_BF16", "FORM_BF16",
     "BuildOpts", "Tile", "CSR", "read_mtx", "METCFBit", "AdpBME",
-    "balanceStrategy", "DeviceGraph", "Batch", "gen_chung_lu", "gen_stencil27",
+    "balanceStrategy", "DeviceGraph", "Batch", "Server", "gen_chung_lu", "gen_stencil27",
     "gen_dense", "MetcfInfo",
 ]
 
@@ -122,6 +122,8 @@ _SIGS = {
     "accspmm_build_opts_default": (None, [C.POINTER(BuildOpts)]),
     "accspmm_metcf_build": (C.c_int, [C.POINTER(CsrView), C.POINTER(BuildOpts),
                                       C.POINTER(C.c_void_p), C.c_void_p]),
+    "accspmm_metcf_build_host": (C.c_int, [C.POINTER(CsrView), C.POINTER(BuildOpts),
+                                           C.POINTER(C.c_void_p)]),
     "accspmm_metcf_destroy": (C.c_int, [C.c_void_p]),
     "accspmm_metcf_get_info": (C.c_int, [C.c_void_p, C.POINTER(MetcfInfo)]),
     "accspmm_metcf_export": (C.c_int, [C.c_void_p] + [C.c_void_p] * 7),
@@ -163,6 +165,7 @@ _SIGS = {
                                         C.POINTER(C.c_uint64)]),
     "accspmm_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double)]),
     "accspmm_server_drain": (C.c_int, [C.c_void_p]),
+    "accspmm_server_pause": (C.c_int, [C.c_void_p, C.c_int]),
     "accspmm_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "accspmm_server_destroy": (C.c_int, [C.c_void_p]),
 }
@@ -506,6 +509,82 @@ class Batch:
     def close(self):
         if getattr(self, "handle", None):
             lib().accspmm_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- serving
+class Server:
+    """Continuous-batching SpMM server over resident METCF graphs."""
+
+    def __init__(self, hbm_budget_bytes: int = 64 << 30, max_batch: int = 64,
+                 max_queue: int = 1 << 16, max_inflight: int = 2,
+                 form: int = FORM_SPLIT_BF16, one_per_launch: bool = False):
+        o = ServerOpts()
+        lib().accspmm_server_opts_default(C.byref(o))
+        o.hbm_budget_bytes = hbm_budget_bytes
+        o.max_batch = max_batch
+        o.max_queue = max_queue
+        o.max_inflight = max_inflight
+        o.form = form
+        o.one_per_launch = int(one_per_launch)
+        h = C.c_void_p()
+        _check(lib().accspmm_server_create(C.byref(o), C.byref(h)), "server_create")
+        self.handle = h
+        self._keep = {}
+
+    def register(self, csr: CSR) -> int:
+        import torch
+        rp, ci, va = csr.to_device()
+        view = CsrView(csr.rows, csr.cols, csr.nnz, rp.data_ptr(), ci.data_ptr(),
+                       va.data_ptr())
+        gid = C.c_uint32()
+        torch.cuda.synchronize()
+        _check(lib().accspmm_server_register(self.handle, C.byref(view), C.byref(gid)),
+               "server_register")
+        return gid.value
+
+    def unregister(self, gid: int) -> None:
+        _check(lib().accspmm_server_unregister(self.handle, gid), "server_unregister")
+
+    def submit(self, gid: int, B, C_out) -> int:
+        rid = C.c_uint64()
+        st = lib().accspmm_server_submit(self.handle, gid, B.data_ptr(), B.stride(0),
+                                         C_out.data_ptr(), C_out.stride(0), B.shape[1],
+                                         C.byref(rid))
+        _check(st, "server_submit")
+        self._keep[rid.value] = (B, C_out)
+        return rid.value
+
+    def poll(self, rid: int):
+        """Latency in seconds if done, else None."""
+        lat = C.c_double()
+        st = lib().accspmm_server_poll(self.handle, rid, C.byref(lat))
+        if st == Status.ERR_NOT_READY:
+            return None
+        self._keep.pop(rid, None)
+        _check(st, "server_poll")
+        return lat.value
+
+    def drain(self) -> None:
+        _check(lib().accspmm_server_drain(self.handle), "server_drain")
+
+    def pause(self, paused: bool = True) -> None:
+        _check(lib().accspmm_server_pause(self.handle, int(paused)), "server_pause")
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib().accspmm_server_get_stats(self.handle, C.byref(st)), "server_stats")
+        return st
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().accspmm_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -31,8 +31,8 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]
 
-HIP_SOURCES = ["scan.hip", "builder.hip", "spmm.hip", "batch.hip"]
-HOST_HIP_SOURCES = ["tune.cpp", "server.cpp"]  # host code using the HIP runtime
+HIP_SOURCES = ["scan.hip", "builder.hip", "spmm.hip", "batch.hip", "server.hip"]
+HOST_HIP_SOURCES = ["tune.cpp"]  # host code using the HIP runtime
 HOST_SOURCES = ["host.cpp"]
 
 

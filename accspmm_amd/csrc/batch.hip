@@ -1,0 +1,415 @@
+// batch.hip — many (graph, B, C, K) SpMM problems in one persistent launch.
+//
+// No reference counterpart (the reference runs one matrix per process,
+// src/mma_tf32.cu:185-243).  Every problem keeps the work units its
+// standalone call would use (same tile, same split threshold), so its C is
+// bit-identical alone and inside a batch.  The task space is implicit: task
+// t belongs to problem p with prefix[p] <= t < prefix[p+1] and is unit
+// (t - prefix[p]) / n_kc, feature chunk (t - prefix[p]) % n_kc of it.
+// A run is four kernels on one stream: reset of the queue head and of the
+// split-window counters, one conversion pass over every problem's B (BF16
+// forms), and the persistent kernel whose waves pop tasks from the atomic
+// head until it passes the last task (every wave reaches that test, so the
+// grid always drains).  Problems are ordered largest first.
+//
+// BatchLauncher is shared with the serving loop: planning is host
+// arithmetic over cached per-graph state, uploads go through pinned staging
+// with hipMemcpyAsync, and device buffers only grow, so steady-state
+// launches neither allocate nor synchronize.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "batch_launcher.hpp"
+#include "spmm_device.hpp"
+
+namespace accspmm {
+accspmm_status spmm_fit_tile(const accspmm_metcf* h, const float* B,
+                             uint32_t ldb, float* C, uint32_t ldc, uint32_t K,
+                             int form, const accspmm_tile* tile,
+                             accspmm_tile* out);
+namespace {
+using namespace dev;
+
+struct ConvDesc {
+  const float* src;
+  void* dst;
+  uint32_t ldb, K, rows, ldx;
+};
+
+constexpr uint32_t kConvRows = 16;
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// last index i in [0, n) with a[i] <= key (a[0] == 0 <= key)
+__device__ inline uint32_t last_le(const uint32_t* __restrict__ a, uint32_t n,
+                                   uint32_t key) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) / 2;
+    if (a[mid] <= key) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int FORM>
+__global__ __launch_bounds__(256) void k_batch_convert(
+    const ConvDesc* __restrict__ d, uint32_t nconv,
+    const uint32_t* __restrict__ prefix) {
+  const uint32_t p = last_le(prefix, nconv, blockIdx.x);
+  const ConvDesc c = d[p];
+  const uint32_t r0 = (blockIdx.x - prefix[p]) * kConvRows;
+  const uint32_t r1 = min(r0 + kConvRows, c.rows);
+  for (uint32_t r = r0; r < r1; ++r)
+    for (uint32_t k = threadIdx.x; k < c.ldx; k += blockDim.x) {
+      float x = k < c.K ? c.src[(uint64_t)r * c.ldb + k] : 0.0f;
+      uint32_t hp = pack_bf16(x, 0.f) & 0xFFFFu;
+      if (FORM == 1) {
+        uint32_t lp = pack_bf16(x - bf16_lo_to_f32(hp), 0.f) & 0xFFFFu;
+        static_cast<uint32_t*>(c.dst)[(uint64_t)r * c.ldx + k] = hp | (lp << 16);
+      } else {
+        static_cast<uint16_t*>(c.dst)[(uint64_t)r * c.ldx + k] = (uint16_t)hp;
+      }
+    }
+}
+
+template <int FORM, int RW>
+__global__ __launch_bounds__(256) void k_batch_spmm(
+    const SpmmArgs* __restrict__ descs, const uint32_t* __restrict__ prefix,
+    uint32_t nprob, uint32_t* __restrict__ head) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t total = prefix[nprob];
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(head, 1u);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+    if (t >= total) return;
+    const uint32_t p = last_le(prefix, nprob, t);
+    const SpmmArgs g = descs[p];
+    const uint32_t local = t - prefix[p];
+    const WorkUnit u = g.units[local / g.n_kc];
+    const uint32_t kc = local % g.n_kc;
+    // one row-window size per kernel keeps the register budget of the
+    // widest variant (NB=4) without spills
+    switch (g.nb) {
+      case 1: run_unit<FORM, 1, RW>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
+      case 2: run_unit<FORM, 2, RW>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
+      case 4: run_unit<FORM, 4, RW>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
+      default: break;
+    }
+  }
+}
+
+inline uint64_t align256(uint64_t x) { return (x + 255) / 256 * 256; }
+
+}  // namespace
+
+// Per-problem plan: tile, schedule and workspace sizes.
+accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
+                            ProblemPlan* out) {
+  if (!p.graph || !p.B || !p.C || p.K == 0) return ACCSPMM_ERR_INVALID_ARG;
+  if (p.ldb < p.K || p.ldc < p.K) return ACCSPMM_ERR_SHAPE;
+  const accspmm_metcf* h = p.graph;
+  accspmm_tile t;
+  // the same tile (after alignment narrowing) as the single-graph call
+  ACC_TRY(spmm_fit_tile(h, p.B, p.ldb, p.C, p.ldc, p.K, form, nullptr, &t));
+  // the persistent kernel carries NB <= 4 (register budget)
+  while (t.feat_per_lane > 4) t.feat_per_lane /= 2;
+  const uint32_t KC = 16 * t.feat_per_lane;
+  out->nb = t.feat_per_lane;
+  out->n_kc = (p.K + KC - 1) / KC;
+  out->kc_padded = out->n_kc * KC;
+  {
+    std::lock_guard<std::mutex> lock(h->mu);
+    ACC_TRY(get_schedule(h, t.split_tiles, s, &out->sched));
+  }
+  out->tasks = (uint64_t)out->sched->num_units * out->n_kc;
+  out->bconv_bytes =
+      form == 0 ? 0
+                : align256((uint64_t)h->cols * out->kc_padded * (form == 1 ? 4 : 2));
+  out->partial_bytes =
+      align256((uint64_t)out->sched->num_pieces * h->rw * out->kc_padded * 4);
+  out->counter_words = (uint64_t)out->sched->num_split_windows * out->n_kc;
+  return ACCSPMM_OK;
+}
+
+uint64_t ProblemPlan::workspace_bytes() const {
+  return bconv_bytes + partial_bytes + align256(counter_words * 4);
+}
+
+BatchLauncher::~BatchLauncher() {
+  if (pinned) (void)hipHostFree(pinned);
+}
+
+accspmm_status BatchLauncher::ensure_pinned(size_t bytes) {
+  if (pinned && pinned_bytes >= bytes) return ACCSPMM_OK;
+  if (pinned) (void)hipHostFree(pinned);
+  pinned = nullptr;
+  size_t n = std::max<size_t>(bytes, 64 << 10);
+  ACC_HIP(hipHostMalloc(&pinned, n, hipHostMallocDefault));
+  pinned_bytes = n;
+  return ACCSPMM_OK;
+}
+
+accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
+                                      const ProblemPlan* plans, hipStream_t s) {
+  nprob = 0;
+  total_tasks = 0;
+  if (n == 0) return ACCSPMM_OK;
+  // one persistent kernel serves one row-window size
+  const uint32_t rw0 = P[0].graph->rw;
+  for (uint32_t i = 1; i < n; ++i)
+    if (P[i].graph->rw != rw0) return ACCSPMM_ERR_UNSUPPORTED;
+  rw = rw0;
+  // largest problems first, so heavy work starts early
+  std::vector<uint32_t> order(n);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return P[a].graph->T * (uint64_t)P[a].K > P[b].graph->T * (uint64_t)P[b].K;
+  });
+  // workspace layout: [partials + converted B per problem][counters]
+  uint64_t ws = 0, cnt_words = 0, tot = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    ws += plans[i].bconv_bytes + plans[i].partial_bytes;
+    cnt_words += plans[i].counter_words;
+    tot += plans[i].tasks;
+  }
+  if (tot > 0xFFFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
+  const uint64_t cnt_off = ws;
+  ws += align256(cnt_words * 4) + 256;
+  // host staging: descs | prefix (n+1) | conv descs | conv prefix (n+1)
+  const size_t d_bytes = sizeof(dev::SpmmArgs) * n;
+  const size_t p_bytes = 4 * ((size_t)n + 1);
+  const size_t c_bytes = sizeof(ConvDesc) * n;
+  const size_t off_p = align256(d_bytes), off_c = off_p + align256(p_bytes),
+               off_cp = off_c + align256(c_bytes);
+  const size_t stage = off_cp + align256(p_bytes);
+  // the staging buffer is reused: wait until its previous upload consumed it
+  if (upload_event_valid) ACC_HIP(hipEventSynchronize(upload_event));
+  if (!upload_event)
+    ACC_HIP(hipEventCreateWithFlags(&upload_event, hipEventDisableTiming));
+  ACC_TRY(ensure_pinned(stage));
+  ACC_TRY(dev_meta.reserve(stage));
+  ACC_TRY(arena.reserve(ws));
+  ACC_TRY(head.reserve(256));
+  char* hb = static_cast<char*>(pinned);
+  auto* D = reinterpret_cast<dev::SpmmArgs*>(hb);
+  auto* PF = reinterpret_cast<uint32_t*>(hb + off_p);
+  auto* CV = reinterpret_cast<ConvDesc*>(hb + off_c);
+  auto* CP = reinterpret_cast<uint32_t*>(hb + off_cp);
+  uint64_t bc = 0, cw = 0;
+  uint32_t run = 0, cblocks = 0, nconv = 0;
+  char* A = arena.as<char>();
+  for (uint32_t j = 0; j < n; ++j) {
+    const uint32_t i = order[j];
+    const accspmm_problem& p = P[i];
+    const ProblemPlan& q = plans[i];
+    const accspmm_metcf* h = p.graph;
+    dev::SpmmArgs& g = D[j];
+    std::memset(&g, 0, sizeof(g));
+    g.tco = h->tco.as<uint32_t>();
+    g.bits = h->bits.as<unsigned long long>();
+    g.a2b = h->a2b.as<uint32_t>();
+    g.data = h->data.as<float>();
+    g.units = q.sched->units.as<WorkUnit>();
+    g.piece_info = q.sched->piece_first.as<uint32_t>();
+    g.C = p.C;
+    g.ldc = p.ldc;
+    g.rows = h->rows;
+    g.K = p.K;
+    g.num_units = q.sched->num_units;
+    g.n_kc = q.n_kc;
+    g.kc_padded = q.kc_padded;
+    g.rw = h->rw;
+    g.nb = q.nb;
+    g.partials = reinterpret_cast<float*>(A + bc);
+    bc += q.partial_bytes;
+    g.counters = reinterpret_cast<uint32_t*>(A + cnt_off) + cw;
+    cw += q.counter_words;
+    if (form == 0) {
+      g.B = p.B;
+      g.ldb = p.ldb;
+    } else {
+      void* dst = A + bc;
+      bc += q.bconv_bytes;
+      g.B = dst;
+      g.ldb = q.kc_padded;
+      CV[nconv] = ConvDesc{p.B, dst, p.ldb, p.K, h->cols, q.kc_padded};
+      CP[nconv] = cblocks;
+      cblocks += (h->cols + kConvRows - 1) / kConvRows;
+      ++nconv;
+    }
+    PF[j] = run;
+    run += (uint32_t)q.tasks;
+  }
+  PF[n] = run;
+  CP[nconv] = cblocks;
+  ACC_HIP(hipMemcpyAsync(dev_meta.p, pinned, stage, hipMemcpyHostToDevice, s));
+  ACC_HIP(hipEventRecord(upload_event, s));
+  upload_event_valid = true;
+  descs = reinterpret_cast<dev::SpmmArgs*>(dev_meta.as<char>());
+  prefix = reinterpret_cast<uint32_t*>(dev_meta.as<char>() + off_p);
+  conv = dev_meta.as<char>() + off_c;
+  conv_prefix = reinterpret_cast<uint32_t*>(dev_meta.as<char>() + off_cp);
+  counters = reinterpret_cast<uint32_t*>(A + cnt_off);
+  counter_words = cw;
+  nprob = n;
+  num_conv = nconv;
+  conv_blocks = cblocks;
+  total_tasks = run;
+  int dv = 0, cus = 0;
+  ACC_HIP(hipGetDevice(&dv));
+  ACC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dv));
+  const uint64_t want = (total_tasks + 3) / 4;
+  grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)cus * 4));
+  return ACCSPMM_OK;
+}
+
+accspmm_status BatchLauncher::launch(hipStream_t s) const {
+  if (nprob == 0) return ACCSPMM_OK;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, head.as<uint32_t>(),
+                     (uint64_t)1, 0u);
+  ACC_HIP(hipGetLastError());
+  if (counter_words) {
+    uint32_t blocks = (uint32_t)std::min<uint64_t>((counter_words + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, counters,
+                       counter_words, 0u);
+    ACC_HIP(hipGetLastError());
+  }
+  if (form != 0 && conv_blocks) {
+    if (form == 1)
+      hipLaunchKernelGGL(k_batch_convert<1>, dim3(conv_blocks), dim3(256), 0, s,
+                         static_cast<const ConvDesc*>(conv), num_conv, conv_prefix);
+    else
+      hipLaunchKernelGGL(k_batch_convert<2>, dim3(conv_blocks), dim3(256), 0, s,
+                         static_cast<const ConvDesc*>(conv), num_conv, conv_prefix);
+    ACC_HIP(hipGetLastError());
+  }
+  if (total_tasks == 0) return ACCSPMM_OK;
+  auto* H = head.as<uint32_t>();
+  const dim3 g(grid), blk(256);
+  if (rw == 8) {
+    if (form == 0) hipLaunchKernelGGL((k_batch_spmm<0, 8>), g, blk, 0, s, descs, prefix, nprob, H);
+    else if (form == 1) hipLaunchKernelGGL((k_batch_spmm<1, 8>), g, blk, 0, s, descs, prefix, nprob, H);
+    else hipLaunchKernelGGL((k_batch_spmm<2, 8>), g, blk, 0, s, descs, prefix, nprob, H);
+  } else {
+    if (form == 0) hipLaunchKernelGGL((k_batch_spmm<0, 16>), g, blk, 0, s, descs, prefix, nprob, H);
+    else if (form == 1) hipLaunchKernelGGL((k_batch_spmm<1, 16>), g, blk, 0, s, descs, prefix, nprob, H);
+    else hipLaunchKernelGGL((k_batch_spmm<2, 16>), g, blk, 0, s, descs, prefix, nprob, H);
+  }
+  ACC_HIP(hipGetLastError());
+  return ACCSPMM_OK;
+}
+
+}  // namespace accspmm
+
+using namespace accspmm;
+
+struct accspmm_batch {
+  BatchLauncher L;
+  std::vector<accspmm_problem> probs;
+  std::vector<ProblemPlan> plans;
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~accspmm_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    if (L.upload_event) (void)hipEventDestroy(L.upload_event);
+  }
+};
+
+extern "C" accspmm_status accspmm_batch_create(const accspmm_problem* problems,
+                                               uint32_t count, accspmm_form form,
+                                               accspmm_batch** out, void* stream) {
+  if (!out || (count && !problems)) return ACCSPMM_ERR_INVALID_ARG;
+  if ((int)form < 0 || (int)form > 2) return ACCSPMM_ERR_INVALID_ARG;
+  *out = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  auto* b = new (std::nothrow) accspmm_batch();
+  if (!b) return ACCSPMM_ERR_OUT_OF_MEMORY;
+  b->L.form = (int)form;
+  b->probs.assign(problems, problems + count);
+  b->plans.resize(count);
+  accspmm_status st = ACCSPMM_OK;
+  for (uint32_t i = 0; i < count && st == ACCSPMM_OK; ++i)
+    st = problem_plan(problems[i], (int)form, s, &b->plans[i]);
+  if (st == ACCSPMM_OK) st = b->L.prepare(problems, count, b->plans.data(), s);
+  if (st == ACCSPMM_OK && hipStreamSynchronize(s) != hipSuccess) st = ACCSPMM_ERR_HIP;
+  if (st != ACCSPMM_OK) {
+    (void)hipStreamSynchronize(s);
+    delete b;
+    return st;
+  }
+  *out = b;
+  return ACCSPMM_OK;
+}
+
+extern "C" accspmm_status accspmm_batch_rebind(accspmm_batch* b,
+                                               const accspmm_problem* problems,
+                                               uint32_t count, void* stream) {
+  if (!b || (count && !problems)) return ACCSPMM_ERR_INVALID_ARG;
+  if (count != b->probs.size()) return ACCSPMM_ERR_SHAPE;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<ProblemPlan> plans(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    if (problems[i].graph != b->probs[i].graph || problems[i].K != b->probs[i].K)
+      return ACCSPMM_ERR_SHAPE;
+    ACC_TRY(problem_plan(problems[i], b->L.form, s, &plans[i]));
+    // the new operands must keep the planned tile (captured graphs replay it)
+    if (plans[i].nb != b->plans[i].nb) return ACCSPMM_ERR_UNSUPPORTED;
+  }
+  ACC_HIP(hipStreamSynchronize(s));
+  b->probs.assign(problems, problems + count);
+  b->plans = plans;
+  // same sizes => no buffer is reallocated, so a captured graph stays valid
+  ACC_TRY(b->L.prepare(problems, count, b->plans.data(), s));
+  ACC_HIP(hipStreamSynchronize(s));
+  return ACCSPMM_OK;
+}
+
+extern "C" accspmm_status accspmm_batch_run(accspmm_batch* b, void* stream) {
+  if (!b) return ACCSPMM_ERR_INVALID_ARG;
+  return b->L.launch((hipStream_t)stream);
+}
+
+extern "C" accspmm_status accspmm_batch_run_graph(accspmm_batch* b, void* stream) {
+  if (!b) return ACCSPMM_ERR_INVALID_ARG;
+  if (!b->exec) {
+    if (!b->cap_stream)
+      ACC_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    ACC_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    accspmm_status st = b->L.launch(b->cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (st != ACCSPMM_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (e != hipSuccess) return ACCSPMM_ERR_HIP;
+    b->graph = g;
+    ACC_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  ACC_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return ACCSPMM_OK;
+}
+
+extern "C" accspmm_status accspmm_batch_destroy(accspmm_batch* b) {
+  if (!b) return ACCSPMM_ERR_INVALID_ARG;
+  (void)hipDeviceSynchronize();
+  delete b;
+  return ACCSPMM_OK;
+}
+
+extern "C" accspmm_status accspmm_batch_num_tasks(const accspmm_batch* b,
+                                                  uint64_t* tasks) {
+  if (!b || !tasks) return ACCSPMM_ERR_INVALID_ARG;
+  *tasks = b->L.total_tasks;
+  return ACCSPMM_OK;
+}

@@ -1,0 +1,57 @@
+// batch_launcher.hpp — host side of the persistent batched SpMM launch,
+// shared by the batch API (batch.hip) and the serving loop (server.hip).
+#pragma once
+#include "internal.hpp"
+
+namespace accspmm {
+namespace dev {
+struct SpmmArgs;
+}
+
+// What one problem needs inside a batch: its tile (the one its standalone
+// call would use), its work-unit schedule and its workspace sizes.
+struct ProblemPlan {
+  uint32_t nb = 1, n_kc = 1, kc_padded = 16;
+  const Schedule* sched = nullptr;
+  uint64_t tasks = 0;          // units x feature chunks
+  uint64_t bconv_bytes = 0;    // converted B (BF16 forms)
+  uint64_t partial_bytes = 0;  // split-window partial sums
+  uint64_t counter_words = 0;  // split-window arrival counters
+  uint64_t workspace_bytes() const;
+};
+
+accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
+                            ProblemPlan* out);
+
+// Device buffers and pinned staging for one stream's batched launches.
+// Buffers only grow; prepare() uploads asynchronously on the stream and
+// launch() enqueues the kernels, so steady-state use neither allocates nor
+// synchronizes.  prepare() must not be called while a previous launch()
+// from the same launcher may still be running.
+struct BatchLauncher {
+  int form = 0;
+  uint32_t rw = 8;
+  DevBuf arena, dev_meta, head;
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  hipEvent_t upload_event = nullptr;
+  bool upload_event_valid = false;
+  dev::SpmmArgs* descs = nullptr;
+  uint32_t* prefix = nullptr;
+  void* conv = nullptr;
+  uint32_t* conv_prefix = nullptr;
+  uint32_t* counters = nullptr;
+  uint64_t counter_words = 0;
+  uint64_t total_tasks = 0;
+  uint32_t nprob = 0, num_conv = 0, conv_blocks = 0, grid = 0;
+  BatchLauncher() = default;
+  BatchLauncher(const BatchLauncher&) = delete;
+  BatchLauncher& operator=(const BatchLauncher&) = delete;
+  ~BatchLauncher();
+  accspmm_status ensure_pinned(size_t bytes);
+  accspmm_status prepare(const accspmm_problem* P, uint32_t n,
+                         const ProblemPlan* plans, hipStream_t s);
+  accspmm_status launch(hipStream_t s) const;
+};
+
+}  // namespace accspmm

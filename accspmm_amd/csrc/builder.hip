@@ -21,6 +21,7 @@
 // tile equals bit order, so step 8 reproduces its `data` exactly.
 #include <algorithm>
 #include <cstring>
+#include <string>
 
 #include "internal.hpp"
 
@@ -405,6 +406,24 @@ accspmm_status build_impl(const accspmm_csr_view* csr,
   ACC_HIP(hipStreamSynchronize(s));
   const uint32_t T = h->h_rwo[W];
   h->T = T;
+  {
+    // tuning key: log2 buckets of tiles per window as 10% steps, plus the
+    // log2 of the window count
+    uint32_t buckets[24] = {0};
+    for (uint32_t w = 0; w < W; ++w) {
+      uint32_t t = h->h_rwo[w + 1] - h->h_rwo[w], b = 0;
+      while (b < 23 && (1u << b) <= t) ++b;
+      buckets[b]++;
+    }
+    uint32_t lw = 0;
+    while ((2ull << lw) <= W) ++lw;
+    std::string key = "rw" + std::to_string(rw) + "-w" + std::to_string(lw) + "-h";
+    for (int b = 0; b < 24; ++b) {
+      uint32_t pct = W ? (uint32_t)((1000ull * buckets[b] / W + 50) / 100) : 0;
+      key += (b ? "." : "") + std::to_string(pct);
+    }
+    h->hist_key = key;
+  }
 
   // 6. sparseA2B + bitmaps
   const uint32_t words = h->words;
@@ -555,6 +574,25 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
   }
   *out = h;
   return ACCSPMM_OK;
+}
+
+extern "C" accspmm_status accspmm_metcf_build_host(const accspmm_csr_view* hc,
+                                                   const accspmm_build_opts* opts,
+                                                   accspmm_metcf** out) {
+  if (!hc || !out || !hc->row_ptr || (hc->nnz && (!hc->col_idx || !hc->val)))
+    return ACCSPMM_ERR_INVALID_ARG;
+  DevBuf rp, ci, va;
+  ACC_TRY(rp.alloc(((uint64_t)hc->rows + 1) * 4));
+  ACC_TRY(ci.alloc((uint64_t)hc->nnz * 4));
+  ACC_TRY(va.alloc((uint64_t)hc->nnz * 4));
+  ACC_HIP(hipMemcpy(rp.p, hc->row_ptr, ((uint64_t)hc->rows + 1) * 4, hipMemcpyHostToDevice));
+  if (hc->nnz) {
+    ACC_HIP(hipMemcpy(ci.p, hc->col_idx, (uint64_t)hc->nnz * 4, hipMemcpyHostToDevice));
+    ACC_HIP(hipMemcpy(va.p, hc->val, (uint64_t)hc->nnz * 4, hipMemcpyHostToDevice));
+  }
+  accspmm_csr_view dv{hc->rows, hc->cols, hc->nnz, rp.as<uint32_t>(), ci.as<uint32_t>(),
+                      va.as<float>()};
+  return accspmm_metcf_build(&dv, opts, out, nullptr);
 }
 
 extern "C" accspmm_status accspmm_metcf_destroy(accspmm_metcf* h) {

@@ -96,6 +96,7 @@ struct accspmm_metcf {
   accspmm_build_opts opts{};
   accspmm::DevBuf rwo, tco, bits, a2b, data, grp, rix;
   std::vector<uint32_t> h_rwo;  // host copy of rowWindowOffset
+  std::string hist_key;         // degree-histogram part of the tuning key
   // lazily built state (guarded by mu; handles are otherwise immutable)
   mutable std::mutex mu;
   mutable std::map<uint32_t, std::unique_ptr<accspmm::Schedule>> schedules;

@@ -1,0 +1,429 @@
+// spmm_device.hpp — device code shared by the single-graph SpMM kernel
+// (spmm.hip) and the persistent batched kernel (batch.hip): the per-work-unit
+// routine run_unit<FORM, NB, RW>.  Both kernels execute the same arithmetic
+// in the same order for a given (graph, K, NB, split threshold), so a problem
+// computed alone and inside a batch gives bit-identical C.
+#pragma once
+#include "internal.hpp"
+
+namespace accspmm {
+namespace dev {
+
+
+typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
+typedef __bf16 bf16x2 __attribute__((__vector_size__(2 * sizeof(__bf16))));
+typedef float f2 __attribute__((__vector_size__(2 * sizeof(float))));
+typedef float f4 __attribute__((__vector_size__(4 * sizeof(float))));
+typedef uint32_t u4 __attribute__((__vector_size__(4 * sizeof(uint32_t))));
+
+enum : int { kFP32 = 0, kSPLIT = 1, kBF16 = 2 };
+
+struct SpmmArgs {
+  const uint32_t* __restrict__ tco;
+  const unsigned long long* __restrict__ bits;
+  const uint32_t* __restrict__ a2b;
+  const float* __restrict__ data;
+  const WorkUnit* __restrict__ units;
+  const uint32_t* __restrict__ piece_info;  // per piece: (first slot, count, counter id)
+  uint32_t* __restrict__ counters;
+  float* __restrict__ partials;
+  const void* __restrict__ B;  // FP32: float; SPLIT: u32 (hi|lo<<16); BF16: u16
+  float* __restrict__ C;
+  uint32_t ldb, ldc;
+  uint32_t rows, K;
+  uint32_t num_units;
+  uint32_t n_kc;  // feature chunks
+  uint32_t kc_padded;  // KC * n_kc (partials row stride)
+  uint32_t xcd_slice;
+  uint32_t rw, nb;     // used by the batched kernel's dispatch
+};
+
+__device__ inline uint32_t popc64(unsigned long long x) { return __popcll(x); }
+
+// bf16 round-to-nearest-even pair
+__device__ inline uint32_t pack_bf16(float a, float b) {
+  f2 v = {a, b};
+  bf16x2 r = __builtin_convertvector(v, bf16x2);
+  return __builtin_bit_cast(uint32_t, r);
+}
+__device__ inline float bf16_lo_to_f32(uint32_t p) {
+  return __builtin_bit_cast(float, p << 16);
+}
+__device__ inline float bf16_hi_to_f32(uint32_t p) {
+  return __builtin_bit_cast(float, p & 0xFFFF0000u);
+}
+
+// Elements each lane gathers per B row: NB features, as 32-bit words.
+template <int FORM, int NB>
+struct BRow {
+  static constexpr int kWords = FORM == kBF16 ? (NB + 1) / 2 : NB;
+  uint32_t w[kWords];
+};
+
+// Branch-free gather of one B row segment.  A padded column (kPad) loads row
+// 0; the consumer masks it out (see Stage::bmask).  Every lane issues the same
+// vector loads and no result is touched until its stage is consumed, which
+// keeps the count of outstanding loads static: the compiler can then wait for
+// one pipeline stage without draining the younger ones.
+template <int FORM, int NB, bool FULL>
+__device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
+                                 uint32_t col, uint32_t feat, uint32_t K) {
+  const uint32_t c = col != kPad ? col : 0u;
+  if (FORM == kBF16) {
+    const uint16_t* p = static_cast<const uint16_t*>(B) + (uint64_t)c * ldb + feat;
+    if (NB == 8) {
+      u4 v = *reinterpret_cast<const u4*>(p);
+      r.w[0] = v[0]; r.w[1] = v[1]; r.w[2] = v[2]; r.w[3] = v[3];
+    } else if (NB == 4) {
+      uint2 v = *reinterpret_cast<const uint2*>(p);
+      r.w[0] = v.x; r.w[1] = v.y;
+    } else if (NB == 2) {
+      r.w[0] = *reinterpret_cast<const uint32_t*>(p);
+    } else {
+      r.w[0] = *p;
+    }
+  } else if (FULL) {
+    const uint32_t* p = static_cast<const uint32_t*>(B) + (uint64_t)c * ldb + feat;
+    if (NB >= 4) {
+#pragma unroll
+      for (int i = 0; i < NB / 4; ++i) {
+        u4 v = *reinterpret_cast<const u4*>(p + 4 * i);
+        r.w[4 * i] = v[0]; r.w[4 * i + 1] = v[1];
+        r.w[4 * i + 2] = v[2]; r.w[4 * i + 3] = v[3];
+      }
+    } else if (NB == 2) {
+      uint2 v = *reinterpret_cast<const uint2*>(p);
+      r.w[0] = v.x; r.w[1] = v.y;
+    } else {
+      r.w[0] = *p;
+    }
+  } else {
+    // feature tail (FP32 form only): clamp inside the row, zero past K
+    const uint32_t* row = static_cast<const uint32_t*>(B) + (uint64_t)c * ldb;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const uint32_t f = feat + i;
+      const uint32_t x = row[min(f, K - 1)];
+      r.w[i] = f < K ? x : 0u;
+    }
+  }
+}
+
+// One pipeline stage: everything a lane needs for one 4-tile group.
+// Loaded values are raw: bit j of bmask marks a real column j (else the row
+// is a padding read), bit j of amask a stored value of A in slot j.
+template <int FORM, int NB>
+struct Stage {
+  BRow<FORM, NB> b[8];
+  float a[8];
+  uint32_t bmask, amask;
+};
+
+// Tile metadata a lane needs for its tile of a group: the 8 condensed
+// columns, its row's bitmap word and the value offset of that word.
+struct Meta {
+  uint32_t col[8];
+  unsigned long long bw;
+  uint32_t off;
+};
+
+template <int RW>
+__device__ inline void load_meta(Meta& mt, const SpmmArgs& g, uint32_t t,
+                                 uint32_t t_last, bool tile_ok, uint32_t m) {
+  // past the unit's last tile: read the last tile, then mask it out
+  const uint32_t tt = tile_ok ? t : t_last;
+  const u4* cp = reinterpret_cast<const u4*>(g.a2b + (uint64_t)tt * 8);
+  const u4 c0 = cp[0], c1 = cp[1];
+  uint32_t off = g.tco[tt];
+  unsigned long long bw;
+  if (RW == 8) {
+    bw = g.bits[tt];
+  } else {
+    // 128-bit tile: rows 0-7 in word 0, rows 8-15 in word 1
+    const unsigned long long w0 = g.bits[2ull * tt], w1 = g.bits[2ull * tt + 1];
+    bw = m < 8 ? w0 : w1;
+    off += m < 8 ? 0u : popc64(w0);
+  }
+  mt.col[0] = c0[0]; mt.col[1] = c0[1]; mt.col[2] = c0[2]; mt.col[3] = c0[3];
+  mt.col[4] = c1[0]; mt.col[5] = c1[1]; mt.col[6] = c1[2]; mt.col[7] = c1[3];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) mt.col[j] = tile_ok ? mt.col[j] : kPad;
+  mt.bw = tile_ok ? bw : 0ull;
+  mt.off = off;
+}
+
+// Gathers the B rows of a tile's 8 columns and decodes the lane's A row.
+// Every value load is issued (at the row's first value when its bit is
+// clear); masking happens when the stage is consumed.
+template <int FORM, int NB, bool FULL>
+__device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
+                                 const SpmmArgs& g, uint32_t m, bool a_rows,
+                                 uint32_t feat, const void* B, uint32_t ldb) {
+  uint32_t bm = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    load_brow<FORM, NB, FULL>(st.b[j], B, ldb, mt.col[j], feat, g.K);
+    bm |= (mt.col[j] != kPad ? 1u : 0u) << j;
+  }
+  const uint32_t sh = 8u * (m & 7u);
+  const uint32_t byte = a_rows ? (uint32_t)(mt.bw >> sh) & 0xFFu : 0u;
+  const uint32_t base = mt.off + (sh ? popc64(mt.bw & ((1ull << sh) - 1ull)) : 0u);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const bool on = (byte >> j) & 1u;
+    const uint32_t rank = __popc(byte & ((1u << j) - 1u));
+    st.a[j] = g.data[on ? base + rank : mt.off];
+  }
+  st.bmask = bm;
+  st.amask = byte;
+}
+
+template <int FORM, int NB, int RW>
+__device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
+                                     uint32_t m) {
+  // apply the stage's masks now that its loads have landed
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const bool bv = (st.bmask >> j) & 1u;
+#pragma unroll
+    for (int i = 0; i < BRow<FORM, NB>::kWords; ++i) st.b[j].w[i] = bv ? st.b[j].w[i] : 0u;
+    st.a[j] = ((st.amask >> j) & 1u) ? st.a[j] : 0.0f;
+  }
+  if (FORM == kFP32) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+            st.a[j], __builtin_bit_cast(float, st.b[j].w[b]), acc[b], 0, 0, 0);
+    return;
+  }
+  // A operand: 8 bf16 along the tile's condensed columns
+  u4 aop, alo;
+  {
+    uint32_t hi[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) hi[d] = pack_bf16(st.a[2 * d], st.a[2 * d + 1]);
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      float l0 = st.a[2 * d] - bf16_lo_to_f32(hi[d]);
+      float l1 = st.a[2 * d + 1] - bf16_hi_to_f32(hi[d]);
+      alo[d] = FORM == kSPLIT ? pack_bf16(l0, l1) : 0u;
+    }
+    if (FORM == kSPLIT && RW == 8 && m >= 8) {
+      aop = alo;  // rows 8-15 carry the low halves
+    } else {
+      aop = u4{hi[0], hi[1], hi[2], hi[3]};
+    }
+  }
+  const bf16x8 A = __builtin_bit_cast(bf16x8, aop);
+  const bf16x8 AL = __builtin_bit_cast(bf16x8, alo);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (FORM == kSPLIT) {
+      u4 bh, bl;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        uint32_t x = st.b[2 * d].w[b], y = st.b[2 * d + 1].w[b];
+        bh[d] = __builtin_amdgcn_perm(y, x, 0x05040100u);
+        bl[d] = __builtin_amdgcn_perm(y, x, 0x07060302u);
+      }
+      const bf16x8 BH = __builtin_bit_cast(bf16x8, bh);
+      const bf16x8 BL = __builtin_bit_cast(bf16x8, bl);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, BH, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, BL, acc[b], 0, 0, 0);
+      if (RW == 16) {
+        // all 16 rows are tile rows: the low halves of A need their own MFMAs
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL, BH, acc[b], 0, 0, 0);
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL, BL, acc[b], 0, 0, 0);
+      }
+    } else {
+      u4 bv;
+      const uint32_t sel = (b & 1) ? 0x07060302u : 0x05040100u;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        uint32_t x = st.b[2 * d].w[b / 2], y = st.b[2 * d + 1].w[b / 2];
+        bv[d] = __builtin_amdgcn_perm(y, x, sel);
+      }
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+          A, __builtin_bit_cast(bf16x8, bv), acc[b], 0, 0, 0);
+    }
+  }
+}
+
+template <int NB>
+__device__ inline void store_vec(float* p, const float* v, uint32_t nvalid) {
+  if (nvalid >= (uint32_t)NB) {
+    if (NB >= 4) {
+#pragma unroll
+      for (int i = 0; i < NB / 4; ++i)
+        *reinterpret_cast<f4*>(p + 4 * i) = f4{v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]};
+    } else if (NB == 2) {
+      *reinterpret_cast<f2*>(p) = f2{v[0], v[1]};
+    } else {
+      p[0] = v[0];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      if ((uint32_t)i < nvalid) p[i] = v[i];
+  }
+}
+
+// Three-stage software pipeline over a unit's 4-tile groups: the metadata
+// of group g+2 and the gathers of group g+1 are in flight while the MFMAs of
+// group g run.  The steady-state loop issues the same loads on every path
+// (order m(g+2), d(g+1), compute(g)), so each wait on the in-order
+// vector-memory counter drains only the stage it needs; the last one to
+// three groups run as straight-line tail code.
+template <int FORM, int NB, int RW, bool FULL>
+__device__ inline void unit_loop(const SpmmArgs& g, f4* acc, uint32_t tb,
+                                 uint32_t te, uint32_t ngroups, uint32_t q,
+                                 uint32_t m, bool a_rows, uint32_t feat,
+                                 const void* B, uint32_t ldb) {
+  Meta mA, mB;
+  Stage<FORM, NB> dA, dB;
+  const uint32_t tl = te - 1;  // ngroups > 0 => the unit has a tile
+  auto tile = [&](uint32_t gi) { return tb + 4 * gi + q; };
+  load_meta<RW>(mA, g, tile(0), tl, tile(0) < te, m);
+  load_meta<RW>(mB, g, tile(1), tl, tile(1) < te, m);
+  load_data<FORM, NB, FULL>(dA, mA, g, m, a_rows, feat, B, ldb);
+  uint32_t gi = 0;
+  // sched_barrier keeps each stage's loads issued before the previous
+  // stage's consumer code (the scheduler would otherwise hoist the cheap
+  // masking and force early waits)
+  for (; gi + 3 < ngroups; gi += 2) {
+    load_meta<RW>(mA, g, tile(gi + 2), tl, true, m);
+    load_data<FORM, NB, FULL>(dB, mB, g, m, a_rows, feat, B, ldb);
+    __builtin_amdgcn_sched_barrier(0);
+    compute_stage<FORM, NB, RW>(dA, acc, m);
+    __builtin_amdgcn_sched_barrier(0);
+    load_meta<RW>(mB, g, tile(gi + 3), tl, tile(gi + 3) < te, m);
+    load_data<FORM, NB, FULL>(dA, mA, g, m, a_rows, feat, B, ldb);
+    __builtin_amdgcn_sched_barrier(0);
+    compute_stage<FORM, NB, RW>(dB, acc, m);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // dA holds group gi, mB the metadata of group gi+1
+  const uint32_t left = ngroups - gi;  // 1, 2 or 3
+  if (left == 3) {
+    load_meta<RW>(mA, g, tile(gi + 2), tl, tile(gi + 2) < te, m);
+    load_data<FORM, NB, FULL>(dB, mB, g, m, a_rows, feat, B, ldb);
+    __builtin_amdgcn_sched_barrier(0);
+    compute_stage<FORM, NB, RW>(dA, acc, m);
+    __builtin_amdgcn_sched_barrier(0);
+    load_data<FORM, NB, FULL>(dA, mA, g, m, a_rows, feat, B, ldb);
+    __builtin_amdgcn_sched_barrier(0);
+    compute_stage<FORM, NB, RW>(dB, acc, m);
+    compute_stage<FORM, NB, RW>(dA, acc, m);
+  } else if (left == 2) {
+    load_data<FORM, NB, FULL>(dB, mB, g, m, a_rows, feat, B, ldb);
+    __builtin_amdgcn_sched_barrier(0);
+    compute_stage<FORM, NB, RW>(dA, acc, m);
+    compute_stage<FORM, NB, RW>(dB, acc, m);
+  } else {
+    compute_stage<FORM, NB, RW>(dA, acc, m);
+  }
+}
+
+template <int FORM, int NB, int RW>
+__device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
+                         const void* B, uint32_t ldb, float* C, uint32_t ldc,
+                         uint32_t K, uint32_t rows) {
+  constexpr uint32_t KC = 16u * NB;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t q = lane >> 4, n = lane & 15u;
+  const uint32_t k0 = kc * KC;
+  const uint32_t feat = k0 + NB * n;
+  // B loads are full-width unless this chunk straddles K (FP32 form; the
+  // converted BF16-form buffers are zero-padded to a multiple of KC)
+  const bool full = FORM != kFP32 || k0 + KC <= K;
+  // A rows: lanes m = n carry tile row m (RW=16) or m%8 (RW=8; rows 8-15
+  // only in the split form, as the low halves)
+  const uint32_t m = n;
+  const bool a_rows = RW == 16 || m < 8 || FORM == kSPLIT;
+
+  f4 acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+
+  const uint32_t tb = u.tile_begin, te = u.tile_end;
+  const uint32_t ngroups = (te - tb + 3) / 4;
+  if (ngroups > 0) {
+    if (full)
+      unit_loop<FORM, NB, RW, true>(g, acc, tb, te, ngroups, q, m, a_rows, feat, B, ldb);
+    else
+      unit_loop<FORM, NB, RW, false>(g, acc, tb, te, ngroups, q, m, a_rows, feat, B, ldb);
+  }
+
+  // combine rows r and r+8 (split form, RW=8): lanes l and l^32
+  if (FORM == kSPLIT && RW == 8) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[b][i] += __shfl_xor(acc[b][i], 32, 64);
+  }
+  // rows held by this lane: 4q + i
+  const bool store_lane = RW == 16 || q < 2;
+  const uint32_t nvalid = feat < K ? min((uint32_t)NB, K - feat) : 0u;
+  if (u.slot < 0) {
+    if (store_lane) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        uint32_t r = u.window * RW + 4 * q + i;
+        if (r < rows && nvalid) {
+          float v[NB];
+#pragma unroll
+          for (int b = 0; b < NB; ++b) v[b] = acc[b][i];
+          store_vec<NB>(C + (uint64_t)r * ldc + feat, v, nvalid);
+        }
+      }
+    }
+    return;
+  }
+  // split piece: write partial (RW rows x kc_padded floats per slot)
+  float* part = g.partials + (uint64_t)u.slot * RW * g.kc_padded;
+  if (store_lane) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float v[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) v[b] = acc[b][i];
+      store_vec<NB>(part + (uint64_t)(4 * q + i) * g.kc_padded + NB * n + k0, v, NB);
+    }
+  }
+  const uint32_t* info = g.piece_info + 3ull * (uint32_t)u.slot;
+  const uint32_t first = info[0], count = info[1], cid = info[2] * g.n_kc + kc;
+  __threadfence();
+  uint32_t prev = 0;
+  if (lane == 0)
+    prev = __hip_atomic_fetch_add(g.counters + cid, 1u, __ATOMIC_ACQ_REL,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+  prev = __shfl(prev, 0, 64);
+  if (prev != count - 1) return;
+  __threadfence();
+  if (lane == 0)
+    __hip_atomic_store(g.counters + cid, 0u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  // last piece: sum partials of pieces first..first+count-1 in order
+  if (!store_lane) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float v[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) v[b] = 0.f;
+    for (uint32_t p = 0; p < count; ++p) {
+      const float* src = g.partials + (uint64_t)(first + p) * RW * g.kc_padded +
+                         (uint64_t)(4 * q + i) * g.kc_padded + NB * n + k0;
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        v[b] += __hip_atomic_load(src + b, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+    }
+    uint32_t r = u.window * RW + 4 * q + i;
+    if (r < rows && nvalid) store_vec<NB>(C + (uint64_t)r * ldc + feat, v, nvalid);
+  }
+}
+
+}  // namespace dev
+}  // namespace accspmm

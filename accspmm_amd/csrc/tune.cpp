@@ -40,24 +40,11 @@ std::mutex g_mu;
 std::map<std::string, accspmm_tile> g_cache;
 bool g_env_loaded = false;
 
-// Log2 buckets of tiles per window, as percentages rounded to 10%, plus the
-// log2 of the window count; graphs with the same key get the same tile.
+// Tuning key: form, K and the handle's degree-histogram key (log2 buckets of
+// tiles per window in 10% steps, log2 of the window count); graphs with the
+// same key get the same tile.
 std::string make_key(const accspmm_metcf* h, uint32_t K, int form) {
-  uint32_t buckets[24] = {0};
-  for (uint32_t w = 0; w < h->W; ++w) {
-    uint32_t t = h->h_rwo[w + 1] - h->h_rwo[w];
-    uint32_t b = 0;
-    while (b < 23 && (1u << b) <= t) ++b;
-    buckets[b]++;
-  }
-  std::ostringstream os;
-  os << "f" << form << "-k" << K << "-rw" << h->rw << "-w"
-     << (h->W ? (int)std::log2((double)h->W) : 0) << "-h";
-  for (int b = 0; b < 24; ++b) {
-    int pct = h->W ? (int)std::lround(100.0 * buckets[b] / h->W / 10.0) : 0;
-    os << (b ? "." : "") << pct;
-  }
-  return os.str();
+  return "f" + std::to_string(form) + "-k" + std::to_string(K) + "-" + h->hist_key;
 }
 
 void load_file(const char* path) {

@@ -116,6 +116,88 @@ def check_errors(csr, B, results):
     return errs
 
 
+def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000):
+    """Serving: Poisson arrivals over a pool of 64 resident Chung-Lu graphs
+    (500-50,000 nodes, mean degree U[4,64], exponent U[2.0,2.5]) plus two
+    2M-node mean-degree-32 graphs; each request is one graph (uniform) and K
+    uniform in {32, 128}, split-BF16.  Runs the continuous-batching server and
+    the one-launch-per-request baseline: saturation throughput of each, then
+    p50/p99 latency of both at the same fixed QPS."""
+    import numpy as np
+    import torch
+
+    import accspmm_amd as acc
+    rng = np.random.default_rng(3000)
+    pool = []
+    for i in range(64):
+        n = int(round(math.exp(rng.uniform(math.log(500), math.log(50000)))))
+        pool.append(acc.gen_chung_lu(n, float(rng.uniform(4, 64)),
+                                     float(rng.uniform(2.0, 2.5)), 3000 + i, 0))
+    for i in range(2):
+        pool.append(acc.gen_chung_lu(big_nodes, 32.0, 2.1, 3100 + i, 0))
+    log(f"serve: pool of {len(pool)} graphs, {sum(c.nnz for c in pool)} nnz")
+    Ks = (32, 128)
+    bufs = {}
+    for gi, c in enumerate(pool):
+        for K in Ks:
+            B = torch.rand((c.cols, K), device="cuda") * 2 - 1
+            Cs = [torch.empty((c.rows, K), device="cuda") for _ in range(2)]
+            bufs[(gi, K)] = (B, Cs)
+    req_rng = np.random.default_rng(seed)
+    gsel = req_rng.integers(0, len(pool), size=requests)
+    ksel = req_rng.choice(Ks, size=requests)
+    flops = np.array([2.0 * pool[g].nnz * k for g, k in zip(gsel, ksel)])
+
+    def run(server, qps):
+        arr_rng = np.random.default_rng(5000)
+        gaps = arr_rng.exponential(1.0 / qps, size=requests) if qps else np.zeros(requests)
+        rids = []
+        t0 = time.perf_counter()
+        t_next = t0
+        for i in range(requests):
+            t_next += gaps[i]
+            while qps and time.perf_counter() < t_next:
+                pass
+            B, Cs = bufs[(int(gsel[i]), int(ksel[i]))]
+            rids.append(server.submit(gids[int(gsel[i])], B, Cs[i % 2]))
+        server.drain()
+        t1 = time.perf_counter()
+        lats = np.array([server.poll(r) for r in rids])
+        return t1 - t0, lats
+
+    out = {}
+    for mode in ("one_per_launch", "batched"):
+        srv = acc.Server(hbm_budget_bytes=200 << 30, max_batch=64, max_inflight=3,
+                         form=acc.FORM_SPLIT_BF16, one_per_launch=(mode == "one_per_launch"))
+        gids = [srv.register(c) for c in pool]
+        run(srv, 0)  # warm-up: schedules, workspaces
+        wall, lats = run(srv, 0)
+        st = srv.stats()
+        out[mode] = {"saturation_req_per_s": requests / wall,
+                     "saturation_gflops": flops.sum() / wall / 1e9,
+                     "launches": int(st.launches)}
+        srv.close()
+        log(f"serve: {mode}: {requests / wall:.0f} req/s, {flops.sum() / wall / 1e9:.1f} GFLOP/s "
+            f"at saturation")
+    qps = 0.7 * out["one_per_launch"]["saturation_req_per_s"]
+    for mode in ("one_per_launch", "batched"):
+        srv = acc.Server(hbm_budget_bytes=200 << 30, max_batch=64, max_inflight=3,
+                         form=acc.FORM_SPLIT_BF16, one_per_launch=(mode == "one_per_launch"))
+        gids = [srv.register(c) for c in pool]
+        run(srv, 0)
+        wall, lats = run(srv, qps)
+        srv.close()
+        out[mode].update({"qps": round(qps, 1), "p50_ms": float(np.percentile(lats, 50) * 1e3),
+                          "p99_ms": float(np.percentile(lats, 99) * 1e3),
+                          "gflops_at_qps": flops.sum() / wall / 1e9})
+        log(f"serve: {mode} at {qps:.0f} QPS: p50 {out[mode]['p50_ms']:.3f} ms, "
+            f"p99 {out[mode]['p99_ms']:.3f} ms")
+    out["throughput_ratio"] = (out["batched"]["saturation_req_per_s"] /
+                               out["one_per_launch"]["saturation_req_per_s"])
+    out["requests"] = requests
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +209,8 @@ def main():
     ap.add_argument("--tile", default="", help="nb,wpb,split,xcd (default: library pick)")
     ap.add_argument("--tune", action="store_true", help="autotune each form first")
     ap.add_argument("--no-check", action="store_true", help="skip errors and CPU baseline")
+    ap.add_argument("--serve-requests", type=int, default=10000,
+                    help="requests per serving run (0 skips the serving benchmark)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -202,6 +286,12 @@ def main():
                          f"median of {cpu.pop('runs')} runs")
         log(f"bench: CPU baseline {cpu['value']:.1f} GFLOP/s on {cpu['cores']} threads")
 
+    serving = None
+    if args.serve_requests > 0:
+        del g, B, Cd, outputs
+        torch.cuda.empty_cache()
+        serving = serve_bench(args.serve_requests)
+
     head = forms[0]
     hr = per_form[head]
     head_form = name_to_form[head]
@@ -234,6 +324,7 @@ def main():
                      "kernel": "spmm_kernel", "kernel_ms": round(hr["kernel_ms_mean"], 4),
                      "alg_bytes_per_launch": ab},
         "cpu_baseline": cpu,
+        "serving": serving,
         "build_ms": round(build_ms, 1),
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv)
                       for kk, vv in v.items()} for k, v in per_form.items()},

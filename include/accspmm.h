@@ -93,6 +93,11 @@ typedef struct accspmm_metcf_info {
 accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
                                    const accspmm_build_opts* opts,
                                    accspmm_metcf** out, void* stream);
+/* Same as accspmm_metcf_build, from a HOST CSR (uploaded by the library on
+ * the null stream).  Used by the format.hpp drop-in classes. */
+accspmm_status accspmm_metcf_build_host(const accspmm_csr_view* host_csr,
+                                        const accspmm_build_opts* opts,
+                                        accspmm_metcf** out);
 accspmm_status accspmm_metcf_destroy(accspmm_metcf* h);
 accspmm_status accspmm_metcf_get_info(const accspmm_metcf* h,
                                       accspmm_metcf_info* info);
@@ -278,8 +283,12 @@ accspmm_status accspmm_server_submit(accspmm_server* s, uint32_t graph_id,
  * submit to completion), ACCSPMM_ERR_NOT_READY otherwise. */
 accspmm_status accspmm_server_poll(accspmm_server* s, uint64_t request_id,
                                    double* latency_s);
-/* Blocks until every submitted request has completed. */
+/* Blocks until every submitted request has completed (pending requests of a
+ * paused server are not waited for). */
 accspmm_status accspmm_server_drain(accspmm_server* s);
+/* Stops (paused != 0) or resumes admitting queued requests; launches
+ * already on the GPU finish either way. */
+accspmm_status accspmm_server_pause(accspmm_server* s, int paused);
 typedef struct accspmm_server_stats {
   uint64_t submitted, completed, launches, rejected_budget;
   uint64_t resident_bytes, peak_inflight_bytes;

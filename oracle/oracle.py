@@ -107,7 +107,9 @@ def _unpack(m, adp: bool) -> dict:
     T, W, words = s.num_tiles, s.num_windows, s.rw // 8
 
     def arr(ptr, n, dt):
-        return np.ctypeslib.as_array(ptr, shape=(max(n, 1),))[:n].astype(dt, copy=True)
+        if n == 0 or not ptr:
+            return np.zeros(0, dt)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True)
 
     out = {
         "rowWindowOffset": arr(s.rowWindowOffset, W + 1, np.uint32),

@@ -1,0 +1,50 @@
+"""The C-ABI library loads and exports every symbol declared in include/*.h
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import re
+from pathlib import Path
+
+import accspmm_amd as acc
+
+REPO = Path(__file__).resolve().parent.parent
+
+
+def _declared():
+    names = set()
+    for h in (REPO / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names |= set(re.findall(r"\b(accspmm_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    L = ctypes.CDLL(str(acc.LIB_PATH))
+    decl = _declared()
+    assert len(decl) > 30
+    missing = sorted(n for n in decl if not hasattr(L, n))
+    assert not missing, missing
+
+
+def test_python_bindings_cover_the_abi():
+    assert _declared() <= set(acc._SIGS), sorted(_declared() - set(acc._SIGS))
+
+
+def test_status_strings_and_version():
+    L = acc.lib()
+    assert L.accspmm_status_string(0) == b"ok"
+    assert b"increasing" in L.accspmm_status_string(2)
+    assert b"gfx950" in L.accspmm_version()
+    o = acc.BuildOpts()
+    L.accspmm_build_opts_default(ctypes.byref(o))
+    assert (o.row_window, o.col_window, o.t_load, o.t_comp, o.t_write, o.target_num,
+            o.imbalance_threshold) == (8, 8, 3, 1, 2, 16, 8)
+
+
+def test_null_arguments_rejected_without_gpu():
+    L = acc.lib()
+    assert L.accspmm_metcf_build(None, None, None, None) == acc.Status.ERR_INVALID_ARG
+    assert L.accspmm_spmm(None, None, 0, None, 0, 0, 0, None, None) == \
+        acc.Status.ERR_INVALID_ARG
+    assert L.accspmm_metcf_destroy(None) == acc.Status.ERR_INVALID_ARG
+    assert L.accspmm_batch_run(None, None) == acc.Status.ERR_INVALID_ARG
+    assert L.accspmm_server_poll(None, 0, None) == acc.Status.ERR_INVALID_ARG

@@ -1,0 +1,109 @@
+"""Serving loop: results under continuous batching equal the standalone
+results bit for bit, admission honours the HBM budget, and the error paths
+(unknown graph, full queue, budget) are reported."""
+import time
+
+import numpy as np
+import pytest
+
+import accspmm_amd as acc
+
+pytestmark = pytest.mark.gpu
+
+
+def _pool(n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nodes = int(round(np.exp(rng.uniform(np.log(500), np.log(8000)))))
+        out.append(acc.gen_chung_lu(nodes, float(rng.uniform(4, 64)),
+                                    float(rng.uniform(2.0, 2.5)), 3000 + i, 0))
+    return out
+
+
+def test_mix_equals_alone():
+    import torch
+    pool = _pool(8, 1)
+    srv = acc.Server(max_batch=8, max_inflight=2, form=acc.FORM_SPLIT_BF16)
+    gids = [srv.register(c) for c in pool]
+    graphs = [acc.DeviceGraph(c) for c in pool]
+    rng = np.random.default_rng(2)
+    jobs = []
+    for i in range(40):
+        j = int(rng.integers(len(pool)))
+        K = int(rng.choice([32, 128]))
+        B = torch.from_numpy(acc.gen_dense(100 + i, pool[j].cols, K)).cuda()
+        Cm = torch.full((pool[j].rows, K), float("nan"), device="cuda")
+        jobs.append((j, B, Cm, srv.submit(gids[j], B, Cm)))
+        if i % 7 == 0:
+            time.sleep(0.002)  # arrivals while earlier launches are running
+    srv.drain()
+    for j, B, Cm, rid in jobs:
+        lat = srv.poll(rid)
+        assert lat is not None and lat > 0
+        alone = graphs[j].spmm(B, None, acc.FORM_SPLIT_BF16)
+        torch.cuda.synchronize()
+        assert torch.equal(Cm, alone)
+    st = srv.stats()
+    assert st.completed == 40 and st.launches < 40  # requests were batched
+    srv.close()
+
+
+def test_one_per_launch_baseline_matches():
+    import torch
+    pool = _pool(3, 3)
+    srv = acc.Server(form=acc.FORM_SPLIT_BF16, one_per_launch=True)
+    gids = [srv.register(c) for c in pool]
+    outs = []
+    for i, (c, g) in enumerate(zip(pool, gids)):
+        B = torch.from_numpy(acc.gen_dense(7 + i, c.cols, 64)).cuda()
+        Cm = torch.zeros((c.rows, 64), device="cuda")
+        outs.append((c, B, Cm, srv.submit(g, B, Cm)))
+    srv.drain()
+    for c, B, Cm, rid in outs:
+        assert srv.poll(rid) is not None
+        assert torch.equal(Cm, acc.DeviceGraph(c).spmm(B, None, acc.FORM_SPLIT_BF16))
+    assert srv.stats().launches == 3
+    srv.close()
+
+
+def test_budget_queue_and_unknown_graph():
+    import torch
+    c = acc.gen_chung_lu(4000, 16.0, 2.2, 9, 0)
+    probe = acc.DeviceGraph(c)
+    gbytes = probe.info.device_bytes
+    # a budget that holds the graph but not a second copy
+    srv = acc.Server(hbm_budget_bytes=int(gbytes * 1.5) + (8 << 20), max_queue=2,
+                     max_inflight=1)
+    gid = srv.register(c)
+    with pytest.raises(acc.AccSpMMError) as e:
+        srv.register(c)
+    assert e.value.status == acc.Status.ERR_BUDGET
+    B = torch.zeros((c.cols, 16), device="cuda")
+    Cm = torch.zeros((c.rows, 16), device="cuda")
+    with pytest.raises(acc.AccSpMMError) as e:
+        srv.submit(gid + 99, B, Cm)
+    assert e.value.status == acc.Status.ERR_NOT_FOUND
+    # a request whose workspace can never fit is refused up front
+    Bbig = torch.zeros((c.cols, 4096), device="cuda")
+    Cbig = torch.zeros((c.rows, 4096), device="cuda")
+    with pytest.raises(acc.AccSpMMError) as e:
+        srv.submit(gid, Bbig, Cbig)
+    assert e.value.status == acc.Status.ERR_BUDGET
+    # queue full while paused
+    srv.pause(True)
+    r1 = srv.submit(gid, B, Cm)
+    r2 = srv.submit(gid, B, Cm)
+    with pytest.raises(acc.AccSpMMError) as e:
+        srv.submit(gid, B, Cm)
+    assert e.value.status == acc.Status.ERR_QUEUE_FULL
+    assert srv.poll(r1) is None  # still queued
+    with pytest.raises(acc.AccSpMMError) as e:
+        srv.unregister(gid)      # requests still reference it
+    assert e.value.status == acc.Status.ERR_NOT_READY
+    srv.pause(False)
+    srv.drain()
+    assert srv.poll(r1) is not None and srv.poll(r2) is not None
+    srv.unregister(gid)
+    assert srv.stats().resident_bytes == 0
+    srv.close()

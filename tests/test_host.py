@@ -1,0 +1,127 @@
+"""Host utilities of the product library (MTX ingest, CSR, generators)
+against the oracle and closed forms.  No GPU."""
+import numpy as np
+import pytest
+
+import accspmm_amd as acc
+from helpers import kat
+from oracle import oracle as O
+
+K = kat()
+
+
+def _write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return p
+
+
+@pytest.mark.parametrize("name", ["symmetric_3x3", "general_pattern"])
+def test_read_mtx_known_answers(tmp_path, name):
+    case = K["mtx"][name]
+    csr = acc.read_mtx(_write(tmp_path, name + ".mtx", case["text"]))
+    assert csr.row_ptr.tolist() == case["row_ptr"]
+    assert csr.col_idx.tolist() == case["col_idx"]
+    assert csr.data.tolist() == case["val"]
+
+
+def test_read_mtx_bad_banners(tmp_path):
+    for i, text in enumerate(K["mtx"]["bad_banners"]):
+        with pytest.raises(acc.AccSpMMError) as e:
+            acc.read_mtx(_write(tmp_path, f"b{i}.mtx", text))
+        assert e.value.mtx_code == -2
+    with pytest.raises(acc.AccSpMMError) as e:
+        acc.read_mtx(tmp_path / "missing.mtx")
+    assert e.value.mtx_code == -1
+
+
+@pytest.mark.parametrize("field", ["real", "pattern", "integer"])
+@pytest.mark.parametrize("sym", ["general", "symmetric"])
+def test_read_mtx_matches_oracle(tmp_path, field, sym):
+    rng = np.random.default_rng(hash((field, sym)) % 1000)
+    n = 60
+    ents = set()
+    while len(ents) < 300:
+        r, c = rng.integers(1, n + 1, 2)
+        if sym == "symmetric" and c > r:
+            r, c = c, r
+        ents.add((int(r), int(c)))
+    lines = [f"%%MatrixMarket matrix coordinate {field} {sym}", "% comment",
+             f"{n} {n} {len(ents)}"]
+    for r, c in sorted(ents, key=lambda x: (x[1], x[0])):
+        if field == "real":
+            lines.append(f"{r} {c} {rng.uniform(-5, 5):.9g}")
+        elif field == "integer":
+            lines.append(f"{r} {c} {int(rng.integers(-9, 10))}")
+        else:
+            lines.append(f"{r} {c}")
+    p = _write(tmp_path, "m.mtx", "\n".join(lines) + "\n")
+    code, m, nn, rr, cc, vv = O.read_mtx(str(p))
+    assert code == 0
+    rp, ci, va = O.coo_to_csr(m, rr, cc, vv)
+    got = acc.read_mtx(p)
+    assert np.array_equal(got.row_ptr, rp)
+    assert np.array_equal(got.col_idx, ci)
+    assert np.array_equal(got.data, va)
+
+
+def test_canonicalize_sorts_and_sums(tmp_path):
+    text = ("%%MatrixMarket matrix coordinate real general\n3 4 5\n"
+            "1 3 1.0\n1 1 2.0\n1 3 0.5\n3 2 4.0\n2 4 1.0\n")
+    p = _write(tmp_path, "u.mtx", text)
+    raw = acc.read_mtx(p)
+    assert not raw.is_canonical()
+    can = acc.read_mtx(p, canonicalize=True)
+    assert can.is_canonical()
+    assert can.row_ptr.tolist() == [0, 2, 3, 4]
+    assert can.col_idx.tolist() == [0, 2, 3, 1]
+    assert can.data.tolist() == [2.0, 1.5, 1.0, 4.0]
+
+
+def test_chung_lu_properties_and_determinism():
+    a = acc.gen_chung_lu(3000, 20.0, 2.1, 5, 0)
+    b = acc.gen_chung_lu(3000, 20.0, 2.1, 5, 0)
+    assert np.array_equal(a.col_idx, b.col_idx) and np.array_equal(a.data, b.data)
+    assert a.is_canonical()
+    import scipy.sparse as sp
+    A = sp.csr_matrix((a.data, a.col_idx, a.row_ptr), shape=(a.rows, a.cols))
+    assert (A != A.T).nnz == 0            # symmetric pattern and values
+    assert np.all(A.diagonal() > 0)       # a self loop on every node
+    assert np.all((a.data > 0) & (a.data <= 1))
+    deg = np.diff(a.row_ptr.astype(np.int64))
+    assert 0.85 * 21 < deg.mean() < 1.05 * 21
+    assert deg.max() > 8 * deg.mean()     # heavy tail
+    c = acc.gen_chung_lu(3000, 20.0, 2.1, 6, 1)
+    rows = np.repeat(np.arange(c.rows), np.diff(c.row_ptr.astype(np.int64)))
+    assert np.allclose(c.data, (1.0 / np.diff(c.row_ptr.astype(np.float64)))[rows])
+
+
+def test_stencil_closed_form():
+    for g in (1, 2, 5):
+        s = acc.gen_stencil27(g, 3)
+        assert s.rows == g ** 3 and s.nnz == (3 * g - 2) ** 3
+        assert s.is_canonical()
+    s = acc.gen_stencil27(4, 3)
+    # interior node (1,1,1) has 27 neighbours, corner (0,0,0) has 8
+    deg = np.diff(s.row_ptr.astype(np.int64))
+    assert deg[1 + 4 + 16] == 27 and deg[0] == 8
+    assert np.all((s.data >= -1) & (s.data < 1))
+
+
+def test_dense_generator_range():
+    d = acc.gen_dense(1, 100, 33)
+    assert d.shape == (100, 33) and np.all((d >= -1) & (d < 1))
+    assert np.array_equal(d, acc.gen_dense(1, 100, 33))
+
+
+def test_mma_driver_exit_codes(tmp_path):
+    import subprocess
+    mma = str(acc.LIB_PATH.parent.parent / "bin" / "mma")
+    r = subprocess.run([mma], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 4 and "Usage" in r.stdout
+    r = subprocess.run([mma, str(tmp_path / "none.mtx"), "32"], capture_output=True,
+                       text=True, cwd=tmp_path)
+    assert r.returncode == 3
+    bad = _write(tmp_path, "bad.mtx", K["mtx"]["bad_banners"][0])
+    r = subprocess.run([mma, str(bad), "32"], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 3 and "banner" in r.stdout
