@@ -85,6 +85,17 @@ def build_driver() -> Path:
     return MMA
 
 
+def build_test_programs() -> None:
+    """C++ programs the GPU tests run (reference-style users of the headers)."""
+    src = REPO / "tests" / "cpp" / "format_dropin.cpp"
+    if not src.exists():
+        return
+    _run(["g++", *HOST_FLAGS, "-I", str(PKG / "include"), str(src), "-o",
+          str(src.with_suffix("")), "-L", str(LIB.parent), "-laccspmm",
+          "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../accspmm_amd/lib",
+          "-Wl,-rpath,/opt/rocm/lib"])
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", str(ORACLE)])
     return ORACLE / "liboracle.so"
@@ -93,6 +104,7 @@ def build_oracle() -> Path:
 def build_all() -> None:
     build_library()
     build_driver()
+    build_test_programs()
     build_oracle()
 
 

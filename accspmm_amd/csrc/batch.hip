@@ -97,9 +97,9 @@ __global__ __launch_bounds__(256) void k_batch_spmm(
     // one row-window size per kernel keeps the register budget of the
     // widest variant (NB=4) without spills
     switch (g.nb) {
-      case 1: run_unit<FORM, 1, RW>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
-      case 2: run_unit<FORM, 2, RW>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
-      case 4: run_unit<FORM, 4, RW>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
+      case 1: run_unit<FORM, 1, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
+      case 2: run_unit<FORM, 2, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
+      case 4: run_unit<FORM, 4, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
       default: break;
     }
   }
@@ -126,7 +126,7 @@ accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
   out->kc_padded = out->n_kc * KC;
   {
     std::lock_guard<std::mutex> lock(h->mu);
-    ACC_TRY(get_schedule(h, t.split_tiles, s, &out->sched));
+    ACC_TRY(get_schedule(h, t.split_tiles, t.pack_tiles, s, &out->sched));
   }
   out->tasks = (uint64_t)out->sched->num_units * out->n_kc;
   out->bconv_bytes =
@@ -212,7 +212,8 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     const accspmm_metcf* h = p.graph;
     dev::SpmmArgs& g = D[j];
     std::memset(&g, 0, sizeof(g));
-    g.tco = h->tco.as<uint32_t>();
+    g.rwo = h->rwo.as<uint32_t>();
+  g.tco = h->tco.as<uint32_t>();
     g.bits = h->bits.as<unsigned long long>();
     g.a2b = h->a2b.as<uint32_t>();
     g.data = h->data.as<float>();

@@ -65,23 +65,31 @@ struct DevBuf {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
-// Work unit of the SpMM kernels: a contiguous tile range of one row window.
-// `slot` >= 0 marks a piece of a split window (partials combined in a fixed
-// order by the last piece to finish); -1 writes C directly.
+// Work unit of the SpMM kernels: either a run of consecutive whole row
+// windows [window, window + nwin) with their tiles [tile_begin, tile_end)
+// (slot < 0; C rows are written directly, window by window), or one piece
+// of a split window (nwin == 1, slot >= 0: a partial, combined in a fixed
+// order by the last piece to finish).  ngroups counts the unit's 4-tile
+// groups; a group never spans two windows.
 struct alignas(16) WorkUnit {
   uint32_t window;
+  uint32_t nwin;
   uint32_t tile_begin;
   uint32_t tile_end;
-  int32_t slot;  // split-piece slot (global piece index), or -1
+  int32_t slot;
+  uint32_t ngroups;
+  uint32_t pad0, pad1;
 };
 
 struct Schedule {
   uint32_t split_tiles = 0xFFFFFFFFu;
+  uint32_t pack_tiles = 0;
   uint32_t num_units = 0;
   uint32_t num_split_windows = 0;
   uint32_t num_pieces = 0;  // units with slot >= 0
+  bool packed = false;      // some unit spans several non-empty windows
   DevBuf units;             // WorkUnit[num_units]
-  DevBuf piece_first;       // per split window: first piece slot, count
+  DevBuf piece_first;       // per piece: first slot, piece count, split window id
   DevBuf counters;          // per split window arrival counter (self-resetting)
 };
 
@@ -99,7 +107,7 @@ struct accspmm_metcf {
   std::string hist_key;         // degree-histogram part of the tuning key
   // lazily built state (guarded by mu; handles are otherwise immutable)
   mutable std::mutex mu;
-  mutable std::map<uint32_t, std::unique_ptr<accspmm::Schedule>> schedules;
+  mutable std::map<uint64_t, std::unique_ptr<accspmm::Schedule>> schedules;
   mutable accspmm::DevBuf bconv;    // converted-B workspace (BF16 forms)
   mutable accspmm::DevBuf partials; // split-window partial sums
   uint64_t device_bytes() const {
@@ -116,5 +124,6 @@ accspmm_status inclusive_scan_u32(const uint32_t* in, uint32_t* out,
                                   uint64_t n, hipStream_t s);
 // spmm.hip
 accspmm_status get_schedule(const accspmm_metcf* h, uint32_t split_tiles,
-                            hipStream_t s, const Schedule** out);
+                            uint32_t pack_tiles, hipStream_t s,
+                            const Schedule** out);
 }  // namespace accspmm

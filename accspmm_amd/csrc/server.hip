@@ -253,7 +253,7 @@ extern "C" accspmm_status accspmm_server_register(accspmm_server* s,
   if (st == ACCSPMM_OK) {
     std::lock_guard<std::mutex> hl(h->mu);
     const Schedule* sch = nullptr;
-    st = get_schedule(h, t.split_tiles, s->setup, &sch);
+    st = get_schedule(h, t.split_tiles, t.pack_tiles, s->setup, &sch);
   }
   if (st != ACCSPMM_OK) {
     accspmm_metcf_destroy(h);

@@ -42,7 +42,7 @@ namespace accspmm {
 namespace {
 using namespace dev;
 
-template <int FORM, int NB, int RW>
+template <int FORM, int NB, int RW, bool PACKED>
 __global__ __launch_bounds__(512) void spmm_kernel(SpmmArgs g) {
   const uint32_t wpb = blockDim.x >> 6;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(512) void spmm_kernel(SpmmArgs g) {
   }
   if (unit >= g.num_units) return;
   const WorkUnit u = g.units[unit];
-  run_unit<FORM, NB, RW>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows);
+  run_unit<FORM, NB, RW, PACKED>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows);
 }
 
 // B conversion for the BF16 forms: zero-padded to ldx >= K columns.
@@ -92,122 +92,109 @@ __global__ void convert_b_bf16(const float* __restrict__ B, uint32_t N,
     }
 }
 
-// ---- schedule construction (one thread per window) ----------------------
-__global__ void k_unit_counts(const uint32_t* __restrict__ rwo, uint32_t W,
-                              uint32_t split, uint32_t* __restrict__ nunits,
-                              uint32_t* __restrict__ npieces,
-                              uint32_t* __restrict__ nsplit) {
-  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < W;
-       w += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t tw = rwo[w + 1] - rwo[w];
-    uint32_t nu = tw > split ? (tw + split - 1) / split : 1u;
-    nunits[w] = nu;
-    npieces[w] = nu > 1 ? nu : 0u;
-    nsplit[w] = nu > 1 ? 1u : 0u;
-  }
-}
-
-__global__ void k_unit_write(const uint32_t* __restrict__ rwo, uint32_t W,
-                             const uint32_t* __restrict__ unit_off,
-                             const uint32_t* __restrict__ piece_off,
-                             const uint32_t* __restrict__ split_off,
-                             WorkUnit* __restrict__ units,
-                             uint32_t* __restrict__ piece_info) {
-  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < W;
-       w += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t tb = rwo[w], te = rwo[w + 1];
-    const uint32_t nu = unit_off[w + 1] - unit_off[w];
-    const uint32_t base = unit_off[w];
-    if (nu == 1) {
-      units[base] = WorkUnit{(uint32_t)w, tb, te, -1};
-      continue;
-    }
-    // balanced pieces, boundaries on 4-tile groups where possible
-    const uint32_t tw = te - tb;
-    const uint32_t first = piece_off[w];
-    for (uint32_t p = 0; p < nu; ++p) {
-      uint32_t a = (uint32_t)((uint64_t)tw * p / nu);
-      uint32_t b = (uint32_t)((uint64_t)tw * (p + 1) / nu);
-      a = p == 0 ? 0 : min(tw, (a + 3u) & ~3u);
-      b = p + 1 == nu ? tw : min(tw, (b + 3u) & ~3u);
-      units[base + p] = WorkUnit{(uint32_t)w, tb + a, tb + max(a, b),
-                                 (int32_t)(first + p)};
-      piece_info[3ull * (first + p)] = first;
-      piece_info[3ull * (first + p) + 1] = nu;
-      piece_info[3ull * (first + p) + 2] = split_off[w];
-    }
-  }
-}
-
 }  // namespace
 
+// Work-unit schedule, built on the host in one pass over rowWindowOffset
+// (cached per handle and (split, pack) pair):
+//   * a window with more than `split_tiles` tiles becomes ceil(T_w/split)
+//     balanced pieces (boundaries on 4-tile groups), each with a partial
+//     slot; the pieces of a window share one arrival counter;
+//   * other windows are packed, in order, into units of at least
+//     `pack_tiles` tiles (a heavy window closes the open unit first); runs
+//     of empty windows form units of their own, which write zero rows.
 accspmm_status get_schedule(const accspmm_metcf* h, uint32_t split_tiles,
-                            hipStream_t s, const Schedule** out) {
+                            uint32_t pack_tiles, hipStream_t s,
+                            const Schedule** out) {
   if (split_tiles == 0) split_tiles = 0xFFFFFFFFu;
-  auto it = h->schedules.find(split_tiles);
+  const uint64_t key = ((uint64_t)split_tiles << 32) | pack_tiles;
+  auto it = h->schedules.find(key);
   if (it != h->schedules.end()) {
     *out = it->second.get();
     return ACCSPMM_OK;
   }
   auto sch = std::make_unique<Schedule>();
   sch->split_tiles = split_tiles;
-  const uint32_t W = h->W;
-  DevBuf nunits, npieces, nsplit, uoff, poff, soff;
-  ACC_TRY(nunits.alloc(((uint64_t)W + 1) * 4));
-  ACC_TRY(npieces.alloc(((uint64_t)W + 1) * 4));
-  ACC_TRY(nsplit.alloc(((uint64_t)W + 1) * 4));
-  ACC_TRY(uoff.alloc(((uint64_t)W + 1) * 4));
-  ACC_TRY(poff.alloc(((uint64_t)W + 1) * 4));
-  ACC_TRY(soff.alloc(((uint64_t)W + 1) * 4));
-  ACC_HIP(hipMemsetAsync(nunits.p, 0, ((uint64_t)W + 1) * 4, s));
-  ACC_HIP(hipMemsetAsync(npieces.p, 0, ((uint64_t)W + 1) * 4, s));
-  ACC_HIP(hipMemsetAsync(nsplit.p, 0, ((uint64_t)W + 1) * 4, s));
-  uint32_t nb = (uint32_t)std::min<uint64_t>((W + 255) / 256, 65535);
-  hipLaunchKernelGGL(k_unit_counts, dim3(std::max(nb, 1u)), dim3(256), 0, s,
-                     h->rwo.as<uint32_t>(), W, split_tiles,
-                     nunits.as<uint32_t>(), npieces.as<uint32_t>(),
-                     nsplit.as<uint32_t>());
-  ACC_HIP(hipGetLastError());
-  ACC_TRY(exclusive_scan_u32(nunits.as<uint32_t>(), uoff.as<uint32_t>(), (uint64_t)W + 1, s));
-  ACC_TRY(exclusive_scan_u32(npieces.as<uint32_t>(), poff.as<uint32_t>(), (uint64_t)W + 1, s));
-  ACC_TRY(exclusive_scan_u32(nsplit.as<uint32_t>(), soff.as<uint32_t>(), (uint64_t)W + 1, s));
-  uint32_t tot[3];
-  ACC_HIP(hipMemcpyAsync(&tot[0], uoff.as<uint32_t>() + W, 4, hipMemcpyDeviceToHost, s));
-  ACC_HIP(hipMemcpyAsync(&tot[1], poff.as<uint32_t>() + W, 4, hipMemcpyDeviceToHost, s));
-  ACC_HIP(hipMemcpyAsync(&tot[2], soff.as<uint32_t>() + W, 4, hipMemcpyDeviceToHost, s));
-  ACC_HIP(hipStreamSynchronize(s));
-  sch->num_units = tot[0];
-  sch->num_pieces = tot[1];
-  sch->num_split_windows = tot[2];
-  ACC_TRY(sch->units.alloc((uint64_t)tot[0] * sizeof(WorkUnit)));
-  ACC_TRY(sch->piece_first.alloc((uint64_t)std::max(tot[1], 1u) * 12));
-  hipLaunchKernelGGL(k_unit_write, dim3(std::max(nb, 1u)), dim3(256), 0, s,
-                     h->rwo.as<uint32_t>(), W, uoff.as<uint32_t>(),
-                     poff.as<uint32_t>(), soff.as<uint32_t>(),
-                     sch->units.as<WorkUnit>(), sch->piece_first.as<uint32_t>());
-  ACC_HIP(hipGetLastError());
+  sch->pack_tiles = pack_tiles;
+  const std::vector<uint32_t>& rwo = h->h_rwo;
+  std::vector<WorkUnit> units;
+  std::vector<uint32_t> pinfo;  // per piece: first slot, count, split window
+  units.reserve(h->W / 2 + 16);
+  WorkUnit open{0, 0, 0, 0, -1, 0, 0, 0};
+  auto close_open = [&]() {
+    if (open.nwin) units.push_back(open);
+    open.nwin = 0;
+  };
+  uint32_t pieces = 0, split_windows = 0;
+  for (uint32_t w = 0; w < h->W; ++w) {
+    const uint32_t tb = rwo[w], te = rwo[w + 1], tw = te - tb;
+    if (tw > split_tiles) {
+      close_open();
+      const uint32_t nu = (tw + split_tiles - 1) / split_tiles;
+      for (uint32_t p = 0; p < nu; ++p) {
+        uint32_t a = (uint32_t)((uint64_t)tw * p / nu);
+        uint32_t b = (uint32_t)((uint64_t)tw * (p + 1) / nu);
+        a = p == 0 ? 0 : std::min(tw, (a + 3u) & ~3u);
+        b = p + 1 == nu ? tw : std::min(tw, (b + 3u) & ~3u);
+        b = std::max(a, b);
+        units.push_back(WorkUnit{w, 1, tb + a, tb + b, (int32_t)(pieces + p),
+                                 (b - a + 3) / 4, 0, 0});
+        pinfo.push_back(pieces);
+        pinfo.push_back(nu);
+        pinfo.push_back(split_windows);
+      }
+      pieces += nu;
+      ++split_windows;
+      continue;
+    }
+    // empty windows never sit inside a unit that has tiles
+    if (open.nwin && ((tw == 0) != (open.tile_end == open.tile_begin))) close_open();
+    if (open.nwin == 0) open = WorkUnit{w, 0, tb, tb, -1, 0, 0, 0};
+    open.nwin++;
+    open.tile_end = te;
+    open.ngroups += (tw + 3) / 4;
+    if (open.tile_end - open.tile_begin >= pack_tiles) close_open();
+  }
+  close_open();
+  sch->num_units = (uint32_t)units.size();
+  sch->packed = false;
+  for (const WorkUnit& u : units) sch->packed |= (u.slot < 0 && u.nwin > 1 && u.ngroups > 0);
+  sch->num_pieces = pieces;
+  sch->num_split_windows = split_windows;
+  ACC_TRY(sch->units.alloc(std::max<size_t>(units.size(), 1) * sizeof(WorkUnit)));
+  ACC_TRY(sch->piece_first.alloc(std::max<size_t>(pinfo.size(), 3) * 4));
+  if (!units.empty())
+    ACC_HIP(hipMemcpyAsync(sch->units.p, units.data(), units.size() * sizeof(WorkUnit),
+                           hipMemcpyHostToDevice, s));
+  if (!pinfo.empty())
+    ACC_HIP(hipMemcpyAsync(sch->piece_first.p, pinfo.data(), pinfo.size() * 4,
+                           hipMemcpyHostToDevice, s));
   ACC_HIP(hipStreamSynchronize(s));
   *out = sch.get();
-  h->schedules[split_tiles] = std::move(sch);
+  h->schedules[key] = std::move(sch);
   return ACCSPMM_OK;
 }
 
 namespace {
 
 template <int FORM, int NB, int RW>
-void launch_t(const SpmmArgs& g, uint32_t blocks, uint32_t wpb, hipStream_t s) {
-  hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW>), dim3(blocks), dim3(64 * wpb),
-                     0, s, g);
+void launch_t(const SpmmArgs& g, uint32_t blocks, uint32_t wpb, bool packed,
+              hipStream_t s) {
+  if (packed)
+    hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, true>), dim3(blocks),
+                       dim3(64 * wpb), 0, s, g);
+  else
+    hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, false>), dim3(blocks),
+                       dim3(64 * wpb), 0, s, g);
 }
 
 template <int FORM, int RW>
 accspmm_status launch_nb(uint32_t nb, const SpmmArgs& g, uint32_t blocks,
-                         uint32_t wpb, hipStream_t s) {
+                         uint32_t wpb, bool packed, hipStream_t s) {
   switch (nb) {
-    case 1: launch_t<FORM, 1, RW>(g, blocks, wpb, s); break;
-    case 2: launch_t<FORM, 2, RW>(g, blocks, wpb, s); break;
-    case 4: launch_t<FORM, 4, RW>(g, blocks, wpb, s); break;
-    case 8: launch_t<FORM, 8, RW>(g, blocks, wpb, s); break;
+    case 1: launch_t<FORM, 1, RW>(g, blocks, wpb, packed, s); break;
+    case 2: launch_t<FORM, 2, RW>(g, blocks, wpb, packed, s); break;
+    case 4: launch_t<FORM, 4, RW>(g, blocks, wpb, packed, s); break;
+    case 8: launch_t<FORM, 8, RW>(g, blocks, wpb, packed, s); break;
     default: return ACCSPMM_ERR_UNSUPPORTED;
   }
   return ACCSPMM_OK;
@@ -215,15 +202,15 @@ accspmm_status launch_nb(uint32_t nb, const SpmmArgs& g, uint32_t blocks,
 
 accspmm_status launch_form(int form, uint32_t rw, uint32_t nb,
                            const SpmmArgs& g, uint32_t blocks, uint32_t wpb,
-                           hipStream_t s) {
+                           bool packed, hipStream_t s) {
   if (rw == 8) {
-    if (form == kFP32) return launch_nb<kFP32, 8>(nb, g, blocks, wpb, s);
-    if (form == kSPLIT) return launch_nb<kSPLIT, 8>(nb, g, blocks, wpb, s);
-    return launch_nb<kBF16, 8>(nb, g, blocks, wpb, s);
+    if (form == kFP32) return launch_nb<kFP32, 8>(nb, g, blocks, wpb, packed, s);
+    if (form == kSPLIT) return launch_nb<kSPLIT, 8>(nb, g, blocks, wpb, packed, s);
+    return launch_nb<kBF16, 8>(nb, g, blocks, wpb, packed, s);
   }
-  if (form == kFP32) return launch_nb<kFP32, 16>(nb, g, blocks, wpb, s);
-  if (form == kSPLIT) return launch_nb<kSPLIT, 16>(nb, g, blocks, wpb, s);
-  return launch_nb<kBF16, 16>(nb, g, blocks, wpb, s);
+  if (form == kFP32) return launch_nb<kFP32, 16>(nb, g, blocks, wpb, packed, s);
+  if (form == kSPLIT) return launch_nb<kSPLIT, 16>(nb, g, blocks, wpb, packed, s);
+  return launch_nb<kBF16, 16>(nb, g, blocks, wpb, packed, s);
 }
 
 }  // namespace
@@ -241,6 +228,8 @@ void default_tile(const accspmm_metcf* h, uint32_t K, int form,
   uint32_t s = std::max<uint32_t>(64, 4 * std::max<uint32_t>(mean, 16));
   t->split_tiles = h->max_window_tiles > s ? s : 0;
   t->xcd_slice = 0;
+  // pack small windows so a wave's pipeline runs over >= 16 groups
+  t->pack_tiles = 64;
 }
 
 // phases: bit 0 = convert B (BF16 forms), bit 1 = SpMM kernel
@@ -263,9 +252,10 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
 
   std::lock_guard<std::mutex> lock(h->mu);
   const Schedule* sch = nullptr;
-  ACC_TRY(get_schedule(h, tile.split_tiles, s, &sch));
+  ACC_TRY(get_schedule(h, tile.split_tiles, tile.pack_tiles, s, &sch));
 
   SpmmArgs g{};
+  g.rwo = h->rwo.as<uint32_t>();
   g.tco = h->tco.as<uint32_t>();
   g.bits = h->bits.as<unsigned long long>();
   g.a2b = h->a2b.as<uint32_t>();
@@ -326,7 +316,7 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   }
   if (blocks == 0) return ACCSPMM_OK;
   if (blocks > 0x7FFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
-  ACC_TRY(launch_form(form, h->rw, nb, g, (uint32_t)blocks, wpb, s));
+  ACC_TRY(launch_form(form, h->rw, nb, g, (uint32_t)blocks, wpb, sch->packed, s));
   ACC_HIP(hipGetLastError());
   return ACCSPMM_OK;
 }

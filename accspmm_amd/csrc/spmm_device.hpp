@@ -19,6 +19,7 @@ typedef uint32_t u4 __attribute__((__vector_size__(4 * sizeof(uint32_t))));
 enum : int { kFP32 = 0, kSPLIT = 1, kBF16 = 2 };
 
 struct SpmmArgs {
+  const uint32_t* __restrict__ rwo;
   const uint32_t* __restrict__ tco;
   const unsigned long long* __restrict__ bits;
   const uint32_t* __restrict__ a2b;
@@ -117,6 +118,7 @@ struct Stage {
   BRow<FORM, NB> b[8];
   float a[8];
   uint32_t bmask, amask;
+  uint32_t win;  // row window of the group (wave-uniform)
 };
 
 // Tile metadata a lane needs for its tile of a group: the 8 condensed
@@ -125,13 +127,34 @@ struct Meta {
   uint32_t col[8];
   unsigned long long bw;
   uint32_t off;
+  uint32_t win;  // row window of the group (wave-uniform)
 };
 
+// Position of a 4-tile group (wave-uniform): its window w, first tile t and
+// the end tile of its range.  Groups never span two windows.
+struct GroupPos {
+  uint32_t w, t, tend;
+};
+
+// Advances to the next group of the unit; only called while groups remain.
+// Packed units contain no empty windows, so the next window has tiles.
+__device__ inline void next_group(GroupPos& p, const uint32_t* __restrict__ rwo) {
+  p.t += 4;
+  if (p.t >= p.tend) {
+    ++p.w;
+    p.t = rwo[p.w];
+    p.tend = rwo[p.w + 1];
+  }
+}
+
 template <int RW>
-__device__ inline void load_meta(Meta& mt, const SpmmArgs& g, uint32_t t,
-                                 uint32_t t_last, bool tile_ok, uint32_t m) {
-  // past the unit's last tile: read the last tile, then mask it out
-  const uint32_t tt = tile_ok ? t : t_last;
+__device__ inline void load_meta(Meta& mt, const SpmmArgs& g, const GroupPos& p,
+                                 bool valid, uint32_t q, uint32_t m) {
+  // a tile past the group's range (or an invalid group) reads the group's
+  // first tile and is masked out
+  const uint32_t t = p.t + q;
+  const bool tile_ok = valid && t < p.tend;
+  const uint32_t tt = tile_ok ? t : p.t;
   const u4* cp = reinterpret_cast<const u4*>(g.a2b + (uint64_t)tt * 8);
   const u4 c0 = cp[0], c1 = cp[1];
   uint32_t off = g.tco[tt];
@@ -150,6 +173,7 @@ __device__ inline void load_meta(Meta& mt, const SpmmArgs& g, uint32_t t,
   for (int j = 0; j < 8; ++j) mt.col[j] = tile_ok ? mt.col[j] : kPad;
   mt.bw = tile_ok ? bw : 0ull;
   mt.off = off;
+  mt.win = p.w;
 }
 
 // Gathers the B rows of a tile's 8 columns and decodes the lane's A row.
@@ -176,6 +200,7 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
   }
   st.bmask = bm;
   st.amask = byte;
+  st.win = mt.win;
 }
 
 template <int FORM, int NB, int RW>
@@ -270,63 +295,143 @@ __device__ inline void store_vec(float* p, const float* v, uint32_t nvalid) {
   }
 }
 
+// Writes the accumulated rows of window w to C (split form, RW=8: rows r
+// and r+8 of the MFMA tile hold the A_hi and A_lo halves and are added in
+// place; the caller resets the accumulators afterwards).
+template <int FORM, int NB, int RW>
+__device__ inline void store_window(f4* acc, uint32_t w, uint32_t q,
+                                    uint32_t feat, uint32_t nvalid, float* C,
+                                    uint32_t ldc, uint32_t rows) {
+  if (FORM == kSPLIT && RW == 8) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[b][i] += __shfl_xor(acc[b][i], 32, 64);
+  }
+  if (RW == 8 && q >= 2) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t r = w * RW + 4 * q + i;
+    if (r < rows && nvalid) {
+      float v[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) v[b] = acc[b][i];
+      store_vec<NB>(C + (uint64_t)r * ldc + feat, v, nvalid);
+    }
+  }
+}
+
+template <int NB, int RW>
+__device__ inline void zero_window(uint32_t w, uint32_t q, uint32_t feat,
+                                   uint32_t nvalid, float* C, uint32_t ldc,
+                                   uint32_t rows) {
+  if (RW == 8 && q >= 2) return;
+  float v[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) v[b] = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t r = w * RW + 4 * q + i;
+    if (r < rows && nvalid) store_vec<NB>(C + (uint64_t)r * ldc + feat, v, nvalid);
+  }
+}
+
+// Per-unit output state: the window being accumulated and its accumulators.
+// A group of a later window first flushes the current one (and writes zeros
+// for any empty windows in between).
+template <int FORM, int NB, int RW>
+struct Accum {
+  f4 acc[NB];
+  uint32_t cur_w;
+  bool direct;  // packed unit: flush to C; split piece: keep for the partial
+  uint32_t q, feat, nvalid, ldc, rows;
+  float* C;
+  __device__ inline void reset() {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+  }
+  // flush the current window and start window w (packed units hold no
+  // empty windows except as a whole unit, so no zero-fill is needed here)
+  __device__ inline void move_to(uint32_t w) {
+    if (w == cur_w) return;
+    store_window<FORM, NB, RW>(acc, cur_w, q, feat, nvalid, C, ldc, rows);
+    reset();
+    cur_w = w;
+  }
+};
+
 // Three-stage software pipeline over a unit's 4-tile groups: the metadata
 // of group g+2 and the gathers of group g+1 are in flight while the MFMAs of
 // group g run.  The steady-state loop issues the same loads on every path
 // (order m(g+2), d(g+1), compute(g)), so each wait on the in-order
 // vector-memory counter drains only the stage it needs; the last one to
-// three groups run as straight-line tail code.
-template <int FORM, int NB, int RW, bool FULL>
-__device__ inline void unit_loop(const SpmmArgs& g, f4* acc, uint32_t tb,
-                                 uint32_t te, uint32_t ngroups, uint32_t q,
+// three groups run as straight-line tail code.  sched_barrier keeps each
+// stage's loads issued before the previous stage's consumer code (the
+// scheduler would otherwise hoist the cheap masking and force early waits).
+template <int FORM, int NB, int RW, bool FULL, bool PACKED>
+__device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
+                                 GroupPos pos, uint32_t ngroups, uint32_t q,
                                  uint32_t m, bool a_rows, uint32_t feat,
                                  const void* B, uint32_t ldb) {
   Meta mA, mB;
   Stage<FORM, NB> dA, dB;
-  const uint32_t tl = te - 1;  // ngroups > 0 => the unit has a tile
-  auto tile = [&](uint32_t gi) { return tb + 4 * gi + q; };
-  load_meta<RW>(mA, g, tile(0), tl, tile(0) < te, m);
-  load_meta<RW>(mB, g, tile(1), tl, tile(1) < te, m);
+  auto consume = [&](Stage<FORM, NB>& d) {
+    if (PACKED && out.direct) out.move_to(d.win);
+    compute_stage<FORM, NB, RW>(d, out.acc, m);
+  };
+  // pos: position of the most recently loaded metadata (group gn)
+  uint32_t gn = 0;
+  load_meta<RW>(mA, g, pos, true, q, m);
+  if (ngroups > 1) {
+    next_group(pos, g.rwo);
+    gn = 1;
+  }
+  load_meta<RW>(mB, g, pos, ngroups > 1, q, m);
   load_data<FORM, NB, FULL>(dA, mA, g, m, a_rows, feat, B, ldb);
   uint32_t gi = 0;
-  // sched_barrier keeps each stage's loads issued before the previous
-  // stage's consumer code (the scheduler would otherwise hoist the cheap
-  // masking and force early waits)
   for (; gi + 3 < ngroups; gi += 2) {
-    load_meta<RW>(mA, g, tile(gi + 2), tl, true, m);
+    next_group(pos, g.rwo);  // group gi + 2
+    load_meta<RW>(mA, g, pos, true, q, m);
     load_data<FORM, NB, FULL>(dB, mB, g, m, a_rows, feat, B, ldb);
     __builtin_amdgcn_sched_barrier(0);
-    compute_stage<FORM, NB, RW>(dA, acc, m);
+    consume(dA);
     __builtin_amdgcn_sched_barrier(0);
-    load_meta<RW>(mB, g, tile(gi + 3), tl, tile(gi + 3) < te, m);
+    const bool v3 = gi + 3 < ngroups;
+    if (v3) next_group(pos, g.rwo);  // group gi + 3
+    load_meta<RW>(mB, g, pos, v3, q, m);
     load_data<FORM, NB, FULL>(dA, mA, g, m, a_rows, feat, B, ldb);
     __builtin_amdgcn_sched_barrier(0);
-    compute_stage<FORM, NB, RW>(dB, acc, m);
+    consume(dB);
     __builtin_amdgcn_sched_barrier(0);
   }
+  (void)gn;
   // dA holds group gi, mB the metadata of group gi+1
   const uint32_t left = ngroups - gi;  // 1, 2 or 3
   if (left == 3) {
-    load_meta<RW>(mA, g, tile(gi + 2), tl, tile(gi + 2) < te, m);
+    next_group(pos, g.rwo);  // group gi + 2
+    load_meta<RW>(mA, g, pos, true, q, m);
     load_data<FORM, NB, FULL>(dB, mB, g, m, a_rows, feat, B, ldb);
     __builtin_amdgcn_sched_barrier(0);
-    compute_stage<FORM, NB, RW>(dA, acc, m);
+    consume(dA);
     __builtin_amdgcn_sched_barrier(0);
     load_data<FORM, NB, FULL>(dA, mA, g, m, a_rows, feat, B, ldb);
     __builtin_amdgcn_sched_barrier(0);
-    compute_stage<FORM, NB, RW>(dB, acc, m);
-    compute_stage<FORM, NB, RW>(dA, acc, m);
+    consume(dB);
+    consume(dA);
   } else if (left == 2) {
     load_data<FORM, NB, FULL>(dB, mB, g, m, a_rows, feat, B, ldb);
     __builtin_amdgcn_sched_barrier(0);
-    compute_stage<FORM, NB, RW>(dA, acc, m);
-    compute_stage<FORM, NB, RW>(dB, acc, m);
+    consume(dA);
+    consume(dB);
   } else {
-    compute_stage<FORM, NB, RW>(dA, acc, m);
+    consume(dA);
   }
 }
 
-template <int FORM, int NB, int RW>
+// PACKED: the schedule may hold multi-window units (the loop then flushes C
+// at window changes); without it every direct unit is one window and the
+// loop carries no flush code (fewer registers).
+template <int FORM, int NB, int RW, bool PACKED>
 __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
                          const void* B, uint32_t ldb, float* C, uint32_t ldc,
                          uint32_t K, uint32_t rows) {
@@ -342,46 +447,47 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
   // only in the split form, as the low halves)
   const uint32_t m = n;
   const bool a_rows = RW == 16 || m < 8 || FORM == kSPLIT;
+  const uint32_t nvalid = feat < K ? min((uint32_t)NB, K - feat) : 0u;
 
-  f4 acc[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+  Accum<FORM, NB, RW> out;
+  out.reset();
+  out.cur_w = u.window;
+  out.direct = u.slot < 0;
+  out.q = q;
+  out.feat = feat;
+  out.nvalid = nvalid;
+  out.C = C;
+  out.ldc = ldc;
+  out.rows = rows;
 
-  const uint32_t tb = u.tile_begin, te = u.tile_end;
-  const uint32_t ngroups = (te - tb + 3) / 4;
+  const uint32_t ngroups = u.ngroups;
   if (ngroups > 0) {
+    GroupPos pos{u.window, u.tile_begin, u.tile_end};
+    // packed unit: group ranges are whole windows (its first window is
+    // not empty when it has groups)
+    if (u.slot < 0) pos.tend = g.rwo[pos.w + 1];
     if (full)
-      unit_loop<FORM, NB, RW, true>(g, acc, tb, te, ngroups, q, m, a_rows, feat, B, ldb);
+      unit_loop<FORM, NB, RW, true, PACKED>(g, out, pos, ngroups, q, m, a_rows, feat, B, ldb);
     else
-      unit_loop<FORM, NB, RW, false>(g, acc, tb, te, ngroups, q, m, a_rows, feat, B, ldb);
+      unit_loop<FORM, NB, RW, false, PACKED>(g, out, pos, ngroups, q, m, a_rows, feat, B, ldb);
   }
 
-  // combine rows r and r+8 (split form, RW=8): lanes l and l^32
+  if (u.slot < 0) {
+    // last window of the unit; a unit of empty windows writes zero rows
+    store_window<FORM, NB, RW>(out.acc, out.cur_w, q, feat, nvalid, C, ldc, rows);
+    for (uint32_t e = out.cur_w + 1; e < u.window + u.nwin; ++e)
+      zero_window<NB, RW>(e, q, feat, nvalid, C, ldc, rows);
+    return;
+  }
+  // split piece: write partial (RW rows x kc_padded floats per slot)
+  f4* acc = out.acc;
   if (FORM == kSPLIT && RW == 8) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[b][i] += __shfl_xor(acc[b][i], 32, 64);
   }
-  // rows held by this lane: 4q + i
   const bool store_lane = RW == 16 || q < 2;
-  const uint32_t nvalid = feat < K ? min((uint32_t)NB, K - feat) : 0u;
-  if (u.slot < 0) {
-    if (store_lane) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        uint32_t r = u.window * RW + 4 * q + i;
-        if (r < rows && nvalid) {
-          float v[NB];
-#pragma unroll
-          for (int b = 0; b < NB; ++b) v[b] = acc[b][i];
-          store_vec<NB>(C + (uint64_t)r * ldc + feat, v, nvalid);
-        }
-      }
-    }
-    return;
-  }
-  // split piece: write partial (RW rows x kc_padded floats per slot)
   float* part = g.partials + (uint64_t)u.slot * RW * g.kc_padded;
   if (store_lane) {
 #pragma unroll
@@ -420,7 +526,7 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
         v[b] += __hip_atomic_load(src + b, __ATOMIC_RELAXED,
                                   __HIP_MEMORY_SCOPE_AGENT);
     }
-    uint32_t r = u.window * RW + 4 * q + i;
+    const uint32_t r = u.window * RW + 4 * q + i;
     if (r < rows && nvalid) store_vec<NB>(C + (uint64_t)r * ldc + feat, v, nvalid);
   }
 }

@@ -51,7 +51,7 @@ void load_file(const char* path) {
   std::ifstream in(path);
   if (!in) return;
   std::string line;
-  // one entry per line: "key": [nb, wpb, split, xcd],
+  // one entry per line: "key": [nb, wpb, split, xcd, pack],
   while (std::getline(in, line)) {
     size_t a = line.find('"'), b = line.find('"', a + 1);
     size_t l = line.find('['), r = line.find(']');
@@ -59,10 +59,10 @@ void load_file(const char* path) {
         l == std::string::npos || r == std::string::npos)
       continue;
     std::string key = line.substr(a + 1, b - a - 1);
-    unsigned v[4];
-    if (std::sscanf(line.c_str() + l + 1, "%u , %u , %u , %u", &v[0], &v[1],
-                    &v[2], &v[3]) == 4)
-      g_cache[key] = accspmm_tile{v[0], v[1], v[2], v[3]};
+    unsigned v[5];
+    if (std::sscanf(line.c_str() + l + 1, "%u , %u , %u , %u , %u", &v[0], &v[1],
+                    &v[2], &v[3], &v[4]) == 5)
+      g_cache[key] = accspmm_tile{v[0], v[1], v[2], v[3], v[4]};
   }
 }
 
@@ -74,7 +74,8 @@ void save_file(const char* path) {
   for (auto& kv : g_cache) {
     out << "  \"" << kv.first << "\": [" << kv.second.feat_per_lane << ", "
         << kv.second.waves_per_block << ", " << kv.second.split_tiles << ", "
-        << kv.second.xcd_slice << "]" << (++i < g_cache.size() ? "," : "")
+        << kv.second.xcd_slice << ", " << kv.second.pack_tiles << "]"
+        << (++i < g_cache.size() ? "," : "")
         << "\n";
   }
   out << "}\n";
@@ -134,12 +135,14 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
   for (uint32_t nb : {1u, 2u, 4u, 8u}) {
     if (nb > 1 && 16 * nb > 2 * ((K + 15) / 16 * 16)) continue;
     const uint32_t n_kc = (K + 16 * nb - 1) / (16 * nb);
-    for (uint32_t wpb : {2u, 4u, 8u})
+    for (uint32_t wpb : {1u, 2u, 4u})
       for (uint32_t sp : splits)
-        for (uint32_t xcd : {0u, 1u}) {
-          if (xcd && (n_kc == 1 || (8 % n_kc) != 0)) continue;
-          cands.push_back(accspmm_tile{nb, wpb, sp, xcd});
-        }
+        for (uint32_t xcd : {0u, 1u})
+          for (uint32_t pack : {0u, 64u}) {
+            if (xcd && (n_kc == 1 || (8 % n_kc) != 0)) continue;
+            if (pack && mean >= 64) continue;  // windows already long
+            cands.push_back(accspmm_tile{nb, wpb, sp, xcd, pack});
+          }
   }
   hipEvent_t e0, e1;
   ACC_HIP(hipEventCreate(&e0));

@@ -28,15 +28,25 @@ def test_mix_equals_alone():
     gids = [srv.register(c) for c in pool]
     graphs = [acc.DeviceGraph(c) for c in pool]
     rng = np.random.default_rng(2)
-    jobs = []
+    reqs = []
     for i in range(40):
         j = int(rng.integers(len(pool)))
         K = int(rng.choice([32, 128]))
         B = torch.from_numpy(acc.gen_dense(100 + i, pool[j].cols, K)).cuda()
         Cm = torch.full((pool[j].rows, K), float("nan"), device="cuda")
+        reqs.append((j, B, Cm))
+    torch.cuda.synchronize()
+    jobs = []
+    # the first 20 queue up while admission is paused, so they must share
+    # launches; the rest arrive while those launches are on the GPU
+    srv.pause(True)
+    for j, B, Cm in reqs[:20]:
+        jobs.append((j, B, Cm, srv.submit(gids[j], B, Cm)))
+    srv.pause(False)
+    for i, (j, B, Cm) in enumerate(reqs[20:]):
         jobs.append((j, B, Cm, srv.submit(gids[j], B, Cm)))
         if i % 7 == 0:
-            time.sleep(0.002)  # arrivals while earlier launches are running
+            time.sleep(0.001)
     srv.drain()
     for j, B, Cm, rid in jobs:
         lat = srv.poll(rid)
@@ -70,8 +80,10 @@ def test_one_per_launch_baseline_matches():
 def test_budget_queue_and_unknown_graph():
     import torch
     c = acc.gen_chung_lu(4000, 16.0, 2.2, 9, 0)
-    probe = acc.DeviceGraph(c)
-    gbytes = probe.info.device_bytes
+    probe = acc.Server()
+    probe.register(c)
+    gbytes = probe.stats().resident_bytes
+    probe.close()
     # a budget that holds the graph but not a second copy
     srv = acc.Server(hbm_budget_bytes=int(gbytes * 1.5) + (8 << 20), max_queue=2,
                      max_inflight=1)

@@ -59,12 +59,13 @@ def test_forms_random(form, K):
 @pytest.mark.parametrize("nb", [1, 2, 4, 8])
 @pytest.mark.parametrize("split", [0, 4])
 @pytest.mark.parametrize("xcd", [0, 1])
-def test_tile_variants(form, nb, split, xcd):
+@pytest.mark.parametrize("pack", [0, 64])
+def test_tile_variants(form, nb, split, xcd, pack):
     rng = np.random.default_rng(nb * 10 + split)
     rows, cols = 777, 900
     rp, ci, va = random_csr(rng, rows, cols, 0.02, empty_rows=0.05, heavy_rows=4)
     csr = acc.CSR(rows, cols, rp, ci, va)
-    tile = acc.Tile(nb, 4, split, xcd)
+    tile = acc.Tile(nb, 4, split, xcd, pack)
     mx, fr, _ = run_case(csr, 128, form, tile=tile, seed=3)
     assert mx <= TOL[form], (mx, fr)
 
@@ -92,6 +93,27 @@ def test_stencil(form):
     csr = acc.gen_stencil27(24, 13)
     mx, fr, _ = run_case(csr, 128, form, seed=14)
     assert mx <= TOL_HIGH_DEGREE[form], (mx, fr)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+@pytest.mark.parametrize("pack", [8, 64, 1000])
+def test_packed_units_with_empty_windows(form, pack):
+    # short windows, runs of empty windows (rows 16..95 = windows 2..11) and
+    # a ragged last window
+    rows, cols = 1003, 700
+    rp0, ci0, va0 = random_csr(np.random.default_rng(pack), rows, cols, 0.004,
+                               empty_rows=0.6)
+    cnt = np.diff(rp0.astype(np.int64))
+    starts = rp0[:-1].astype(np.int64)
+    sel = np.concatenate([np.arange(s0, s0 + c) for r, (s0, c) in enumerate(zip(starts, cnt))
+                          if not 16 <= r < 96])
+    cnt[16:96] = 0
+    rp = np.concatenate([[0], np.cumsum(cnt)]).astype(np.uint32)
+    csr = acc.CSR(rows, cols, rp, ci0[sel], va0[sel])
+    for K in (16, 100):
+        mx, fr, Cg = run_case(csr, K, form, tile=acc.Tile(2, 4, 0, 0, pack), seed=K)
+        assert mx <= TOL[form], (mx, fr)
+        assert np.all(Cg[16:96] == 0)
 
 
 def test_rectangular_and_strided():
