@@ -85,11 +85,11 @@ class MetcfInfo(C.Structure):
 class Tile(C.Structure):
     _fields_ = [("feat_per_lane", C.c_uint32), ("waves_per_block", C.c_uint32),
                 ("split_tiles", C.c_uint32), ("xcd_slice", C.c_uint32),
-                ("pack_tiles", C.c_uint32)]
+                ("pack_tiles", C.c_uint32), ("col_panels", C.c_uint32)]
 
     def as_tuple(self):
         return (self.feat_per_lane, self.waves_per_block, self.split_tiles, self.xcd_slice,
-                self.pack_tiles)
+                self.pack_tiles, self.col_panels)
 
 
 class HostCsr(C.Structure):

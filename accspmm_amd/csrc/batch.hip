@@ -6,11 +6,13 @@
 // bit-identical alone and inside a batch.  The task space is implicit: task
 // t belongs to problem p with prefix[p] <= t < prefix[p+1] and is unit
 // (t - prefix[p]) / n_kc, feature chunk (t - prefix[p]) % n_kc of it.
-// A run is four kernels on one stream: reset of the queue head and of the
-// split-window counters, one conversion pass over every problem's B (BF16
-// forms), and the persistent kernel whose waves pop tasks from the atomic
-// head until it passes the last task (every wave reaches that test, so the
-// grid always drains).  Problems are ordered largest first.
+// A run is four kernels on one stream: reset of the queue head, one
+// conversion pass over every problem's B (BF16 forms), the persistent kernel
+// whose waves pop tasks from the atomic head until it passes the last task
+// (every wave reaches that test, so the grid always drains), and one
+// reduction block per split window of any problem, which sums its piece
+// partials in slot order (as the single-graph call does).  Problems are
+// ordered largest first.
 //
 // BatchLauncher is shared with the serving loop: planning is host
 // arithmetic over cached per-graph state, uploads go through pinned staging
@@ -63,11 +65,12 @@ __global__ __launch_bounds__(256) void k_batch_convert(
     const uint32_t* __restrict__ prefix) {
   const uint32_t p = last_le(prefix, nconv, blockIdx.x);
   const ConvDesc c = d[p];
+  // rows [0, c.rows) of B and the zero row c.rows that padded columns read
   const uint32_t r0 = (blockIdx.x - prefix[p]) * kConvRows;
-  const uint32_t r1 = min(r0 + kConvRows, c.rows);
+  const uint32_t r1 = min(r0 + kConvRows, c.rows + 1);
   for (uint32_t r = r0; r < r1; ++r)
     for (uint32_t k = threadIdx.x; k < c.ldx; k += blockDim.x) {
-      float x = k < c.K ? c.src[(uint64_t)r * c.ldb + k] : 0.0f;
+      float x = k < c.K && r < c.rows ? c.src[(uint64_t)r * c.ldb + k] : 0.0f;
       uint32_t hp = pack_bf16(x, 0.f) & 0xFFFFu;
       if (FORM == 1) {
         uint32_t lp = pack_bf16(x - bf16_lo_to_f32(hp), 0.f) & 0xFFFFu;
@@ -105,6 +108,16 @@ __global__ __launch_bounds__(256) void k_batch_spmm(
   }
 }
 
+template <int RW>
+__global__ __launch_bounds__(256) void k_batch_reduce(
+    const SpmmArgs* __restrict__ descs, const uint32_t* __restrict__ sprefix,
+    uint32_t nprob) {
+  const uint32_t p = last_le(sprefix, nprob, blockIdx.x);
+  const SpmmArgs& g = descs[p];
+  reduce_split_window<RW>(g.split_info, blockIdx.x - sprefix[p], g.partials,
+                          g.kc_padded, g.K, g.C, g.ldc, g.rows);
+}
+
 inline uint64_t align256(uint64_t x) { return (x + 255) / 256 * 256; }
 
 }  // namespace
@@ -126,20 +139,19 @@ accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
   out->kc_padded = out->n_kc * KC;
   {
     std::lock_guard<std::mutex> lock(h->mu);
-    ACC_TRY(get_schedule(h, t.split_tiles, t.pack_tiles, s, &out->sched));
+    ACC_TRY(get_schedule(h, t, s, &out->sched));
   }
   out->tasks = (uint64_t)out->sched->num_units * out->n_kc;
   out->bconv_bytes =
       form == 0 ? 0
-                : align256((uint64_t)h->cols * out->kc_padded * (form == 1 ? 4 : 2));
+                : align256(((uint64_t)h->cols + 1) * out->kc_padded * (form == 1 ? 4 : 2));
   out->partial_bytes =
       align256((uint64_t)out->sched->num_pieces * h->rw * out->kc_padded * 4);
-  out->counter_words = (uint64_t)out->sched->num_split_windows * out->n_kc;
   return ACCSPMM_OK;
 }
 
 uint64_t ProblemPlan::workspace_bytes() const {
-  return bconv_bytes + partial_bytes + align256(counter_words * 4);
+  return bconv_bytes + partial_bytes;
 }
 
 BatchLauncher::~BatchLauncher() {
@@ -172,23 +184,22 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
     return P[a].graph->T * (uint64_t)P[a].K > P[b].graph->T * (uint64_t)P[b].K;
   });
-  // workspace layout: [partials + converted B per problem][counters]
-  uint64_t ws = 0, cnt_words = 0, tot = 0;
+  // workspace layout: [partials + converted B per problem]
+  uint64_t ws = 256, tot = 0, tot_split = 0;
   for (uint32_t i = 0; i < n; ++i) {
     ws += plans[i].bconv_bytes + plans[i].partial_bytes;
-    cnt_words += plans[i].counter_words;
     tot += plans[i].tasks;
+    tot_split += plans[i].sched->num_split_windows;
   }
-  if (tot > 0xFFFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
-  const uint64_t cnt_off = ws;
-  ws += align256(cnt_words * 4) + 256;
-  // host staging: descs | prefix (n+1) | conv descs | conv prefix (n+1)
+  if (tot > 0xFFFFFFFFull || tot_split > 0x7FFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
+  // host staging: descs | prefix (n+1) | conv descs | conv prefix (n+1) |
+  // split prefix (n+1)
   const size_t d_bytes = sizeof(dev::SpmmArgs) * n;
   const size_t p_bytes = 4 * ((size_t)n + 1);
   const size_t c_bytes = sizeof(ConvDesc) * n;
   const size_t off_p = align256(d_bytes), off_c = off_p + align256(p_bytes),
-               off_cp = off_c + align256(c_bytes);
-  const size_t stage = off_cp + align256(p_bytes);
+               off_cp = off_c + align256(c_bytes), off_sp = off_cp + align256(p_bytes);
+  const size_t stage = off_sp + align256(p_bytes);
   // the staging buffer is reused: wait until its previous upload consumed it
   if (upload_event_valid) ACC_HIP(hipEventSynchronize(upload_event));
   if (!upload_event)
@@ -202,8 +213,9 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
   auto* PF = reinterpret_cast<uint32_t*>(hb + off_p);
   auto* CV = reinterpret_cast<ConvDesc*>(hb + off_c);
   auto* CP = reinterpret_cast<uint32_t*>(hb + off_cp);
-  uint64_t bc = 0, cw = 0;
-  uint32_t run = 0, cblocks = 0, nconv = 0;
+  auto* SP = reinterpret_cast<uint32_t*>(hb + off_sp);
+  uint64_t bc = 0;
+  uint32_t run = 0, cblocks = 0, nconv = 0, nsplit = 0;
   char* A = arena.as<char>();
   for (uint32_t j = 0; j < n; ++j) {
     const uint32_t i = order[j];
@@ -218,7 +230,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     g.a2b = h->a2b.as<uint32_t>();
     g.data = h->data.as<float>();
     g.units = q.sched->units.as<WorkUnit>();
-    g.piece_info = q.sched->piece_first.as<uint32_t>();
+    g.split_info = q.sched->split_info.as<uint32_t>();
     g.C = p.C;
     g.ldc = p.ldc;
     g.rows = h->rows;
@@ -228,10 +240,11 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     g.kc_padded = q.kc_padded;
     g.rw = h->rw;
     g.nb = q.nb;
+    g.zero_val = h->nnz;
+    g.zero_row = h->cols;
+    g.pad_tile = h->T;
     g.partials = reinterpret_cast<float*>(A + bc);
     bc += q.partial_bytes;
-    g.counters = reinterpret_cast<uint32_t*>(A + cnt_off) + cw;
-    cw += q.counter_words;
     if (form == 0) {
       g.B = p.B;
       g.ldb = p.ldb;
@@ -242,14 +255,17 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
       g.ldb = q.kc_padded;
       CV[nconv] = ConvDesc{p.B, dst, p.ldb, p.K, h->cols, q.kc_padded};
       CP[nconv] = cblocks;
-      cblocks += (h->cols + kConvRows - 1) / kConvRows;
+      cblocks += (h->cols + 1 + kConvRows - 1) / kConvRows;
       ++nconv;
     }
     PF[j] = run;
     run += (uint32_t)q.tasks;
+    SP[j] = nsplit;
+    nsplit += q.sched->num_split_windows;
   }
   PF[n] = run;
   CP[nconv] = cblocks;
+  SP[n] = nsplit;
   ACC_HIP(hipMemcpyAsync(dev_meta.p, pinned, stage, hipMemcpyHostToDevice, s));
   ACC_HIP(hipEventRecord(upload_event, s));
   upload_event_valid = true;
@@ -257,8 +273,8 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
   prefix = reinterpret_cast<uint32_t*>(dev_meta.as<char>() + off_p);
   conv = dev_meta.as<char>() + off_c;
   conv_prefix = reinterpret_cast<uint32_t*>(dev_meta.as<char>() + off_cp);
-  counters = reinterpret_cast<uint32_t*>(A + cnt_off);
-  counter_words = cw;
+  split_prefix = reinterpret_cast<uint32_t*>(dev_meta.as<char>() + off_sp);
+  total_split = nsplit;
   nprob = n;
   num_conv = nconv;
   conv_blocks = cblocks;
@@ -276,12 +292,6 @@ accspmm_status BatchLauncher::launch(hipStream_t s) const {
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, head.as<uint32_t>(),
                      (uint64_t)1, 0u);
   ACC_HIP(hipGetLastError());
-  if (counter_words) {
-    uint32_t blocks = (uint32_t)std::min<uint64_t>((counter_words + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, counters,
-                       counter_words, 0u);
-    ACC_HIP(hipGetLastError());
-  }
   if (form != 0 && conv_blocks) {
     if (form == 1)
       hipLaunchKernelGGL(k_batch_convert<1>, dim3(conv_blocks), dim3(256), 0, s,
@@ -304,6 +314,15 @@ accspmm_status BatchLauncher::launch(hipStream_t s) const {
     else hipLaunchKernelGGL((k_batch_spmm<2, 16>), g, blk, 0, s, descs, prefix, nprob, H);
   }
   ACC_HIP(hipGetLastError());
+  if (total_split) {
+    if (rw == 8)
+      hipLaunchKernelGGL(k_batch_reduce<8>, dim3(total_split), blk, 0, s, descs,
+                         split_prefix, nprob);
+    else
+      hipLaunchKernelGGL(k_batch_reduce<16>, dim3(total_split), blk, 0, s, descs,
+                         split_prefix, nprob);
+    ACC_HIP(hipGetLastError());
+  }
   return ACCSPMM_OK;
 }
 

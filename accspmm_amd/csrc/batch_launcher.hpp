@@ -16,7 +16,6 @@ struct ProblemPlan {
   uint64_t tasks = 0;          // units x feature chunks
   uint64_t bconv_bytes = 0;    // converted B (BF16 forms)
   uint64_t partial_bytes = 0;  // split-window partial sums
-  uint64_t counter_words = 0;  // split-window arrival counters
   uint64_t workspace_bytes() const;
 };
 
@@ -40,10 +39,10 @@ struct BatchLauncher {
   uint32_t* prefix = nullptr;
   void* conv = nullptr;
   uint32_t* conv_prefix = nullptr;
-  uint32_t* counters = nullptr;
-  uint64_t counter_words = 0;
+  uint32_t* split_prefix = nullptr;  // split windows before each problem
   uint64_t total_tasks = 0;
   uint32_t nprob = 0, num_conv = 0, conv_blocks = 0, grid = 0;
+  uint32_t total_split = 0;          // split windows of all problems
   BatchLauncher() = default;
   BatchLauncher(const BatchLauncher&) = delete;
   BatchLauncher& operator=(const BatchLauncher&) = delete;

@@ -427,12 +427,17 @@ accspmm_status build_impl(const accspmm_csr_view* csr,
 
   // 6. sparseA2B + bitmaps
   const uint32_t words = h->words;
-  ACC_TRY(h->a2b.alloc((uint64_t)T * kColWindow * 4));
-  ACC_TRY(h->bits.alloc((uint64_t)T * words * 8));
+  // one extra all-padding tile T (columns kPad, empty bitmap, tcOffset[T] =
+  // nnz): the SpMM kernels read it for group slots past a range's last tile
+  ACC_TRY(h->a2b.alloc(((uint64_t)T + 1) * kColWindow * 4));
+  ACC_TRY(h->bits.alloc(((uint64_t)T + 1) * words * 8));
   ACC_TRY(h->tco.alloc(((uint64_t)T + 1) * 4));
-  ACC_TRY(h->data.alloc((uint64_t)nnz * 4));
-  ACC_HIP(hipMemsetAsync(h->a2b.p, 0xFF, (uint64_t)T * kColWindow * 4, s));
-  ACC_HIP(hipMemsetAsync(h->bits.p, 0, (uint64_t)T * words * 8, s));
+  // one zero past the values: the SpMM kernels read it for bitmap slots
+  // that hold no value (it is not part of the exported array)
+  ACC_TRY(h->data.alloc(((uint64_t)nnz + 1) * 4));
+  ACC_HIP(hipMemsetAsync(static_cast<char*>(h->data.p) + (uint64_t)nnz * 4, 0, 4, s));
+  ACC_HIP(hipMemsetAsync(h->a2b.p, 0xFF, ((uint64_t)T + 1) * kColWindow * 4, s));
+  ACC_HIP(hipMemsetAsync(h->bits.p, 0, ((uint64_t)T + 1) * words * 8, s));
   DevBuf tile_of;
   ACC_TRY(tile_of.alloc((uint64_t)nnz * 4));
   if (nnz) {

@@ -7,6 +7,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/accspmm.h"
@@ -68,9 +69,9 @@ struct DevBuf {
 // Work unit of the SpMM kernels: either a run of consecutive whole row
 // windows [window, window + nwin) with their tiles [tile_begin, tile_end)
 // (slot < 0; C rows are written directly, window by window), or one piece
-// of a split window (nwin == 1, slot >= 0: a partial, combined in a fixed
-// order by the last piece to finish).  ngroups counts the unit's 4-tile
-// groups; a group never spans two windows.
+// of a split window (nwin == 1, slot >= 0: a partial, combined in slot order
+// by a second launch).  ngroups counts the unit's 4-tile groups; a group
+// never spans two windows.
 struct alignas(16) WorkUnit {
   uint32_t window;
   uint32_t nwin;
@@ -84,13 +85,13 @@ struct alignas(16) WorkUnit {
 struct Schedule {
   uint32_t split_tiles = 0xFFFFFFFFu;
   uint32_t pack_tiles = 0;
+  uint32_t col_panels = 1;
   uint32_t num_units = 0;
   uint32_t num_split_windows = 0;
   uint32_t num_pieces = 0;  // units with slot >= 0
   bool packed = false;      // some unit spans several non-empty windows
   DevBuf units;             // WorkUnit[num_units]
-  DevBuf piece_first;       // per piece: first slot, piece count, split window id
-  DevBuf counters;          // per split window arrival counter (self-resetting)
+  DevBuf split_info;        // per split window: window, first slot, piece count
 };
 
 }  // namespace accspmm
@@ -107,7 +108,10 @@ struct accspmm_metcf {
   std::string hist_key;         // degree-histogram part of the tuning key
   // lazily built state (guarded by mu; handles are otherwise immutable)
   mutable std::mutex mu;
-  mutable std::map<uint64_t, std::unique_ptr<accspmm::Schedule>> schedules;
+  // keyed by (split_tiles, pack_tiles, col_panels)
+  mutable std::map<std::tuple<uint32_t, uint32_t, uint32_t>,
+                   std::unique_ptr<accspmm::Schedule>>
+      schedules;
   mutable accspmm::DevBuf bconv;    // converted-B workspace (BF16 forms)
   mutable accspmm::DevBuf partials; // split-window partial sums
   uint64_t device_bytes() const {
@@ -123,7 +127,6 @@ accspmm_status exclusive_scan_u32(const uint32_t* in, uint32_t* out,
 accspmm_status inclusive_scan_u32(const uint32_t* in, uint32_t* out,
                                   uint64_t n, hipStream_t s);
 // spmm.hip
-accspmm_status get_schedule(const accspmm_metcf* h, uint32_t split_tiles,
-                            uint32_t pack_tiles, hipStream_t s,
-                            const Schedule** out);
+accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
+                            hipStream_t s, const Schedule** out);
 }  // namespace accspmm

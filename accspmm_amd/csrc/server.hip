@@ -3,9 +3,9 @@
 // No reference counterpart.  Graphs are registered once and stay resident in
 // METCF form; requests (graph id + caller-owned B and C) are queued and a
 // dispatcher thread admits them into persistent batched launches:
-//   * admission: a request's workspace (converted B, split partials,
-//     counters) is charged against the HBM budget together with the resident
-//     graphs; requests are taken in arrival order while they fit;
+//   * admission: a request's workspace (converted B, split partials) is
+//     charged against the HBM budget together with the resident graphs;
+//     requests are taken in arrival order while they fit;
 //   * continuous batching: up to `max_inflight` launches run at once, each
 //     on its own stream; whenever one completes, pending requests (including
 //     those that arrived meanwhile) form the next launch, so new work
@@ -253,7 +253,7 @@ extern "C" accspmm_status accspmm_server_register(accspmm_server* s,
   if (st == ACCSPMM_OK) {
     std::lock_guard<std::mutex> hl(h->mu);
     const Schedule* sch = nullptr;
-    st = get_schedule(h, t.split_tiles, t.pack_tiles, s->setup, &sch);
+    st = get_schedule(h, t, s->setup, &sch);
   }
   if (st != ACCSPMM_OK) {
     accspmm_metcf_destroy(h);

@@ -27,10 +27,16 @@
 //              gives all four products; rows r and r+8 are added at the end.
 //
 // Split windows: a window with more than `split_tiles` tiles is cut into
-// balanced pieces; each piece writes a partial, and the last piece to arrive
-// (per-window atomic counter) sums all partials in piece order and writes
-// C, so the result bits do not depend on scheduling.  The counter is reset
-// by that last piece, ready for the next launch.
+// balanced pieces; each piece writes a partial, and k_reduce_split sums a
+// window's partials in piece order and writes C, so the result bits do not
+// depend on scheduling.  The batched kernel runs the same second phase.
+//
+// Column panels: with col_panels P > 1, long windows are also cut where
+// their (column-sorted) tiles cross P equal column ranges of B, and the
+// pieces run range by range over all windows.  A graph whose B does not fit
+// the 4 MiB L2 of an XCD otherwise gathers every B row from the MALL/HBM at
+// random; panel by panel the rows in flight span 1/P of B and are re-read
+// from L2.
 #include <algorithm>
 #include <cstring>
 
@@ -47,12 +53,20 @@ __global__ __launch_bounds__(512) void spmm_kernel(SpmmArgs g) {
   const uint32_t wpb = blockDim.x >> 6;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   uint32_t unit, kc;
-  if (g.xcd_slice) {
+  if (g.xcd_slice == 1) {
     // blocks round-robin over the 8 XCDs: keep feature chunk kc on XCDs
     // x with x % n_kc == kc so each XCD's L2 holds one column slice of B
     const uint32_t x = blockIdx.x & 7u, per = 8u / g.n_kc;
     kc = x % g.n_kc;
     unit = ((blockIdx.x >> 3) * per + x / g.n_kc) * wpb + wave;
+  } else if (g.xcd_slice == 2) {
+    // XCD x (blocks x, x+8, ...) runs the x-th eighth of the work units in
+    // order: neighbouring windows (banded matrices) or one range of column
+    // panels (panel-major schedules) share that XCD's L2
+    const uint32_t per = gridDim.x >> 3;
+    const uint32_t tau = ((blockIdx.x & 7u) * per + (blockIdx.x >> 3)) * wpb + wave;
+    unit = tau / g.n_kc;
+    kc = tau % g.n_kc;
   } else {
     const uint32_t tau = blockIdx.x * wpb + wave;
     unit = tau / g.n_kc;
@@ -63,14 +77,15 @@ __global__ __launch_bounds__(512) void spmm_kernel(SpmmArgs g) {
   run_unit<FORM, NB, RW, PACKED>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows);
 }
 
-// B conversion for the BF16 forms: zero-padded to ldx >= K columns.
+// B conversion for the BF16 forms: zero-padded to ldx >= K columns, plus
+// one all-zero row N that padded tile columns read.
 __global__ void convert_b_split(const float* __restrict__ B, uint32_t N,
                                 uint32_t K, uint32_t ldb,
                                 uint32_t* __restrict__ out, uint32_t ldx) {
-  for (uint32_t r = blockIdx.x; r < N; r += gridDim.x)
+  for (uint32_t r = blockIdx.x; r <= N; r += gridDim.x)
     for (uint32_t k = threadIdx.x; k < ldx; k += blockDim.x) {
       uint32_t v = 0;
-      if (k < K) {
+      if (k < K && r < N) {
         float x = B[(uint64_t)r * ldb + k];
         uint32_t hp = pack_bf16(x, 0.f) & 0xFFFFu;
         float lo = x - bf16_lo_to_f32(hp);
@@ -84,29 +99,69 @@ __global__ void convert_b_split(const float* __restrict__ B, uint32_t N,
 __global__ void convert_b_bf16(const float* __restrict__ B, uint32_t N,
                                uint32_t K, uint32_t ldb,
                                uint16_t* __restrict__ out, uint32_t ldx) {
-  for (uint32_t r = blockIdx.x; r < N; r += gridDim.x)
+  for (uint32_t r = blockIdx.x; r <= N; r += gridDim.x)
     for (uint32_t k = threadIdx.x; k < ldx; k += blockDim.x) {
       uint16_t v = 0;
-      if (k < K) v = (uint16_t)(pack_bf16(B[(uint64_t)r * ldb + k], 0.f) & 0xFFFFu);
+      if (k < K && r < N) v = (uint16_t)(pack_bf16(B[(uint64_t)r * ldb + k], 0.f) & 0xFFFFu);
       out[(uint64_t)r * ldx + k] = v;
     }
 }
 
+// For window w and panel boundary p in [1, P): the number of the window's
+// tiles whose first condensed column lies below column p * pc.  Tiles of a
+// window are in ascending column order, so this is a lower bound search.
+__global__ void k_panel_bounds(const uint32_t* __restrict__ rwo,
+                               const uint32_t* __restrict__ a2b, uint32_t W,
+                               uint32_t P, uint32_t pc,
+                               uint32_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)W * (P - 1)) return;
+  const uint32_t w = (uint32_t)(i / (P - 1)), p = (uint32_t)(i % (P - 1)) + 1;
+  const uint64_t key = (uint64_t)p * pc;
+  const uint32_t base = rwo[w];
+  uint32_t lo = base, hi = rwo[w + 1];
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (a2b[8ull * mid] < key) lo = mid + 1; else hi = mid;
+  }
+  out[i] = lo - base;
+}
+
+// Second phase for split windows (one block per split window).
+template <int RW>
+__global__ __launch_bounds__(256) void k_reduce_split(
+    const uint32_t* __restrict__ sinfo, const float* __restrict__ partials,
+    uint32_t kcp, uint32_t K, float* __restrict__ C, uint32_t ldc, uint32_t rows) {
+  reduce_split_window<RW>(sinfo, blockIdx.x, partials, kcp, K, C, ldc, rows);
+}
+
+// Shortest piece a column panel cut may leave (shorter ones are merged
+// with their neighbour: a piece pays a partial write and read).
+constexpr uint32_t kPanelMinTiles = 8;
+
 }  // namespace
 
 // Work-unit schedule, built on the host in one pass over rowWindowOffset
-// (cached per handle and (split, pack) pair):
-//   * a window with more than `split_tiles` tiles becomes ceil(T_w/split)
-//     balanced pieces (boundaries on 4-tile groups), each with a partial
-//     slot; the pieces of a window share one arrival counter;
+// (cached per handle and (split, pack, panels)):
+//   * with col_panels P > 1, a window of at least 2*kPanelMinTiles tiles is
+//     cut where its tiles cross the column boundaries p*ceil(cols/P)
+//     (rounded to 4-tile groups, short pieces merged); its pieces are
+//     scheduled panel by panel across all windows, so the GPU gathers from
+//     one column range of B at a time;
+//   * a window (or panel piece) with more than `split_tiles` tiles becomes
+//     ceil(T/split) balanced pieces (boundaries on 4-tile groups);
+//   * each piece has a partial slot; the pieces of a window have
+//     consecutive slots, summed in slot order by the second phase;
 //   * other windows are packed, in order, into units of at least
 //     `pack_tiles` tiles (a heavy window closes the open unit first); runs
 //     of empty windows form units of their own, which write zero rows.
-accspmm_status get_schedule(const accspmm_metcf* h, uint32_t split_tiles,
-                            uint32_t pack_tiles, hipStream_t s,
-                            const Schedule** out) {
-  if (split_tiles == 0) split_tiles = 0xFFFFFFFFu;
-  const uint64_t key = ((uint64_t)split_tiles << 32) | pack_tiles;
+// Unit order: whole-window units first, then panel 0's pieces, panel 1's...
+accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
+                            hipStream_t s, const Schedule** out) {
+  const uint32_t split_tiles = tile.split_tiles ? tile.split_tiles : 0xFFFFFFFFu;
+  const uint32_t pack_tiles = tile.pack_tiles;
+  const uint32_t P = std::max<uint32_t>(1, std::min<uint32_t>(tile.col_panels, 1024));
+  const auto key = std::make_tuple(split_tiles, pack_tiles, P);
   auto it = h->schedules.find(key);
   if (it != h->schedules.end()) {
     *out = it->second.get();
@@ -115,9 +170,30 @@ accspmm_status get_schedule(const accspmm_metcf* h, uint32_t split_tiles,
   auto sch = std::make_unique<Schedule>();
   sch->split_tiles = split_tiles;
   sch->pack_tiles = pack_tiles;
+  sch->col_panels = P;
   const std::vector<uint32_t>& rwo = h->h_rwo;
+
+  // panel cut positions (tiles below each boundary), per window
+  std::vector<uint32_t> bounds;
+  const uint32_t pc = (uint32_t)(((uint64_t)h->cols + P - 1) / P);
+  if (P > 1 && h->W > 0 && h->T > 0) {
+    const uint64_t n = (uint64_t)h->W * (P - 1);
+    DevBuf d;
+    ACC_TRY(d.alloc(n * 4));
+    const uint64_t blocks = (n + 255) / 256;
+    if (blocks > 0x7FFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL(k_panel_bounds, dim3((uint32_t)blocks), dim3(256), 0, s,
+                       h->rwo.as<uint32_t>(), h->a2b.as<uint32_t>(), h->W, P,
+                       pc, d.as<uint32_t>());
+    ACC_HIP(hipGetLastError());
+    bounds.resize(n);
+    ACC_HIP(hipMemcpyAsync(bounds.data(), d.p, n * 4, hipMemcpyDeviceToHost, s));
+    ACC_HIP(hipStreamSynchronize(s));
+  }
+
   std::vector<WorkUnit> units;
-  std::vector<uint32_t> pinfo;  // per piece: first slot, count, split window
+  std::vector<std::vector<WorkUnit>> by_panel(P);
+  std::vector<uint32_t> sinfo;  // per split window: window, first slot, count
   units.reserve(h->W / 2 + 16);
   WorkUnit open{0, 0, 0, 0, -1, 0, 0, 0};
   auto close_open = [&]() {
@@ -125,24 +201,64 @@ accspmm_status get_schedule(const accspmm_metcf* h, uint32_t split_tiles,
     open.nwin = 0;
   };
   uint32_t pieces = 0, split_windows = 0;
+  struct Cut { uint32_t a, b, panel; };
+  std::vector<Cut> cuts;
+  std::vector<uint32_t> edge(P + 1);
   for (uint32_t w = 0; w < h->W; ++w) {
     const uint32_t tb = rwo[w], te = rwo[w + 1], tw = te - tb;
-    if (tw > split_tiles) {
-      close_open();
-      const uint32_t nu = (tw + split_tiles - 1) / split_tiles;
-      for (uint32_t p = 0; p < nu; ++p) {
-        uint32_t a = (uint32_t)((uint64_t)tw * p / nu);
-        uint32_t b = (uint32_t)((uint64_t)tw * (p + 1) / nu);
-        a = p == 0 ? 0 : std::min(tw, (a + 3u) & ~3u);
-        b = p + 1 == nu ? tw : std::min(tw, (b + 3u) & ~3u);
-        b = std::max(a, b);
-        units.push_back(WorkUnit{w, 1, tb + a, tb + b, (int32_t)(pieces + p),
-                                 (b - a + 3) / 4, 0, 0});
-        pinfo.push_back(pieces);
-        pinfo.push_back(nu);
-        pinfo.push_back(split_windows);
+    cuts.clear();
+    if (P > 1 && tw >= 2 * kPanelMinTiles) {
+      edge[0] = 0;
+      for (uint32_t p = 1; p < P; ++p) {
+        const uint32_t e = (bounds[(uint64_t)w * (P - 1) + p - 1] + 2u) & ~3u;
+        edge[p] = std::min(tw, std::max(edge[p - 1], e));
       }
-      pieces += nu;
+      edge[P] = tw;
+      uint32_t a = 0;
+      for (uint32_t p = 1; p <= P; ++p) {
+        if (edge[p] - a < kPanelMinTiles && p < P) continue;
+        if (edge[p] > a) {
+          uint32_t panel = 0;
+          while (panel + 1 < P && edge[panel + 1] <= a) ++panel;
+          cuts.push_back(Cut{a, edge[p], panel});
+        }
+        a = edge[p];
+      }
+      if (cuts.size() >= 2 && cuts.back().b - cuts.back().a < kPanelMinTiles) {
+        cuts[cuts.size() - 2].b = cuts.back().b;
+        cuts.pop_back();
+      }
+      if (cuts.size() < 2) cuts.clear();
+    }
+    if (cuts.empty() && tw > split_tiles) cuts.push_back(Cut{0, tw, 0});
+    if (!cuts.empty()) {
+      close_open();
+      // sub-split long pieces; count the window's pieces first
+      auto npieces = [&](uint32_t len) {
+        return (uint32_t)(((uint64_t)len + split_tiles - 1) / split_tiles);
+      };
+      uint32_t nwp = 0;
+      for (const Cut& c : cuts) nwp += npieces(c.b - c.a);
+      uint32_t slot = pieces;
+      for (const Cut& c : cuts) {
+        const uint32_t len = c.b - c.a;
+        const uint32_t nu = npieces(len);
+        std::vector<WorkUnit>& dst = P > 1 && cuts.size() >= 2 ? by_panel[c.panel] : units;
+        for (uint32_t p = 0; p < nu; ++p) {
+          uint32_t a = (uint32_t)((uint64_t)len * p / nu);
+          uint32_t b = (uint32_t)((uint64_t)len * (p + 1) / nu);
+          a = p == 0 ? 0 : std::min(len, (a + 3u) & ~3u);
+          b = p + 1 == nu ? len : std::min(len, (b + 3u) & ~3u);
+          b = std::max(a, b);
+          dst.push_back(WorkUnit{w, 1, tb + c.a + a, tb + c.a + b, (int32_t)slot,
+                                 (b - a + 3) / 4, 0, 0});
+          ++slot;
+        }
+      }
+      sinfo.push_back(w);
+      sinfo.push_back(pieces);
+      sinfo.push_back(nwp);
+      pieces += nwp;
       ++split_windows;
       continue;
     }
@@ -155,18 +271,19 @@ accspmm_status get_schedule(const accspmm_metcf* h, uint32_t split_tiles,
     if (open.tile_end - open.tile_begin >= pack_tiles) close_open();
   }
   close_open();
+  for (auto& v : by_panel) units.insert(units.end(), v.begin(), v.end());
   sch->num_units = (uint32_t)units.size();
   sch->packed = false;
   for (const WorkUnit& u : units) sch->packed |= (u.slot < 0 && u.nwin > 1 && u.ngroups > 0);
   sch->num_pieces = pieces;
   sch->num_split_windows = split_windows;
   ACC_TRY(sch->units.alloc(std::max<size_t>(units.size(), 1) * sizeof(WorkUnit)));
-  ACC_TRY(sch->piece_first.alloc(std::max<size_t>(pinfo.size(), 3) * 4));
   if (!units.empty())
     ACC_HIP(hipMemcpyAsync(sch->units.p, units.data(), units.size() * sizeof(WorkUnit),
                            hipMemcpyHostToDevice, s));
-  if (!pinfo.empty())
-    ACC_HIP(hipMemcpyAsync(sch->piece_first.p, pinfo.data(), pinfo.size() * 4,
+  ACC_TRY(sch->split_info.alloc(std::max<size_t>(sinfo.size(), 3) * 4));
+  if (!sinfo.empty())
+    ACC_HIP(hipMemcpyAsync(sch->split_info.p, sinfo.data(), sinfo.size() * 4,
                            hipMemcpyHostToDevice, s));
   ACC_HIP(hipStreamSynchronize(s));
   *out = sch.get();
@@ -215,21 +332,35 @@ accspmm_status launch_form(int form, uint32_t rw, uint32_t nb,
 
 }  // namespace
 
-// Default tile heuristic: widest feature chunk whose alignment the operands
-// allow, capped so small K does not idle lanes.
+// Default tile, from sweeps on gfx950 (232,965-node power-law graph and
+// 27-point stencil, K = 128):
+//   * NB: the widest feature chunk up to K (BF16 form: 8, the others 4; NB=8
+//     spills registers in the FP32 and split forms);
+//   * one wave per block (smaller blocks balance better; no shape measured
+//     faster);
+//   * long windows (mean >= 64 tiles, e.g. power-law graphs): column panels
+//     when B exceeds 8 MiB (2 panels; 4 above 32 MiB), round-robin XCDs;
+//   * short windows (banded matrices): one contiguous range of windows per
+//     XCD, so neighbouring windows share B rows in that XCD's L2;
+//   * windows much longer than the mean are split, short ones packed.
 void default_tile(const accspmm_metcf* h, uint32_t K, int form,
                   accspmm_tile* t) {
-  uint32_t nb = K >= 128 ? 4 : K >= 64 ? 4 : K >= 32 ? 2 : 1;
-  (void)form;
+  uint32_t nb = K >= 128 && form == kBF16 ? 8 : K >= 64 ? 4 : K >= 32 ? 2 : 1;
   t->feat_per_lane = nb;
-  t->waves_per_block = 4;
-  // split windows much longer than the mean (keeps the tail short)
-  uint32_t mean = h->W ? h->T / h->W : 0;
-  uint32_t s = std::max<uint32_t>(64, 4 * std::max<uint32_t>(mean, 16));
+  t->waves_per_block = 1;
+  const uint32_t mean = h->W ? h->T / h->W : 0;
+  const uint32_t s = std::max<uint32_t>(64, 4 * std::max<uint32_t>(mean, 16));
   t->split_tiles = h->max_window_tiles > s ? s : 0;
-  t->xcd_slice = 0;
-  // pack small windows so a wave's pipeline runs over >= 16 groups
   t->pack_tiles = 64;
+  const uint64_t kp = ((uint64_t)K + 16 * nb - 1) / (16 * nb) * (16 * nb);
+  const uint64_t b_bytes = ((uint64_t)h->cols + 1) * kp * (form == kBF16 ? 2 : 4);
+  if (mean >= 64) {
+    t->xcd_slice = 0;
+    t->col_panels = b_bytes > (32ull << 20) ? 4 : b_bytes > (8ull << 20) ? 2 : 0;
+  } else {
+    t->xcd_slice = 2;
+    t->col_panels = 0;
+  }
 }
 
 // phases: bit 0 = convert B (BF16 forms), bit 1 = SpMM kernel
@@ -252,7 +383,7 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
 
   std::lock_guard<std::mutex> lock(h->mu);
   const Schedule* sch = nullptr;
-  ACC_TRY(get_schedule(h, tile.split_tiles, tile.pack_tiles, s, &sch));
+  ACC_TRY(get_schedule(h, tile, s, &sch));
 
   SpmmArgs g{};
   g.rwo = h->rwo.as<uint32_t>();
@@ -261,7 +392,7 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   g.a2b = h->a2b.as<uint32_t>();
   g.data = h->data.as<float>();
   g.units = sch->units.as<WorkUnit>();
-  g.piece_info = sch->piece_first.as<uint32_t>();
+  g.split_info = sch->split_info.as<uint32_t>();
   g.C = C;
   g.ldc = ldc;
   g.rows = h->rows;
@@ -269,20 +400,16 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   g.num_units = sch->num_units;
   g.n_kc = n_kc;
   g.kc_padded = n_kc * KC;
-  g.xcd_slice = (tile.xcd_slice && (8 % n_kc) == 0) ? 1u : 0u;
+  g.xcd_slice = tile.xcd_slice == 2 ? 2u
+                : (tile.xcd_slice == 1 && (8 % n_kc) == 0) ? 1u : 0u;
+  g.zero_val = h->nnz;
+  g.zero_row = h->cols;
+  g.pad_tile = h->T;
 
   if (sch->num_pieces) {
     const uint64_t pbytes = (uint64_t)sch->num_pieces * h->rw * g.kc_padded * 4;
     ACC_TRY(h->partials.reserve(pbytes));
-    // arrival counters: one per (split window, feature chunk); zero at rest
-    const uint64_t cbytes = (uint64_t)sch->num_split_windows * n_kc * 4;
-    auto* mut = const_cast<Schedule*>(sch);
-    if (mut->counters.bytes < cbytes) {
-      ACC_TRY(mut->counters.alloc(cbytes));
-      ACC_HIP(hipMemsetAsync(mut->counters.p, 0, cbytes, s));
-    }
     g.partials = h->partials.as<float>();
-    g.counters = mut->counters.as<uint32_t>();
   }
 
   if (form == kFP32) {
@@ -291,8 +418,8 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   } else {
     const uint32_t ldx = n_kc * KC;
     const uint64_t es = form == kSPLIT ? 4 : 2;
-    ACC_TRY(h->bconv.reserve((uint64_t)h->cols * ldx * es));
-    uint32_t blocks = std::min<uint32_t>(h->cols, 65535u * 4);
+    ACC_TRY(h->bconv.reserve(((uint64_t)h->cols + 1) * ldx * es));
+    uint32_t blocks = std::min<uint32_t>(h->cols + 1, 65535u * 4);
     if (!(phases & 1)) {
       // converted by an earlier prepare call
     } else if (form == kSPLIT)
@@ -308,9 +435,12 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   if (!(phases & 2)) return ACCSPMM_OK;
 
   uint64_t blocks;
-  if (g.xcd_slice) {
+  if (g.xcd_slice == 1) {
     const uint64_t ub = (sch->num_units + wpb - 1) / wpb;
     blocks = (ub * n_kc + 7) / 8 * 8;
+  } else if (g.xcd_slice == 2) {
+    blocks = ((uint64_t)sch->num_units * n_kc + wpb - 1) / wpb;
+    blocks = (blocks + 7) / 8 * 8;
   } else {
     blocks = ((uint64_t)sch->num_units * n_kc + wpb - 1) / wpb;
   }
@@ -318,6 +448,17 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   if (blocks > 0x7FFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
   ACC_TRY(launch_form(form, h->rw, nb, g, (uint32_t)blocks, wpb, sch->packed, s));
   ACC_HIP(hipGetLastError());
+  if (sch->num_split_windows) {
+    if (h->rw == 8)
+      hipLaunchKernelGGL(k_reduce_split<8>, dim3(sch->num_split_windows), dim3(256), 0, s,
+                         sch->split_info.as<uint32_t>(), g.partials, g.kc_padded, K,
+                         C, ldc, h->rows);
+    else
+      hipLaunchKernelGGL(k_reduce_split<16>, dim3(sch->num_split_windows), dim3(256), 0, s,
+                         sch->split_info.as<uint32_t>(), g.partials, g.kc_padded, K,
+                         C, ldc, h->rows);
+    ACC_HIP(hipGetLastError());
+  }
   return ACCSPMM_OK;
 }
 

@@ -25,11 +25,17 @@ struct SpmmArgs {
   const uint32_t* __restrict__ a2b;
   const float* __restrict__ data;
   const WorkUnit* __restrict__ units;
-  const uint32_t* __restrict__ piece_info;  // per piece: (first slot, count, counter id)
-  uint32_t* __restrict__ counters;
+  const uint32_t* __restrict__ split_info;  // per split window: (window, first slot, count)
   float* __restrict__ partials;
   const void* __restrict__ B;  // FP32: float; SPLIT: u32 (hi|lo<<16); BF16: u16
   float* __restrict__ C;
+  // what padded columns and empty bitmap slots read: BF16 forms: row
+  // zero_row of the converted B (its last row, all zero); FP32 form: row 0
+  // of the caller's B, masked after the load (measured faster on gfx950
+  // than reading a separate zero buffer); A: data[zero_val] (one zero past
+  // the values)
+  uint32_t zero_row, zero_val;
+  uint32_t pad_tile;  // the all-padding tile (index T) past the real tiles
   uint32_t ldb, ldc;
   uint32_t rows, K;
   uint32_t num_units;
@@ -61,15 +67,17 @@ struct BRow {
   uint32_t w[kWords];
 };
 
-// Branch-free gather of one B row segment.  A padded column (kPad) loads row
-// 0; the consumer masks it out (see Stage::bmask).  Every lane issues the same
+// Branch-free gather of one B row segment.  A padded column (kPad) reads the
+// converted B's zero row (BF16 forms) or row 0 of B (FP32 form, masked when
+// the stage is consumed).  Every lane issues the same
 // vector loads and no result is touched until its stage is consumed, which
 // keeps the count of outstanding loads static: the compiler can then wait for
 // one pipeline stage without draining the younger ones.
 template <int FORM, int NB, bool FULL>
 __device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
-                                 uint32_t col, uint32_t feat, uint32_t K) {
-  const uint32_t c = col != kPad ? col : 0u;
+                                 uint32_t col, uint32_t feat, uint32_t K,
+                                 uint32_t zero_row) {
+  const uint32_t c = col != kPad ? col : zero_row;
   if (FORM == kBF16) {
     const uint16_t* p = static_cast<const uint16_t*>(B) + (uint64_t)c * ldb + feat;
     if (NB == 8) {
@@ -83,8 +91,22 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
     } else {
       r.w[0] = *p;
     }
-  } else if (FULL) {
+  } else if (FORM == kSPLIT) {
     const uint32_t* p = static_cast<const uint32_t*>(B) + (uint64_t)c * ldb + feat;
+#pragma unroll
+    for (int i = 0; i < NB / 4; ++i) {
+      u4 v = *reinterpret_cast<const u4*>(p + 4 * i);
+      r.w[4 * i] = v[0]; r.w[4 * i + 1] = v[1];
+      r.w[4 * i + 2] = v[2]; r.w[4 * i + 3] = v[3];
+    }
+    if (NB == 2) {
+      uint2 v = *reinterpret_cast<const uint2*>(p);
+      r.w[0] = v.x; r.w[1] = v.y;
+    } else if (NB == 1) {
+      r.w[0] = *p;
+    }
+  } else if (FULL) {
+    const uint32_t* p = static_cast<const uint32_t*>(B) + (uint64_t)(col != kPad ? col : 0u) * ldb + feat;
     if (NB >= 4) {
 #pragma unroll
       for (int i = 0; i < NB / 4; ++i) {
@@ -100,7 +122,7 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
     }
   } else {
     // feature tail (FP32 form only): clamp inside the row, zero past K
-    const uint32_t* row = static_cast<const uint32_t*>(B) + (uint64_t)c * ldb;
+    const uint32_t* row = static_cast<const uint32_t*>(B) + (uint64_t)(col != kPad ? col : 0u) * ldb;
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       const uint32_t f = feat + i;
@@ -110,14 +132,14 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
   }
 }
 
-// One pipeline stage: everything a lane needs for one 4-tile group.
-// Loaded values are raw: bit j of bmask marks a real column j (else the row
-// is a padding read), bit j of amask a stored value of A in slot j.
+// One pipeline stage: everything a lane needs for one 4-tile group.  Empty
+// slots of A and padded columns of the BF16 forms read zeros; bit j of bmask
+// marks a real column j (the FP32 form masks the others).
 template <int FORM, int NB>
 struct Stage {
   BRow<FORM, NB> b[8];
   float a[8];
-  uint32_t bmask, amask;
+  uint32_t bmask;
   uint32_t win;  // row window of the group (wave-uniform)
 };
 
@@ -150,11 +172,11 @@ __device__ inline void next_group(GroupPos& p, const uint32_t* __restrict__ rwo)
 template <int RW>
 __device__ inline void load_meta(Meta& mt, const SpmmArgs& g, const GroupPos& p,
                                  bool valid, uint32_t q, uint32_t m) {
-  // a tile past the group's range (or an invalid group) reads the group's
-  // first tile and is masked out
+  // a tile past the group's range (or an invalid group) reads the
+  // all-padding tile: kPad columns, empty bitmap, offset nnz
   const uint32_t t = p.t + q;
   const bool tile_ok = valid && t < p.tend;
-  const uint32_t tt = tile_ok ? t : p.t;
+  const uint32_t tt = tile_ok ? t : g.pad_tile;
   const u4* cp = reinterpret_cast<const u4*>(g.a2b + (uint64_t)tt * 8);
   const u4 c0 = cp[0], c1 = cp[1];
   uint32_t off = g.tco[tt];
@@ -169,16 +191,14 @@ __device__ inline void load_meta(Meta& mt, const SpmmArgs& g, const GroupPos& p,
   }
   mt.col[0] = c0[0]; mt.col[1] = c0[1]; mt.col[2] = c0[2]; mt.col[3] = c0[3];
   mt.col[4] = c1[0]; mt.col[5] = c1[1]; mt.col[6] = c1[2]; mt.col[7] = c1[3];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) mt.col[j] = tile_ok ? mt.col[j] : kPad;
-  mt.bw = tile_ok ? bw : 0ull;
+  mt.bw = bw;
   mt.off = off;
   mt.win = p.w;
 }
 
-// Gathers the B rows of a tile's 8 columns and decodes the lane's A row.
-// Every value load is issued (at the row's first value when its bit is
-// clear); masking happens when the stage is consumed.
+// Gathers the B rows of a tile's 8 columns and decodes the lane's A row: a
+// set bitmap bit j loads the row's next stored value, a clear one the zero
+// past the values array.
 template <int FORM, int NB, bool FULL>
 __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
                                  const SpmmArgs& g, uint32_t m, bool a_rows,
@@ -186,9 +206,10 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
   uint32_t bm = 0;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    load_brow<FORM, NB, FULL>(st.b[j], B, ldb, mt.col[j], feat, g.K);
-    bm |= (mt.col[j] != kPad ? 1u : 0u) << j;
+    load_brow<FORM, NB, FULL>(st.b[j], B, ldb, mt.col[j], feat, g.K, g.zero_row);
+    if (FORM == kFP32) bm |= (mt.col[j] != kPad ? 1u : 0u) << j;
   }
+  st.bmask = bm;
   const uint32_t sh = 8u * (m & 7u);
   const uint32_t byte = a_rows ? (uint32_t)(mt.bw >> sh) & 0xFFu : 0u;
   const uint32_t base = mt.off + (sh ? popc64(mt.bw & ((1ull << sh) - 1ull)) : 0u);
@@ -196,25 +217,21 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
   for (int j = 0; j < 8; ++j) {
     const bool on = (byte >> j) & 1u;
     const uint32_t rank = __popc(byte & ((1u << j) - 1u));
-    st.a[j] = g.data[on ? base + rank : mt.off];
+    st.a[j] = g.data[on ? base + rank : g.zero_val];
   }
-  st.bmask = bm;
-  st.amask = byte;
   st.win = mt.win;
 }
 
 template <int FORM, int NB, int RW>
 __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
                                      uint32_t m) {
-  // apply the stage's masks now that its loads have landed
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const bool bv = (st.bmask >> j) & 1u;
-#pragma unroll
-    for (int i = 0; i < BRow<FORM, NB>::kWords; ++i) st.b[j].w[i] = bv ? st.b[j].w[i] : 0u;
-    st.a[j] = ((st.amask >> j) & 1u) ? st.a[j] : 0.0f;
-  }
   if (FORM == kFP32) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const bool bv = (st.bmask >> j) & 1u;
+#pragma unroll
+      for (int i = 0; i < NB; ++i) st.b[j].w[i] = bv ? st.b[j].w[i] : 0u;
+    }
 #pragma unroll
     for (int j = 0; j < 8; ++j)
 #pragma unroll
@@ -498,36 +515,32 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
       store_vec<NB>(part + (uint64_t)(4 * q + i) * g.kc_padded + NB * n + k0, v, NB);
     }
   }
-  const uint32_t* info = g.piece_info + 3ull * (uint32_t)u.slot;
-  const uint32_t first = info[0], count = info[1], cid = info[2] * g.n_kc + kc;
-  __threadfence();
-  uint32_t prev = 0;
-  if (lane == 0)
-    prev = __hip_atomic_fetch_add(g.counters + cid, 1u, __ATOMIC_ACQ_REL,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-  prev = __shfl(prev, 0, 64);
-  if (prev != count - 1) return;
-  __threadfence();
-  if (lane == 0)
-    __hip_atomic_store(g.counters + cid, 0u, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  // last piece: sum partials of pieces first..first+count-1 in order
-  if (!store_lane) return;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    float v[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) v[b] = 0.f;
-    for (uint32_t p = 0; p < count; ++p) {
-      const float* src = g.partials + (uint64_t)(first + p) * RW * g.kc_padded +
-                         (uint64_t)(4 * q + i) * g.kc_padded + NB * n + k0;
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-        v[b] += __hip_atomic_load(src + b, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-    }
-    const uint32_t r = u.window * RW + 4 * q + i;
-    if (r < rows && nvalid) store_vec<NB>(C + (uint64_t)r * ldc + feat, v, nvalid);
+  // the partials are combined by reduce_split_window in a second launch
+}
+
+// Sums the piece partials of split window `sw` in slot order and writes the
+// window's C rows; the block's threads stride over its RW x K outputs.
+// Runs in a launch after the SpMM kernel: the kernel boundary orders the
+// partial writes before these reads, so the SpMM kernel needs no
+// device-scope fence (on gfx950 one writes back and invalidates the XCD's
+// whole L2).
+template <int RW>
+__device__ inline void reduce_split_window(const uint32_t* __restrict__ sinfo,
+                                           uint32_t sw,
+                                           const float* __restrict__ partials,
+                                           uint32_t kcp, uint32_t K,
+                                           float* __restrict__ C, uint32_t ldc,
+                                           uint32_t rows) {
+  const uint32_t w = sinfo[3ull * sw], first = sinfo[3ull * sw + 1],
+                 count = sinfo[3ull * sw + 2];
+  for (uint32_t i = threadIdx.x; i < RW * K; i += blockDim.x) {
+    const uint32_t r = i / K, k = i - r * K;
+    const uint64_t row = (uint64_t)w * RW + r;
+    if (row >= rows) continue;
+    const float* src = partials + ((uint64_t)first * RW + r) * kcp + k;
+    float v = 0.f;
+    for (uint32_t p = 0; p < count; ++p) v += src[(uint64_t)p * RW * kcp];
+    C[row * ldc + k] = v;
   }
 }
 

@@ -3,10 +3,11 @@
 // The reference hard-codes its launch shape: warps per block K/32 or K/64
 // (include/tf32_comp.hpp:19-25), one block per row window or AdpBME group
 // (:27-31).  Here the shape (features per lane NB, waves per block, window
-// splitting threshold, XCD feature slicing) is chosen per (degree histogram,
-// K, form): candidates are timed with HIP events on the caller's stream and
-// the fastest is cached in memory and, optionally, in a JSON file
-// (also read at first use from $ACCSPMM_TUNE_CACHE).
+// splitting threshold, XCD feature slicing, window packing, column panels)
+// is chosen per (degree histogram, K, form): candidates are timed with HIP
+// events on the caller's stream and the fastest is cached in memory and,
+// optionally, in a JSON file (also read at first use from
+// $ACCSPMM_TUNE_CACHE).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,7 +52,7 @@ void load_file(const char* path) {
   std::ifstream in(path);
   if (!in) return;
   std::string line;
-  // one entry per line: "key": [nb, wpb, split, xcd, pack],
+  // one entry per line: "key": [nb, wpb, split, xcd, pack, panels],
   while (std::getline(in, line)) {
     size_t a = line.find('"'), b = line.find('"', a + 1);
     size_t l = line.find('['), r = line.find(']');
@@ -59,10 +60,11 @@ void load_file(const char* path) {
         l == std::string::npos || r == std::string::npos)
       continue;
     std::string key = line.substr(a + 1, b - a - 1);
-    unsigned v[5];
-    if (std::sscanf(line.c_str() + l + 1, "%u , %u , %u , %u , %u", &v[0], &v[1],
-                    &v[2], &v[3], &v[4]) == 5)
-      g_cache[key] = accspmm_tile{v[0], v[1], v[2], v[3], v[4]};
+    unsigned v[6] = {0, 0, 0, 0, 0, 0};
+    const int got = std::sscanf(line.c_str() + l + 1, "%u , %u , %u , %u , %u , %u",
+                                &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]);
+    if (got >= 5)  // entries written before col_panels existed have 5 fields
+      g_cache[key] = accspmm_tile{v[0], v[1], v[2], v[3], v[4], v[5]};
   }
 }
 
@@ -74,7 +76,8 @@ void save_file(const char* path) {
   for (auto& kv : g_cache) {
     out << "  \"" << kv.first << "\": [" << kv.second.feat_per_lane << ", "
         << kv.second.waves_per_block << ", " << kv.second.split_tiles << ", "
-        << kv.second.xcd_slice << ", " << kv.second.pack_tiles << "]"
+        << kv.second.xcd_slice << ", " << kv.second.pack_tiles << ", "
+        << kv.second.col_panels << "]"
         << (++i < g_cache.size() ? "," : "")
         << "\n";
   }
@@ -125,33 +128,19 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
                                            float* best_ms, void* stream) {
   if (!h || !B || !C || K == 0) return ACCSPMM_ERR_INVALID_ARG;
   hipStream_t s = (hipStream_t)stream;
-  std::vector<accspmm_tile> cands;
   const uint32_t mean = h->W ? std::max<uint32_t>(1, h->T / h->W) : 1;
   std::vector<uint32_t> splits = {0};
   for (uint32_t f : {2u, 4u, 8u}) {
     uint32_t sp = std::max<uint32_t>(32, mean * f);
     if (h->max_window_tiles > sp) splits.push_back(sp);
   }
-  for (uint32_t nb : {1u, 2u, 4u, 8u}) {
-    if (nb > 1 && 16 * nb > 2 * ((K + 15) / 16 * 16)) continue;
-    const uint32_t n_kc = (K + 16 * nb - 1) / (16 * nb);
-    for (uint32_t wpb : {1u, 2u, 4u})
-      for (uint32_t sp : splits)
-        for (uint32_t xcd : {0u, 1u})
-          for (uint32_t pack : {0u, 64u}) {
-            if (xcd && (n_kc == 1 || (8 % n_kc) != 0)) continue;
-            if (pack && mean >= 64) continue;  // windows already long
-            cands.push_back(accspmm_tile{nb, wpb, sp, xcd, pack});
-          }
-  }
   hipEvent_t e0, e1;
   ACC_HIP(hipEventCreate(&e0));
   ACC_HIP(hipEventCreate(&e1));
-  float bms = 1e30f;
-  accspmm_tile bt{};
-  for (const auto& c : cands) {
-    accspmm_status st = spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3);
-    if (st != ACCSPMM_OK) continue;  // e.g. alignment not met
+  // median of 5 timed launches after 2 warm-up launches; < 0 if rejected
+  auto time_tile = [&](const accspmm_tile& c) -> float {
+    if (spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3) != ACCSPMM_OK)
+      return -1.f;  // e.g. alignment not met
     for (int i = 0; i < 2; ++i) spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3);
     std::vector<float> t;
     for (int i = 0; i < 5; ++i) {
@@ -164,9 +153,46 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
       t.push_back(ms);
     }
     std::sort(t.begin(), t.end());
-    if (t[2] < bms) {
-      bms = t[2];
+    return t[2];
+  };
+  float bms = 1e30f;
+  accspmm_tile bt{};
+  auto consider = [&](const accspmm_tile& c) {
+    const float ms = time_tile(c);
+    if (ms >= 0.f && ms < bms) {
+      bms = ms;
       bt = c;
+    }
+  };
+  // stage 1: lane width, waves per block, split, XCD slicing, packing
+  for (uint32_t nb : {1u, 2u, 4u, 8u}) {
+    if (nb > 1 && 16 * nb > 2 * ((K + 15) / 16 * 16)) continue;
+    const uint32_t n_kc = (K + 16 * nb - 1) / (16 * nb);
+    for (uint32_t wpb : {1u, 2u, 4u})
+      for (uint32_t sp : splits)
+        for (uint32_t xcd : {0u, 1u, 2u})
+          for (uint32_t pack : {0u, 64u}) {
+            if (xcd == 1 && (n_kc == 1 || (8 % n_kc) != 0)) continue;
+            if (pack && mean >= 64) continue;  // windows already long
+            consider(accspmm_tile{nb, wpb, sp, xcd, pack, 0});
+          }
+  }
+  // stage 2: column panels around the best shape, when the B columns one
+  // XCD gathers exceed its 4 MiB L2
+  if (bms < 1e30f) {
+    const uint32_t KC = 16 * bt.feat_per_lane;
+    const uint32_t n_kc = (K + KC - 1) / KC;
+    const uint64_t es = form == ACCSPMM_FORM_FP32 ? 4 : form == ACCSPMM_FORM_SPLIT_BF16 ? 4 : 2;
+    const uint64_t row_bytes = (bt.xcd_slice == 1 ? KC : (uint64_t)n_kc * KC) * es;
+    if ((uint64_t)h->cols * row_bytes > (4ull << 20)) {
+      const accspmm_tile base = bt;
+      for (uint32_t p : {2u, 4u, 8u, 16u, 32u})
+        for (uint32_t sp : splits) {
+          accspmm_tile c = base;
+          c.col_panels = p;
+          c.split_tiles = sp;
+          consider(c);
+        }
     }
   }
   (void)hipEventDestroy(e0);

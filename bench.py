@@ -211,7 +211,22 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip errors and CPU baseline")
     ap.add_argument("--serve-requests", type=int, default=10000,
                     help="requests per serving run (0 skips the serving benchmark)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        n = 1 << 28  # 1 GiB of float32
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"bench: copied {4 * n} bytes three times")
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -321,7 +336,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "spmm_kernel", "kernel_ms": round(hr["kernel_ms_mean"], 4),
+                     "kernel": "spmm_kernel+k_reduce_split",
+                     "kernel_ms": round(hr["kernel_ms_mean"], 4),
                      "alg_bytes_per_launch": ab},
         "cpu_baseline": cpu,
         "serving": serving,

@@ -129,10 +129,16 @@ typedef struct accspmm_tile {
   uint32_t feat_per_lane;  /* NB: features per lane per B row: 1,2,4,8; KC=16*NB */
   uint32_t waves_per_block;/* 1..8                                           */
   uint32_t split_tiles;    /* windows above this many tiles are split (0: no)  */
-  uint32_t xcd_slice;      /* 1: pin feature chunks to XCDs                   */
+  uint32_t xcd_slice;      /* XCD mapping of work: 0 round-robin, 1 pin
+                              feature chunks to XCDs, 2 one contiguous range
+                              of work units per XCD                          */
   uint32_t pack_tiles;     /* consecutive small windows are packed into one
                               work unit of at least this many tiles (0: one
                               window per unit)                               */
+  uint32_t col_panels;     /* > 1: cut long windows at this many equal column
+                              ranges of B and run the pieces range by range,
+                              so each range of B is gathered while it is
+                              cached (0 or 1: off)                           */
 } accspmm_tile;
 
 /* C[rows x K] = A * B[cols x K].  B (ldb >= K) and C (ldc >= K) are

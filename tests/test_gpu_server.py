@@ -85,7 +85,7 @@ def test_budget_queue_and_unknown_graph():
     gbytes = probe.stats().resident_bytes
     probe.close()
     # a budget that holds the graph but not a second copy
-    srv = acc.Server(hbm_budget_bytes=int(gbytes * 1.5) + (8 << 20), max_queue=2,
+    srv = acc.Server(hbm_budget_bytes=int(gbytes * 1.5), max_queue=2,
                      max_inflight=1)
     gid = srv.register(c)
     with pytest.raises(acc.AccSpMMError) as e:

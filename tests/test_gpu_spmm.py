@@ -71,6 +71,26 @@ def test_tile_variants(form, nb, split, xcd, pack):
 
 
 @pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+@pytest.mark.parametrize("panels,split", [(2, 0), (5, 0), (5, 20), (32, 0), (300, 12)])
+def test_column_panels(form, panels, split):
+    # ~200 tiles per window: every window is cut at the panel boundaries
+    # (300 panels leave pieces below the minimum length, which are merged)
+    rng = np.random.default_rng(panels + split)
+    rows, cols = 603, 5000
+    rp, ci, va = random_csr(rng, rows, cols, 0.05, empty_rows=0.05, heavy_rows=3)
+    csr = acc.CSR(rows, cols, rp, ci, va)
+    for rw in (8, 16):
+        g = acc.DeviceGraph(csr, acc.default_build_opts(rw))
+        for K in (32, 100):
+            tile = acc.Tile(2, 4, split, 0, 64, panels)
+            mx, fr, Cg = run_case(csr, K, form, tile=tile, rw=rw, seed=K, graph=g)
+            assert mx <= TOL[form], (rw, K, mx, fr)
+            # a second launch (counters reset by the last piece) is bit-equal
+            _, _, Cg2 = run_case(csr, K, form, tile=tile, rw=rw, seed=K, graph=g)
+            assert np.array_equal(Cg, Cg2)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
 def test_row_window_16(form):
     rng = np.random.default_rng(16)
     rp, ci, va = random_csr(rng, 501, 640, 0.02, empty_rows=0.1, heavy_rows=2)
