@@ -11,10 +11,17 @@
 //   --form fp32|split_bf16|bf16   precision form (default fp32)
 //   --random-b SEED               B uniform in [-1,1) instead of ones
 //   --warmup N / --exe N          launch counts (default 10 / 10)
+//   --check                       random B (seed 1 unless --random-b) and a
+//                                 host FP64 CSR product: prints the
+//                                 componentwise max-rel error (against
+//                                 |A||B|) and the Frobenius error, exit 6
+//                                 above the form's tolerance (the reference
+//                                 keeps B = 1 and never checks C)
 // The CSR is canonicalised on load (rows sorted by column, duplicates
 // summed), which the METCF tile layout requires.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -42,7 +49,7 @@ static std::string match_filename(const std::string& s) {
 int main(int argc, char** argv) {
   if (argc < 3) {
     std::printf("Usage: ./mma <matrix.mtx> <feature_dim> [--form fp32|split_bf16|bf16]"
-                " [--random-b SEED] [--warmup N] [--exe N]\n");
+                " [--random-b SEED] [--warmup N] [--exe N] [--check]\n");
     return ERROR_ARGS;
   }
   const char* filename = argv[1];
@@ -54,6 +61,7 @@ int main(int argc, char** argv) {
   accspmm_form form = ACCSPMM_FORM_FP32;
   long long bseed = -1;
   int warmup = 10, exe = 10;
+  bool check = false;
   for (int i = 3; i < argc; ++i) {
     std::string a = argv[i];
     if (a == "--form" && i + 1 < argc) {
@@ -68,6 +76,8 @@ int main(int argc, char** argv) {
       warmup = std::atoi(argv[++i]);
     } else if (a == "--exe" && i + 1 < argc) {
       exe = std::max(1, std::atoi(argv[++i]));
+    } else if (a == "--check") {
+      check = true;
     } else {
       std::printf("unknown option %s\n", a.c_str());
       return ERROR_ARGS;
@@ -92,6 +102,7 @@ int main(int argc, char** argv) {
       balanceStrategy(metcf_bit, (vint)metcf_bit.tcOffset.size() - 1,
                       (vint)metcf_bit.rowWindowOffset.size() - 1);
 
+  if (check && bseed < 0) bseed = 1;
   const size_t nB = (size_t)csr.cols * feature_dim, nC = (size_t)csr.rows * feature_dim;
   std::vector<float> hB(nB, 1.0f);
   if (bseed >= 0) accspmm_gen_dense((uint64_t)bseed, nB, hB.data());
@@ -103,7 +114,7 @@ int main(int argc, char** argv) {
 
   accspmm_tile tile;
   accspmm_tile_select(metcf_bit.handle, feature_dim, form, &tile);
-  if (!load_balance) tile.split_tiles = 0;  // plain row-window schedule
+  if (!load_balance) tile.split_tiles = 0;  // heavy windows are not split
   std::cout << (load_balance ? "doing AdpBalance pipelining ..." : "doing pipelining mma ...")
             << std::endl;
 
@@ -137,7 +148,43 @@ int main(int argc, char** argv) {
   outFile << mtx_name << "," << feature_dim << "," << elapsed_time << "," << spmm_throughput
           << "\n";
   outFile.close();
+  int rc = 0;
+  if (check) {
+    std::vector<float> hC(nC);
+    HIP_OK(hipMemcpy(hC.data(), dC, nC * sizeof(float), hipMemcpyDeviceToHost));
+    // FP64 product and |A||B| row by row; componentwise error against
+    // |A||B| (an entry with |A||B| = 0 must be exactly 0)
+    std::vector<double> ref(feature_dim), mag(feature_dim);
+    double max_rel = 0, err2 = 0, ref2 = 0;
+    for (vint r = 0; r < csr.rows; ++r) {
+      std::fill(ref.begin(), ref.end(), 0.0);
+      std::fill(mag.begin(), mag.end(), 0.0);
+      for (vint e = csr.row_ptr[r]; e < csr.row_ptr[r + 1]; ++e) {
+        const double a = csr.data[e];
+        const float* b = hB.data() + (size_t)csr.col_idx[e] * feature_dim;
+        for (int k = 0; k < feature_dim; ++k) {
+          ref[k] += a * b[k];
+          mag[k] += std::fabs(a * b[k]);
+        }
+      }
+      const float* c = hC.data() + (size_t)r * feature_dim;
+      for (int k = 0; k < feature_dim; ++k) {
+        const double d = std::fabs((double)c[k] - ref[k]);
+        const double rel = mag[k] > 0 ? d / mag[k] : (d > 0 ? INFINITY : 0.0);
+        max_rel = std::max(max_rel, rel);
+        err2 += d * d;
+        ref2 += ref[k] * ref[k];
+      }
+    }
+    const double fro = ref2 > 0 ? std::sqrt(err2 / ref2) : std::sqrt(err2);
+    const double tol = form == ACCSPMM_FORM_BF16 ? 1e-2
+                       : form == ACCSPMM_FORM_SPLIT_BF16 ? std::ldexp(1.0, -15) + 1e-6
+                                                         : 1e-5;
+    std::printf("check: max_rel=%.3e fro_rel=%.3e tol=%.1e %s\n", max_rel, fro, tol,
+                max_rel <= tol ? "ok" : "FAILED");
+    if (!(max_rel <= tol)) rc = 6;
+  }
   (void)hipFree(dB);
   (void)hipFree(dC);
-  return 0;
+  return rc;
 }

@@ -116,6 +116,69 @@ def check_errors(csr, B, results):
     return errs
 
 
+def batch_bench(ngraphs: int, steps: int, warmup: int):
+    """Batching: ngraphs Chung-Lu graphs (500-50,000 nodes log-uniform, mean
+    degree U[4,64], exponent U[2.0,2.5], U(0,1] values, graph seed 1000+i),
+    K uniform in {16, 64, 128}, B U[-1,1] (seed 2000+i), split-BF16.  Times
+    one persistent launch over all of them (eager and hipGraph replay)
+    against the same problems as back-to-back standalone calls, and checks
+    that every batched C equals its standalone C bit for bit."""
+    import numpy as np
+    import torch
+
+    import accspmm_amd as acc
+    rng = np.random.default_rng(1000)
+    probs, flops = [], 0.0
+    for i in range(ngraphs):
+        n = int(round(math.exp(rng.uniform(math.log(500), math.log(50000)))))
+        deg = float(rng.uniform(4, 64))
+        gam = float(rng.uniform(2.0, 2.5))
+        K = int(rng.choice([16, 64, 128]))
+        csr = acc.gen_chung_lu(n, deg, gam, 1000 + i, 0)
+        g = acc.DeviceGraph(csr, acc.default_build_opts(8, build_adpbme=0))
+        B = torch.from_numpy(acc.gen_dense(2000 + i, csr.cols, K)).cuda()
+        probs.append((g, B, torch.empty((csr.rows, K), device="cuda"),
+                      torch.empty((csr.rows, K), device="cuda")))
+        flops += 2.0 * csr.nnz * K
+    form = acc.FORM_SPLIT_BF16
+    batch = acc.Batch([(g, B, Cb) for g, B, Cb, _ in probs], form)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    def sequential():
+        for g, B, _, Cs in probs:
+            g.spmm(B, Cs, form)
+
+    seq_ms = timed(sequential)
+    eager_ms = timed(lambda: batch.run())
+    graph_ms = timed(lambda: batch.run(graph=True))
+    torch.cuda.synchronize()
+    same = all(torch.equal(Cb, Cs) for _, _, Cb, Cs in probs)
+    nnz = sum(int(g.info.nnz) for g, _, _, _ in probs)
+    tasks = batch.num_tasks
+    batch.close()
+    out = {"graphs": ngraphs, "nnz": nnz, "form": "split_bf16", "tasks": tasks,
+           "sequential_ms": round(seq_ms, 4), "batched_ms": round(eager_ms, 4),
+           "graph_replay_ms": round(graph_ms, 4),
+           "speedup_vs_sequential": round(seq_ms / min(eager_ms, graph_ms), 3),
+           "batched_gflops": round(flops / (min(eager_ms, graph_ms) * 1e-3) / 1e9, 1),
+           "bitwise_equal_to_standalone": bool(same)}
+    log(f"batch: {ngraphs} graphs, {nnz} nnz: sequential {seq_ms:.3f} ms, one launch "
+        f"{eager_ms:.3f} ms, graph replay {graph_ms:.3f} ms, equal={same}")
+    return out
+
+
 def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000):
     """Serving: Poisson arrivals over a pool of 64 resident Chung-Lu graphs
     (500-50,000 nodes, mean degree U[4,64], exponent U[2.0,2.5]) plus two
@@ -211,6 +274,8 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip errors and CPU baseline")
     ap.add_argument("--serve-requests", type=int, default=10000,
                     help="requests per serving run (0 skips the serving benchmark)")
+    ap.add_argument("--batch-graphs", type=int, default=256,
+                    help="graphs in the batched-launch benchmark (0 skips it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -301,10 +366,14 @@ def main():
                          f"median of {cpu.pop('runs')} runs")
         log(f"bench: CPU baseline {cpu['value']:.1f} GFLOP/s on {cpu['cores']} threads")
 
+    del g, B, Cd, outputs
+    torch.cuda.empty_cache()
+    batching = None
+    if args.batch_graphs > 0:
+        batching = batch_bench(args.batch_graphs, max(3, args.steps // 2), 2)
+        torch.cuda.empty_cache()
     serving = None
     if args.serve_requests > 0:
-        del g, B, Cd, outputs
-        torch.cuda.empty_cache()
         serving = serve_bench(args.serve_requests)
 
     head = forms[0]
@@ -340,6 +409,7 @@ def main():
                      "kernel_ms": round(hr["kernel_ms_mean"], 4),
                      "alg_bytes_per_launch": ab},
         "cpu_baseline": cpu,
+        "batching": batching,
         "serving": serving,
         "build_ms": round(build_ms, 1),
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv)

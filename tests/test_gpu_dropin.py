@@ -79,11 +79,14 @@ def test_mma_driver_appends_result_row(tmp_path, form):
     mma = str(REPO / "accspmm_amd" / "bin" / "mma")
     for K_ in (32, 48):
         r = subprocess.run([mma, str(mtx), str(K_), "--form", form, "--random-b", "3",
-                            "--warmup", "2", "--exe", "3"],
+                            "--warmup", "2", "--exe", "3", "--check"],
                            capture_output=True, text=True, cwd=tmp_path, timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         assert "m_tmp: 2000" in r.stdout
         assert "pipelining" in r.stdout
+        # the driver's own FP64 check of C (random B) passes its form's bound
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("check:")]
+        assert line and line[0].endswith("ok"), r.stdout
     rows_csv = (tmp_path / "result.csv").read_text().strip().splitlines()
     assert len(rows_csv) == 2
     name, k, ms, gf = rows_csv[0].split(",")
