@@ -85,11 +85,12 @@ class MetcfInfo(C.Structure):
 class Tile(C.Structure):
     _fields_ = [("feat_per_lane", C.c_uint32), ("waves_per_block", C.c_uint32),
                 ("split_tiles", C.c_uint32), ("xcd_slice", C.c_uint32),
-                ("pack_tiles", C.c_uint32), ("col_panels", C.c_uint32)]
+                ("pack_tiles", C.c_uint32), ("col_panels", C.c_uint32),
+                ("sweep", C.c_uint32)]
 
     def as_tuple(self):
         return (self.feat_per_lane, self.waves_per_block, self.split_tiles, self.xcd_slice,
-                self.pack_tiles, self.col_panels)
+                self.pack_tiles, self.col_panels, self.sweep)
 
 
 class HostCsr(C.Structure):
@@ -106,7 +107,8 @@ class Problem(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_uint32),
                 ("max_queue", C.c_uint32), ("max_inflight", C.c_uint32),
-                ("form", C.c_uint32), ("one_per_launch", C.c_uint32)]
+                ("form", C.c_uint32), ("one_per_launch", C.c_uint32),
+                ("large_work", C.c_uint64)]
 
 
 class ServerStats(C.Structure):
@@ -526,7 +528,8 @@ class Server:
 
     def __init__(self, hbm_budget_bytes: int = 64 << 30, max_batch: int = 64,
                  max_queue: int = 1 << 16, max_inflight: int = 2,
-                 form: int = FORM_SPLIT_BF16, one_per_launch: bool = False):
+                 form: int = FORM_SPLIT_BF16, one_per_launch: bool = False,
+                 large_work: int = 0):
         o = ServerOpts()
         lib().accspmm_server_opts_default(C.byref(o))
         o.hbm_budget_bytes = hbm_budget_bytes
@@ -535,6 +538,8 @@ class Server:
         o.max_inflight = max_inflight
         o.form = form
         o.one_per_launch = int(one_per_launch)
+        if large_work:
+            o.large_work = large_work
         h = C.c_void_p()
         _check(lib().accspmm_server_create(C.byref(o), C.byref(h)), "server_create")
         self.handle = h

@@ -87,6 +87,9 @@ __global__ __launch_bounds__(256) void k_batch_spmm(
     uint32_t nprob, uint32_t* __restrict__ head) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t total = prefix[nprob];
+  __shared__ __align__(16) float lds_vals[4 * kWaveLds];
+  float* lv = lds_vals + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * kWaveLds;
+  if (lane == 0) lv[kGroupValues] = 0.f;
   for (;;) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(head, 1u);
@@ -100,9 +103,9 @@ __global__ __launch_bounds__(256) void k_batch_spmm(
     // one row-window size per kernel keeps the register budget of the
     // widest variant (NB=4) without spills
     switch (g.nb) {
-      case 1: run_unit<FORM, 1, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
-      case 2: run_unit<FORM, 2, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
-      case 4: run_unit<FORM, 4, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows); break;
+      case 1: run_unit<FORM, 1, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv); break;
+      case 2: run_unit<FORM, 2, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv); break;
+      case 4: run_unit<FORM, 4, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv); break;
       default: break;
     }
   }
@@ -240,8 +243,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     g.kc_padded = q.kc_padded;
     g.rw = h->rw;
     g.nb = q.nb;
-    g.zero_val = h->nnz;
-    g.zero_row = h->cols;
+      g.zero_row = h->cols;
     g.pad_tile = h->T;
     g.partials = reinterpret_cast<float*>(A + bc);
     bc += q.partial_bytes;

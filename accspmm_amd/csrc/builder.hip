@@ -431,11 +431,15 @@ accspmm_status build_impl(const accspmm_csr_view* csr,
   // nnz): the SpMM kernels read it for group slots past a range's last tile
   ACC_TRY(h->a2b.alloc(((uint64_t)T + 1) * kColWindow * 4));
   ACC_TRY(h->bits.alloc(((uint64_t)T + 1) * words * 8));
-  ACC_TRY(h->tco.alloc(((uint64_t)T + 1) * 4));
-  // one zero past the values: the SpMM kernels read it for bitmap slots
-  // that hold no value (it is not part of the exported array)
-  ACC_TRY(h->data.alloc(((uint64_t)nnz + 1) * 4));
-  ACC_HIP(hipMemsetAsync(static_cast<char*>(h->data.p) + (uint64_t)nnz * 4, 0, 4, s));
+  // one spare entry: the kernels read tcOffset[t..t+1] as a pair for every
+  // tile t <= T (T is the all-padding tile)
+  ACC_TRY(h->tco.alloc(((uint64_t)T + 2) * 4));
+  // kValuePad zeros past the values (not part of the exported array): the
+  // SpMM kernels load a 4-tile group's values as one 256-float span that
+  // may run past the last value
+  ACC_TRY(h->data.alloc(((uint64_t)nnz + kValuePad) * 4));
+  ACC_HIP(hipMemsetAsync(static_cast<char*>(h->data.p) + (uint64_t)nnz * 4, 0,
+                         (uint64_t)kValuePad * 4, s));
   ACC_HIP(hipMemsetAsync(h->a2b.p, 0xFF, ((uint64_t)T + 1) * kColWindow * 4, s));
   ACC_HIP(hipMemsetAsync(h->bits.p, 0, ((uint64_t)T + 1) * words * 8, s));
   DevBuf tile_of;
@@ -461,6 +465,8 @@ accspmm_status build_impl(const accspmm_csr_view* csr,
   }
   ACC_TRY(exclusive_scan_u32(cnt.as<uint32_t>(), h->tco.as<uint32_t>(),
                              (uint64_t)T + 1, s));
+  ACC_HIP(hipMemcpyAsync(h->tco.as<uint32_t>() + T + 1, h->tco.as<uint32_t>() + T, 4,
+                         hipMemcpyDeviceToDevice, s));
   // 8. values
   if (nnz) {
     hipLaunchKernelGGL(k_values, dim3(blocks_for(nnz)), dim3(kThreads), 0, s,

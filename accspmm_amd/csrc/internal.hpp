@@ -32,6 +32,9 @@ namespace accspmm {
 
 constexpr uint32_t kColWindow = 8;
 constexpr uint32_t kPad = 0xFFFFFFFFu;
+// zero floats kept past the METCF values array (a 4-tile group's values are
+// loaded as one 256-float span)
+constexpr uint32_t kValuePad = 264;
 
 // Device buffer owned by RAII.
 struct DevBuf {
@@ -71,7 +74,8 @@ struct DevBuf {
 // (slot < 0; C rows are written directly, window by window), or one piece
 // of a split window (nwin == 1, slot >= 0: a partial, combined in slot order
 // by a second launch).  ngroups counts the unit's 4-tile groups; a group
-// never spans two windows.
+// never spans two windows.  panel is the column panel of a piece (0 for
+// whole windows).
 struct alignas(16) WorkUnit {
   uint32_t window;
   uint32_t nwin;
@@ -79,7 +83,7 @@ struct alignas(16) WorkUnit {
   uint32_t tile_end;
   int32_t slot;
   uint32_t ngroups;
-  uint32_t pad0, pad1;
+  uint32_t panel, pad1;
 };
 
 struct Schedule {
@@ -92,6 +96,18 @@ struct Schedule {
   bool packed = false;      // some unit spans several non-empty windows
   DevBuf units;             // WorkUnit[num_units]
   DevBuf split_info;        // per split window: window, first slot, piece count
+  std::vector<WorkUnit> h_units;  // host copy (the sweep plan is derived from it)
+};
+
+// Plan of the resident-window sweep (sweep.hip): the schedule's pieces
+// regrouped per wave.  Wave v of round r owns up to M windows (their running
+// sums live in LDS) and runs its pieces in (panel, window, slot) order.
+struct SweepPlan {
+  uint32_t P = 1, M = 1, NW = 0, R = 0;
+  DevBuf pieces;   // {tile_begin, tile_end, window slot | split flag << 8} per piece
+  DevBuf poff;     // [R*NW + 1] first piece of each (round, wave)
+  DevBuf ngroups;  // [R*NW] 4-tile groups of each (round, wave)
+  DevBuf wlist;    // [R*NW*M] windows of each (round, wave); kPad: none
 };
 
 }  // namespace accspmm
@@ -112,6 +128,10 @@ struct accspmm_metcf {
   mutable std::map<std::tuple<uint32_t, uint32_t, uint32_t>,
                    std::unique_ptr<accspmm::Schedule>>
       schedules;
+  // keyed by (split_tiles, pack_tiles, col_panels, waves, slots)
+  mutable std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>,
+                   std::unique_ptr<accspmm::SweepPlan>>
+      sweep_plans;
   mutable accspmm::DevBuf bconv;    // converted-B workspace (BF16 forms)
   mutable accspmm::DevBuf partials; // split-window partial sums
   uint64_t device_bytes() const {

@@ -10,6 +10,11 @@
 //     on its own stream; whenever one completes, pending requests (including
 //     those that arrived meanwhile) form the next launch, so new work
 //     overlaps work already on the GPU;
+//   * size classes: a request of nnz * K >= large_work runs in a launch of
+//     its own and large launches hold at most all slots but one, so small
+//     requests are never queued behind (or batched with) a large one and do
+//     not wait for it to finish; each class is served in arrival order and
+//     the older of the two queue heads goes first;
 //   * completion: the dispatcher polls each launch's event and stamps the
 //     requests' latency (submit -> observed completion).
 // In one_per_launch mode every request is its own standalone SpMM call and
@@ -51,6 +56,7 @@ struct Req {
   Clock::time_point t0;
   double latency = 0;
   int state = 0;  // 0 pending, 1 in flight, 2 done, 3 failed
+  bool large = false;
 };
 
 struct Slot {
@@ -60,6 +66,7 @@ struct Slot {
   std::vector<uint64_t> rids;
   uint64_t ws = 0;
   bool busy = false;
+  bool large = false;  // holds one large request
 };
 
 }  // namespace
@@ -78,7 +85,7 @@ struct accspmm_server {
   uint32_t next_gid = 1;
   std::unordered_map<uint64_t, Req> reqs;
   uint64_t next_rid = 1;
-  std::deque<uint64_t> pending;
+  std::deque<uint64_t> pending, pending_large;
   std::vector<std::unique_ptr<Slot>> slots;
   hipStream_t setup = nullptr;
   uint64_t resident = 0, inflight = 0, peak_inflight = 0;
@@ -128,29 +135,51 @@ bool accspmm_server::poll_completions() {
 bool accspmm_server::admit() {
   bool any = false;
   const uint64_t budget = o.hbm_budget_bytes > resident ? o.hbm_budget_bytes - resident : 0;
+  const uint32_t nslots = (uint32_t)slots.size();
   for (auto& sp : slots) {
-    if (pending.empty() || paused) break;
+    if (paused || (pending.empty() && pending_large.empty())) break;
     Slot& sl = *sp;
     if (sl.busy) continue;
+    uint32_t large_busy = 0;
+    for (auto& q : slots) large_busy += (q->busy && q->large) ? 1u : 0u;
+    // a large request may take this slot if one slot stays for small ones
+    // and its workspace fits; the older queue head goes first
+    bool take_large = false;
+    if (!pending_large.empty() && (nslots == 1 || large_busy + 1 < nslots)) {
+      const Req& rl = reqs[pending_large.front()];
+      const bool fits = inflight + rl.ws <= budget;
+      const bool older = pending.empty() || rl.t0 <= reqs[pending.front()].t0;
+      take_large = fits && older;
+    }
     std::vector<accspmm_problem> probs;
     std::vector<ProblemPlan> plans;
     uint64_t ws = 0;
-    const uint32_t cap = o.one_per_launch ? 1u : std::max(1u, o.max_batch);
-    uint32_t rw = 0;
-    while (!pending.empty() && probs.size() < cap) {
-      Req& r = reqs[pending.front()];
-      const uint32_t grw = r.prob.graph->rw;
-      if (rw && grw != rw) break;  // one launch serves one row-window size
-      if (inflight + ws + r.ws > budget) break;  // FIFO: wait for memory
-      rw = grw;
-      ws += r.ws;
+    if (take_large) {
+      Req& r = reqs[pending_large.front()];
+      ws = r.ws;
       probs.push_back(r.prob);
       plans.push_back(r.plan);
-      sl.rids.push_back(pending.front());
+      sl.rids.push_back(pending_large.front());
       r.state = 1;
-      pending.pop_front();
+      pending_large.pop_front();
+    } else {
+      const uint32_t cap = o.one_per_launch ? 1u : std::max(1u, o.max_batch);
+      uint32_t rw = 0;
+      while (!pending.empty() && probs.size() < cap) {
+        Req& r = reqs[pending.front()];
+        const uint32_t grw = r.prob.graph->rw;
+        if (rw && grw != rw) break;  // one launch serves one row-window size
+        if (inflight + ws + r.ws > budget) break;  // FIFO: wait for memory
+        rw = grw;
+        ws += r.ws;
+        probs.push_back(r.prob);
+        plans.push_back(r.plan);
+        sl.rids.push_back(pending.front());
+        r.state = 1;
+        pending.pop_front();
+      }
     }
-    if (probs.empty()) break;
+    if (probs.empty()) continue;
     accspmm_status st;
     if (o.one_per_launch) {
       accspmm_tile t;
@@ -160,6 +189,9 @@ bool accspmm_server::admit() {
         st = spmm_launch(p.graph, p.B, p.ldb, p.C, p.ldc, p.K, (int)o.form, t,
                          sl.stream, 3);
     } else {
+      // large requests too go through the slot's own launcher (a batch of
+      // one): its workspace is the slot's, so two requests for one graph
+      // on two streams never share a conversion buffer
       sl.L.form = (int)o.form;
       st = sl.L.prepare(probs.data(), (uint32_t)probs.size(), plans.data(), sl.stream);
       if (st == ACCSPMM_OK) st = sl.L.launch(sl.stream);
@@ -167,6 +199,7 @@ bool accspmm_server::admit() {
     if (st == ACCSPMM_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess)
       st = ACCSPMM_ERR_HIP;
     sl.ws = ws;
+    sl.large = take_large;
     inflight += ws;
     peak_inflight = std::max(peak_inflight, inflight);
     sl.busy = true;
@@ -191,7 +224,7 @@ void accspmm_server::run() {
     if (!busy_now && was_busy)
       busy_s += std::chrono::duration<double>(Clock::now() - busy_since).count();
     was_busy = busy_now;
-    if (stop && pending.empty() && !busy_now) break;
+    if (stop && pending.empty() && pending_large.empty() && !busy_now) break;
     if (busy_now) {
       // short poll: completion latency is bounded by this interval
       lock.unlock();
@@ -211,6 +244,7 @@ extern "C" void accspmm_server_opts_default(accspmm_server_opts* o) {
   o->max_inflight = 2;
   o->form = ACCSPMM_FORM_SPLIT_BF16;
   o->one_per_launch = 0;
+  o->large_work = 1ull << 30;
 }
 
 extern "C" accspmm_status accspmm_server_create(const accspmm_server_opts* opts,
@@ -300,7 +334,8 @@ extern "C" accspmm_status accspmm_server_submit(accspmm_server* s, uint32_t grap
     std::lock_guard<std::mutex> lock(s->mu);
     auto it = s->graphs.find(graph_id);
     if (it == s->graphs.end()) return ACCSPMM_ERR_NOT_FOUND;
-    if (s->pending.size() >= s->o.max_queue) return ACCSPMM_ERR_QUEUE_FULL;
+    if (s->pending.size() + s->pending_large.size() >= s->o.max_queue)
+      return ACCSPMM_ERR_QUEUE_FULL;
     h = it->second.h;
     it->second.users++;  // keeps the graph registered until completion
   }
@@ -326,8 +361,10 @@ extern "C" accspmm_status accspmm_server_submit(accspmm_server* s, uint32_t grap
     return ACCSPMM_ERR_BUDGET;  // can never fit
   }
   const uint64_t id = s->next_rid++;
+  const uint64_t large_work = s->o.large_work ? s->o.large_work : (1ull << 30);
+  r.large = !s->o.one_per_launch && (uint64_t)h->nnz * K >= large_work;
   s->reqs.emplace(id, r);
-  s->pending.push_back(id);
+  (r.large ? s->pending_large : s->pending).push_back(id);
   s->submitted++;
   *request_id = id;
   s->cv_work.notify_one();
@@ -352,7 +389,7 @@ extern "C" accspmm_status accspmm_server_drain(accspmm_server* s) {
   std::unique_lock<std::mutex> lock(s->mu);
   s->cv_work.notify_one();
   s->cv_idle.wait(lock, [s] {
-    if (!s->pending.empty() && !s->paused) return false;
+    if ((!s->pending.empty() || !s->pending_large.empty()) && !s->paused) return false;
     for (auto& sp : s->slots)
       if (sp->busy) return false;
     return true;

@@ -73,38 +73,55 @@ __global__ __launch_bounds__(512) void spmm_kernel(SpmmArgs g) {
     kc = tau % g.n_kc;
   }
   if (unit >= g.num_units) return;
+  __shared__ __align__(16) float lds_vals[8 * kWaveLds];
+  float* lv = lds_vals + wave * kWaveLds;
+  if ((threadIdx.x & 63u) == 0) lv[kGroupValues] = 0.f;
   const WorkUnit u = g.units[unit];
-  run_unit<FORM, NB, RW, PACKED>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows);
+  run_unit<FORM, NB, RW, PACKED>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv);
 }
 
 // B conversion for the BF16 forms: zero-padded to ldx >= K columns, plus
-// one all-zero row N that padded tile columns read.
-__global__ void convert_b_split(const float* __restrict__ B, uint32_t N,
-                                uint32_t K, uint32_t ldb,
-                                uint32_t* __restrict__ out, uint32_t ldx) {
-  for (uint32_t r = blockIdx.x; r <= N; r += gridDim.x)
-    for (uint32_t k = threadIdx.x; k < ldx; k += blockDim.x) {
-      uint32_t v = 0;
-      if (k < K && r < N) {
-        float x = B[(uint64_t)r * ldb + k];
-        uint32_t hp = pack_bf16(x, 0.f) & 0xFFFFu;
-        float lo = x - bf16_lo_to_f32(hp);
-        uint32_t lp = pack_bf16(lo, 0.f) & 0xFFFFu;
-        v = hp | (lp << 16);
+// one all-zero row N that padded tile columns read.  One thread converts
+// four consecutive features (ldx is a multiple of 16): a 16-byte load when
+// the row segment lies inside K and B is 16-byte aligned with ldb % 4 == 0,
+// and one 16-byte (split: hi | lo << 16 per feature) or 8-byte (BF16)
+// store.  Grid-stride over (N + 1) * ldx / 4 chunks.
+template <int FORM>
+__global__ __launch_bounds__(256) void convert_b(const float* __restrict__ B, uint32_t N,
+                                                 uint32_t K, uint32_t ldb, void* __restrict__ out,
+                                                 uint32_t ldx, uint32_t vec) {
+  const uint32_t cpr = ldx / 4;  // chunks per row
+  const uint64_t total = (uint64_t)(N + 1) * cpr;
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = (uint32_t)(c / cpr), k = (uint32_t)(c % cpr) * 4;
+    float x[4] = {0.f, 0.f, 0.f, 0.f};
+    if (r < N) {
+      const float* src = B + (uint64_t)r * ldb + k;
+      if (vec && k + 4 <= K) {
+        const f4 v = *reinterpret_cast<const f4*>(src);
+        x[0] = v[0]; x[1] = v[1]; x[2] = v[2]; x[3] = v[3];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i] = k + i < K ? src[i] : 0.f;
       }
-      out[(uint64_t)r * ldx + k] = v;
     }
-}
-
-__global__ void convert_b_bf16(const float* __restrict__ B, uint32_t N,
-                               uint32_t K, uint32_t ldb,
-                               uint16_t* __restrict__ out, uint32_t ldx) {
-  for (uint32_t r = blockIdx.x; r <= N; r += gridDim.x)
-    for (uint32_t k = threadIdx.x; k < ldx; k += blockDim.x) {
-      uint16_t v = 0;
-      if (k < K && r < N) v = (uint16_t)(pack_bf16(B[(uint64_t)r * ldb + k], 0.f) & 0xFFFFu);
-      out[(uint64_t)r * ldx + k] = v;
+    if (FORM == kSPLIT) {
+      u4 o;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t hp = pack_bf16(x[i], 0.f) & 0xFFFFu;
+        const uint32_t lp = pack_bf16(x[i] - bf16_lo_to_f32(hp), 0.f) & 0xFFFFu;
+        o[i] = hp | (lp << 16);
+      }
+      reinterpret_cast<u4*>(out)[c] = o;
+    } else {
+      uint2 o;
+      o.x = pack_bf16(x[0], x[1]);
+      o.y = pack_bf16(x[2], x[3]);
+      reinterpret_cast<uint2*>(out)[c] = o;
     }
+  }
 }
 
 // For window w and panel boundary p in [1, P): the number of the window's
@@ -251,7 +268,7 @@ accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
           b = p + 1 == nu ? len : std::min(len, (b + 3u) & ~3u);
           b = std::max(a, b);
           dst.push_back(WorkUnit{w, 1, tb + c.a + a, tb + c.a + b, (int32_t)slot,
-                                 (b - a + 3) / 4, 0, 0});
+                                 (b - a + 3) / 4, c.panel, 0});
           ++slot;
         }
       }
@@ -273,6 +290,7 @@ accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
   close_open();
   for (auto& v : by_panel) units.insert(units.end(), v.begin(), v.end());
   sch->num_units = (uint32_t)units.size();
+  sch->h_units = units;
   sch->packed = false;
   for (const WorkUnit& u : units) sch->packed |= (u.slot < 0 && u.nwin > 1 && u.ngroups > 0);
   sch->num_pieces = pieces;
@@ -363,6 +381,11 @@ void default_tile(const accspmm_metcf* h, uint32_t K, int form,
   }
 }
 
+// sweep.hip: the resident-window sweep over the same pieces
+accspmm_status sweep_launch(const accspmm_metcf* h, const dev::SpmmArgs& g,
+                            const accspmm_tile& tile, uint32_t nb, int form,
+                            const Schedule* sch, hipStream_t s);
+
 // phases: bit 0 = convert B (BF16 forms), bit 1 = SpMM kernel
 accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
                            float* C, uint32_t ldc, uint32_t K, int form,
@@ -402,11 +425,10 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   g.kc_padded = n_kc * KC;
   g.xcd_slice = tile.xcd_slice == 2 ? 2u
                 : (tile.xcd_slice == 1 && (8 % n_kc) == 0) ? 1u : 0u;
-  g.zero_val = h->nnz;
   g.zero_row = h->cols;
   g.pad_tile = h->T;
 
-  if (sch->num_pieces) {
+  if (sch->num_pieces && !tile.sweep && (phases & 2)) {
     const uint64_t pbytes = (uint64_t)sch->num_pieces * h->rw * g.kc_padded * 4;
     ACC_TRY(h->partials.reserve(pbytes));
     g.partials = h->partials.as<float>();
@@ -419,20 +441,23 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
     const uint32_t ldx = n_kc * KC;
     const uint64_t es = form == kSPLIT ? 4 : 2;
     ACC_TRY(h->bconv.reserve(((uint64_t)h->cols + 1) * ldx * es));
-    uint32_t blocks = std::min<uint32_t>(h->cols + 1, 65535u * 4);
+    const uint64_t chunks = ((uint64_t)h->cols + 1) * (ldx / 4);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((chunks + 255) / 256, 8192);
+    const uint32_t vec = (ldb % 4 == 0 && ((uintptr_t)B % 16) == 0) ? 1u : 0u;
     if (!(phases & 1)) {
       // converted by an earlier prepare call
     } else if (form == kSPLIT)
-      hipLaunchKernelGGL(convert_b_split, dim3(blocks), dim3(256), 0, s, B,
-                         h->cols, K, ldb, h->bconv.as<uint32_t>(), ldx);
+      hipLaunchKernelGGL(convert_b<kSPLIT>, dim3(blocks), dim3(256), 0, s, B, h->cols, K,
+                         ldb, h->bconv.p, ldx, vec);
     else
-      hipLaunchKernelGGL(convert_b_bf16, dim3(blocks), dim3(256), 0, s, B,
-                         h->cols, K, ldb, h->bconv.as<uint16_t>(), ldx);
+      hipLaunchKernelGGL(convert_b<kBF16>, dim3(blocks), dim3(256), 0, s, B, h->cols, K,
+                         ldb, h->bconv.p, ldx, vec);
     ACC_HIP(hipGetLastError());
     g.B = h->bconv.p;
     g.ldb = ldx;
   }
   if (!(phases & 2)) return ACCSPMM_OK;
+  if (tile.sweep) return sweep_launch(h, g, tile, nb, form, sch, s);
 
   uint64_t blocks;
   if (g.xcd_slice == 1) {

@@ -29,12 +29,11 @@ struct SpmmArgs {
   float* __restrict__ partials;
   const void* __restrict__ B;  // FP32: float; SPLIT: u32 (hi|lo<<16); BF16: u16
   float* __restrict__ C;
-  // what padded columns and empty bitmap slots read: BF16 forms: row
-  // zero_row of the converted B (its last row, all zero); FP32 form: row 0
-  // of the caller's B, masked after the load (measured faster on gfx950
-  // than reading a separate zero buffer); A: data[zero_val] (one zero past
-  // the values)
-  uint32_t zero_row, zero_val;
+  // what padded columns read: BF16 forms: row zero_row of the converted B
+  // (its last row, all zero); FP32 form: row 0 of the caller's B, masked
+  // after the load (measured faster on gfx950 than reading a separate zero
+  // buffer)
+  uint32_t zero_row;
   uint32_t pad_tile;  // the all-padding tile (index T) past the real tiles
   uint32_t ldb, ldc;
   uint32_t rows, K;
@@ -78,43 +77,32 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
                                  uint32_t col, uint32_t feat, uint32_t K,
                                  uint32_t zero_row) {
   const uint32_t c = col != kPad ? col : zero_row;
-  if (FORM == kBF16) {
+  if constexpr (FORM == kBF16) {
     const uint16_t* p = static_cast<const uint16_t*>(B) + (uint64_t)c * ldb + feat;
-    if (NB == 8) {
+    if constexpr (NB == 8) {
       u4 v = *reinterpret_cast<const u4*>(p);
       r.w[0] = v[0]; r.w[1] = v[1]; r.w[2] = v[2]; r.w[3] = v[3];
-    } else if (NB == 4) {
+    } else if constexpr (NB == 4) {
       uint2 v = *reinterpret_cast<const uint2*>(p);
       r.w[0] = v.x; r.w[1] = v.y;
-    } else if (NB == 2) {
+    } else if constexpr (NB == 2) {
       r.w[0] = *reinterpret_cast<const uint32_t*>(p);
     } else {
       r.w[0] = *p;
     }
-  } else if (FORM == kSPLIT) {
-    const uint32_t* p = static_cast<const uint32_t*>(B) + (uint64_t)c * ldb + feat;
-#pragma unroll
-    for (int i = 0; i < NB / 4; ++i) {
-      u4 v = *reinterpret_cast<const u4*>(p + 4 * i);
-      r.w[4 * i] = v[0]; r.w[4 * i + 1] = v[1];
-      r.w[4 * i + 2] = v[2]; r.w[4 * i + 3] = v[3];
-    }
-    if (NB == 2) {
-      uint2 v = *reinterpret_cast<const uint2*>(p);
-      r.w[0] = v.x; r.w[1] = v.y;
-    } else if (NB == 1) {
-      r.w[0] = *p;
-    }
-  } else if (FULL) {
-    const uint32_t* p = static_cast<const uint32_t*>(B) + (uint64_t)(col != kPad ? col : 0u) * ldb + feat;
-    if (NB >= 4) {
+  } else if constexpr (FORM == kSPLIT || FULL) {
+    // split form: the converted B (zero row for padded columns); FP32 form:
+    // the caller's B (padded columns read row 0, masked when consumed)
+    const uint32_t row = FORM == kSPLIT ? c : (col != kPad ? col : 0u);
+    const uint32_t* p = static_cast<const uint32_t*>(B) + (uint64_t)row * ldb + feat;
+    if constexpr (NB >= 4) {
 #pragma unroll
       for (int i = 0; i < NB / 4; ++i) {
         u4 v = *reinterpret_cast<const u4*>(p + 4 * i);
         r.w[4 * i] = v[0]; r.w[4 * i + 1] = v[1];
         r.w[4 * i + 2] = v[2]; r.w[4 * i + 3] = v[3];
       }
-    } else if (NB == 2) {
+    } else if constexpr (NB == 2) {
       uint2 v = *reinterpret_cast<const uint2*>(p);
       r.w[0] = v.x; r.w[1] = v.y;
     } else {
@@ -132,24 +120,40 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
   }
 }
 
-// One pipeline stage: everything a lane needs for one 4-tile group.  Empty
-// slots of A and padded columns of the BF16 forms read zeros; bit j of bmask
+// A 4-tile group's A values are consecutive in the METCF values array (its
+// tiles are consecutive), at most 4 x 64 of them: one 16-byte load per lane
+// brings a 256-float span starting at the group's first value, which goes
+// through a per-wave LDS buffer (kGroupValues floats, then one zero read by
+// empty bitmap slots).  One vector-memory instruction instead of one per
+// tile column: the texture-address unit, which handles every lane address
+// of every load at four lanes a clock, bounds this kernel.
+constexpr uint32_t kGroupValues = 256;
+constexpr uint32_t kWaveLds = kGroupValues + 4;  // floats per wave (16-byte multiple)
+
+// One pipeline stage: everything a lane needs for one 4-tile group.  Padded
+// columns of the BF16 forms read the converted B's zero row; bit j of bmask
 // marks a real column j (the FP32 form masks the others).
 template <int FORM, int NB>
 struct Stage {
   BRow<FORM, NB> b[8];
-  float a[8];
+  u4 vals;        // this lane's 4 floats of the group's value span
+  uint32_t byte;  // the lane's tile row of the bitmap (0: no A row)
+  uint32_t base;  // index of the row's first value within the span
   uint32_t bmask;
-  uint32_t win;  // row window of the group (wave-uniform)
+  uint32_t win;   // tag of the group (wave-uniform)
 };
 
-// Tile metadata a lane needs for its tile of a group: the 8 condensed
-// columns, its row's bitmap word and the value offset of that word.
+// Tile metadata as loaded: lane n of a quarter holds columns 4(n&1)..+3 of
+// its tile, and its even/odd lane pair one 8-byte word: the bitmap (even
+// lane) or tcOffset[t], tcOffset[t+1] (odd lane).  RW=16 tiles load both
+// bitmap words and tcOffset[t] in every lane.
 struct Meta {
-  uint32_t col[8];
-  unsigned long long bw;
-  uint32_t off;
-  uint32_t win;  // row window of the group (wave-uniform)
+  u4 half;
+  uint32_t x, y;       // RW=8: bitmap lo/hi (even) or tcOffset pair (odd)
+  uint32_t w0, w1;     // RW=16: bitmap word 0 (rows 0-7), lo/hi
+  uint32_t w2, w3;     // RW=16: bitmap word 1 (rows 8-15), lo/hi
+  uint32_t off16;      // RW=16: tcOffset[t]
+  uint32_t win;        // tag of the group (wave-uniform)
 };
 
 // Position of a 4-tile group (wave-uniform): its window w, first tile t and
@@ -159,72 +163,108 @@ struct GroupPos {
 };
 
 // Advances to the next group of the unit; only called while groups remain.
-// Packed units contain no empty windows, so the next window has tiles.
+// A unit of one window (or one piece) never leaves its range; packed units
+// contain no empty windows, so the next window has tiles.
+template <bool PACKED>
 __device__ inline void next_group(GroupPos& p, const uint32_t* __restrict__ rwo) {
   p.t += 4;
-  if (p.t >= p.tend) {
+  if (PACKED && p.t >= p.tend) {
     ++p.w;
     p.t = rwo[p.w];
     p.tend = rwo[p.w + 1];
   }
 }
 
+typedef uint32_t u4a __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint32_t u2a __attribute__((ext_vector_type(2), aligned(4)));
+
+// The value lane n ^ 1 holds (DPP quad permutation [1,0,3,2]).
+__device__ inline uint32_t from_pair(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);
+}
+
 template <int RW>
 __device__ inline void load_meta(Meta& mt, const SpmmArgs& g, const GroupPos& p,
-                                 bool valid, uint32_t q, uint32_t m) {
+                                 bool valid, uint32_t q, uint32_t n) {
   // a tile past the group's range (or an invalid group) reads the
   // all-padding tile: kPad columns, empty bitmap, offset nnz
   const uint32_t t = p.t + q;
   const bool tile_ok = valid && t < p.tend;
   const uint32_t tt = tile_ok ? t : g.pad_tile;
-  const u4* cp = reinterpret_cast<const u4*>(g.a2b + (uint64_t)tt * 8);
-  const u4 c0 = cp[0], c1 = cp[1];
-  uint32_t off = g.tco[tt];
-  unsigned long long bw;
+  mt.half = *reinterpret_cast<const u4*>(g.a2b + (uint64_t)tt * 8 + 4 * (n & 1u));
   if (RW == 8) {
-    bw = g.bits[tt];
+    // even lanes: bitmap word; odd lanes: tcOffset[t], tcOffset[t+1]
+    // (integer address select, one load instruction)
+    const uint64_t ab = reinterpret_cast<uint64_t>(g.bits + tt);
+    const uint64_t ao = reinterpret_cast<uint64_t>(g.tco + tt);
+    const u2a v = *reinterpret_cast<const u2a*>((n & 1u) ? ao : ab);
+    mt.x = v.x;
+    mt.y = v.y;
   } else {
-    // 128-bit tile: rows 0-7 in word 0, rows 8-15 in word 1
-    const unsigned long long w0 = g.bits[2ull * tt], w1 = g.bits[2ull * tt + 1];
-    bw = m < 8 ? w0 : w1;
-    off += m < 8 ? 0u : popc64(w0);
+    const u4a w = *reinterpret_cast<const u4a*>(g.bits + 2ull * tt);
+    mt.w0 = w.x; mt.w1 = w.y; mt.w2 = w.z; mt.w3 = w.w;
+    mt.off16 = g.tco[tt];
   }
-  mt.col[0] = c0[0]; mt.col[1] = c0[1]; mt.col[2] = c0[2]; mt.col[3] = c0[3];
-  mt.col[4] = c1[0]; mt.col[5] = c1[1]; mt.col[6] = c1[2]; mt.col[7] = c1[3];
-  mt.bw = bw;
-  mt.off = off;
   mt.win = p.w;
 }
 
-// Gathers the B rows of a tile's 8 columns and decodes the lane's A row: a
-// set bitmap bit j loads the row's next stored value, a clear one the zero
-// past the values array.
-template <int FORM, int NB, bool FULL>
+// Gathers the B rows of the tile's 8 columns and the lane's share of the
+// group's value span; records the lane's A row (bitmap byte, first value).
+template <int FORM, int NB, bool FULL, int RW>
 __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
-                                 const SpmmArgs& g, uint32_t m, bool a_rows,
-                                 uint32_t feat, const void* B, uint32_t ldb) {
+                                 const SpmmArgs& g, uint32_t n, bool a_rows,
+                                 uint32_t feat, const void* B, uint32_t ldb,
+                                 uint32_t lane) {
+  const bool odd = (n & 1u) != 0u;
+  uint32_t col[8];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t other = from_pair(mt.half[j]);
+    col[j] = odd ? other : mt.half[j];
+    col[4 + j] = odd ? mt.half[j] : other;
+  }
+  unsigned long long bw;
+  uint32_t off;
+  if (RW == 8) {
+    const uint32_t px = from_pair(mt.x), py = from_pair(mt.y);
+    bw = odd ? ((unsigned long long)py << 32 | px) : ((unsigned long long)mt.y << 32 | mt.x);
+    off = odd ? mt.x : px;
+  } else {
+    const unsigned long long w0 = (unsigned long long)mt.w1 << 32 | mt.w0;
+    const unsigned long long w1 = (unsigned long long)mt.w3 << 32 | mt.w2;
+    bw = n < 8 ? w0 : w1;
+    off = mt.off16 + (n < 8 ? 0u : popc64(w0));
+  }
   uint32_t bm = 0;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    load_brow<FORM, NB, FULL>(st.b[j], B, ldb, mt.col[j], feat, g.K, g.zero_row);
-    if (FORM == kFP32) bm |= (mt.col[j] != kPad ? 1u : 0u) << j;
+    load_brow<FORM, NB, FULL>(st.b[j], B, ldb, col[j], feat, g.K, g.zero_row);
+    if (FORM == kFP32) bm |= (col[j] != kPad ? 1u : 0u) << j;
   }
   st.bmask = bm;
-  const uint32_t sh = 8u * (m & 7u);
-  const uint32_t byte = a_rows ? (uint32_t)(mt.bw >> sh) & 0xFFu : 0u;
-  const uint32_t base = mt.off + (sh ? popc64(mt.bw & ((1ull << sh) - 1ull)) : 0u);
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const bool on = (byte >> j) & 1u;
-    const uint32_t rank = __popc(byte & ((1u << j) - 1u));
-    st.a[j] = g.data[on ? base + rank : g.zero_val];
-  }
+  // the group's first value: tcOffset of quarter 0's tile (lane 0)
+  const uint32_t vbase = __builtin_amdgcn_readfirstlane(off);
+  const u4a v = *reinterpret_cast<const u4a*>(g.data + vbase + 4u * lane);
+  st.vals = u4{v.x, v.y, v.z, v.w};
+  const uint32_t sh = 8u * (n & 7u);
+  st.byte = a_rows ? (uint32_t)(bw >> sh) & 0xFFu : 0u;
+  st.base = off - vbase + (sh ? popc64(bw & ((1ull << sh) - 1ull)) : 0u);
   st.win = mt.win;
 }
 
 template <int FORM, int NB, int RW>
 __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
-                                     uint32_t m) {
+                                     uint32_t m, float* lv, uint32_t lane) {
+  // the group's value span through the wave's LDS buffer: a set bitmap bit
+  // j reads the row's next value, a clear one the zero after the span
+  *reinterpret_cast<u4*>(lv + 4u * lane) = st.vals;
+  float a[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const bool on = (st.byte >> j) & 1u;
+    const uint32_t rank = __popc(st.byte & ((1u << j) - 1u));
+    a[j] = lv[on ? st.base + rank : kGroupValues];
+  }
   if (FORM == kFP32) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -237,7 +277,7 @@ __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
 #pragma unroll
       for (int b = 0; b < NB; ++b)
         acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-            st.a[j], __builtin_bit_cast(float, st.b[j].w[b]), acc[b], 0, 0, 0);
+            a[j], __builtin_bit_cast(float, st.b[j].w[b]), acc[b], 0, 0, 0);
     return;
   }
   // A operand: 8 bf16 along the tile's condensed columns
@@ -245,11 +285,11 @@ __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
   {
     uint32_t hi[4];
 #pragma unroll
-    for (int d = 0; d < 4; ++d) hi[d] = pack_bf16(st.a[2 * d], st.a[2 * d + 1]);
+    for (int d = 0; d < 4; ++d) hi[d] = pack_bf16(a[2 * d], a[2 * d + 1]);
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
-      float l0 = st.a[2 * d] - bf16_lo_to_f32(hi[d]);
-      float l1 = st.a[2 * d + 1] - bf16_hi_to_f32(hi[d]);
+      float l0 = a[2 * d] - bf16_lo_to_f32(hi[d]);
+      float l1 = a[2 * d + 1] - bf16_hi_to_f32(hi[d]);
       alo[d] = FORM == kSPLIT ? pack_bf16(l0, l1) : 0u;
     }
     if (FORM == kSPLIT && RW == 8 && m >= 8) {
@@ -389,34 +429,35 @@ template <int FORM, int NB, int RW, bool FULL, bool PACKED>
 __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
                                  GroupPos pos, uint32_t ngroups, uint32_t q,
                                  uint32_t m, bool a_rows, uint32_t feat,
-                                 const void* B, uint32_t ldb) {
+                                 const void* B, uint32_t ldb, float* lv,
+                                 uint32_t lane) {
   Meta mA, mB;
   Stage<FORM, NB> dA, dB;
   auto consume = [&](Stage<FORM, NB>& d) {
     if (PACKED && out.direct) out.move_to(d.win);
-    compute_stage<FORM, NB, RW>(d, out.acc, m);
+    compute_stage<FORM, NB, RW>(d, out.acc, m, lv, lane);
   };
   // pos: position of the most recently loaded metadata (group gn)
   uint32_t gn = 0;
   load_meta<RW>(mA, g, pos, true, q, m);
   if (ngroups > 1) {
-    next_group(pos, g.rwo);
+    next_group<PACKED>(pos, g.rwo);
     gn = 1;
   }
   load_meta<RW>(mB, g, pos, ngroups > 1, q, m);
-  load_data<FORM, NB, FULL>(dA, mA, g, m, a_rows, feat, B, ldb);
+  load_data<FORM, NB, FULL, RW>(dA, mA, g, m, a_rows, feat, B, ldb, lane);
   uint32_t gi = 0;
   for (; gi + 3 < ngroups; gi += 2) {
-    next_group(pos, g.rwo);  // group gi + 2
+    next_group<PACKED>(pos, g.rwo);  // group gi + 2
     load_meta<RW>(mA, g, pos, true, q, m);
-    load_data<FORM, NB, FULL>(dB, mB, g, m, a_rows, feat, B, ldb);
+    load_data<FORM, NB, FULL, RW>(dB, mB, g, m, a_rows, feat, B, ldb, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
     __builtin_amdgcn_sched_barrier(0);
     const bool v3 = gi + 3 < ngroups;
-    if (v3) next_group(pos, g.rwo);  // group gi + 3
+    if (v3) next_group<PACKED>(pos, g.rwo);  // group gi + 3
     load_meta<RW>(mB, g, pos, v3, q, m);
-    load_data<FORM, NB, FULL>(dA, mA, g, m, a_rows, feat, B, ldb);
+    load_data<FORM, NB, FULL, RW>(dA, mA, g, m, a_rows, feat, B, ldb, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dB);
     __builtin_amdgcn_sched_barrier(0);
@@ -425,18 +466,18 @@ __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
   // dA holds group gi, mB the metadata of group gi+1
   const uint32_t left = ngroups - gi;  // 1, 2 or 3
   if (left == 3) {
-    next_group(pos, g.rwo);  // group gi + 2
+    next_group<PACKED>(pos, g.rwo);  // group gi + 2
     load_meta<RW>(mA, g, pos, true, q, m);
-    load_data<FORM, NB, FULL>(dB, mB, g, m, a_rows, feat, B, ldb);
+    load_data<FORM, NB, FULL, RW>(dB, mB, g, m, a_rows, feat, B, ldb, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
     __builtin_amdgcn_sched_barrier(0);
-    load_data<FORM, NB, FULL>(dA, mA, g, m, a_rows, feat, B, ldb);
+    load_data<FORM, NB, FULL, RW>(dA, mA, g, m, a_rows, feat, B, ldb, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dB);
     consume(dA);
   } else if (left == 2) {
-    load_data<FORM, NB, FULL>(dB, mB, g, m, a_rows, feat, B, ldb);
+    load_data<FORM, NB, FULL, RW>(dB, mB, g, m, a_rows, feat, B, ldb, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
     consume(dB);
@@ -451,7 +492,7 @@ __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
 template <int FORM, int NB, int RW, bool PACKED>
 __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
                          const void* B, uint32_t ldb, float* C, uint32_t ldc,
-                         uint32_t K, uint32_t rows) {
+                         uint32_t K, uint32_t rows, float* lv) {
   constexpr uint32_t KC = 16u * NB;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t q = lane >> 4, n = lane & 15u;
@@ -479,14 +520,19 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
 
   const uint32_t ngroups = u.ngroups;
   if (ngroups > 0) {
-    GroupPos pos{u.window, u.tile_begin, u.tile_end};
+    // the unit is wave-uniform: keep its position in scalar registers
+    GroupPos pos{(uint32_t)__builtin_amdgcn_readfirstlane(u.window),
+                 (uint32_t)__builtin_amdgcn_readfirstlane(u.tile_begin),
+                 (uint32_t)__builtin_amdgcn_readfirstlane(u.tile_end)};
     // packed unit: group ranges are whole windows (its first window is
     // not empty when it has groups)
     if (u.slot < 0) pos.tend = g.rwo[pos.w + 1];
     if (full)
-      unit_loop<FORM, NB, RW, true, PACKED>(g, out, pos, ngroups, q, m, a_rows, feat, B, ldb);
+      unit_loop<FORM, NB, RW, true, PACKED>(g, out, pos, ngroups, q, m, a_rows, feat, B, ldb,
+                                            lv, lane);
     else
-      unit_loop<FORM, NB, RW, false, PACKED>(g, out, pos, ngroups, q, m, a_rows, feat, B, ldb);
+      unit_loop<FORM, NB, RW, false, PACKED>(g, out, pos, ngroups, q, m, a_rows, feat, B, ldb,
+                                             lv, lane);
   }
 
   if (u.slot < 0) {

@@ -52,7 +52,7 @@ void load_file(const char* path) {
   std::ifstream in(path);
   if (!in) return;
   std::string line;
-  // one entry per line: "key": [nb, wpb, split, xcd, pack, panels],
+  // one entry per line: "key": [nb, wpb, split, xcd, pack, panels, sweep],
   while (std::getline(in, line)) {
     size_t a = line.find('"'), b = line.find('"', a + 1);
     size_t l = line.find('['), r = line.find(']');
@@ -60,11 +60,11 @@ void load_file(const char* path) {
         l == std::string::npos || r == std::string::npos)
       continue;
     std::string key = line.substr(a + 1, b - a - 1);
-    unsigned v[6] = {0, 0, 0, 0, 0, 0};
-    const int got = std::sscanf(line.c_str() + l + 1, "%u , %u , %u , %u , %u , %u",
-                                &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]);
-    if (got >= 5)  // entries written before col_panels existed have 5 fields
-      g_cache[key] = accspmm_tile{v[0], v[1], v[2], v[3], v[4], v[5]};
+    unsigned v[7] = {0, 0, 0, 0, 0, 0, 0};
+    const int got = std::sscanf(line.c_str() + l + 1, "%u , %u , %u , %u , %u , %u , %u",
+                                &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6]);
+    if (got >= 5)  // older entries lack col_panels and sweep (both 0)
+      g_cache[key] = accspmm_tile{v[0], v[1], v[2], v[3], v[4], v[5], v[6]};
   }
 }
 
@@ -77,7 +77,7 @@ void save_file(const char* path) {
     out << "  \"" << kv.first << "\": [" << kv.second.feat_per_lane << ", "
         << kv.second.waves_per_block << ", " << kv.second.split_tiles << ", "
         << kv.second.xcd_slice << ", " << kv.second.pack_tiles << ", "
-        << kv.second.col_panels << "]"
+        << kv.second.col_panels << ", " << kv.second.sweep << "]"
         << (++i < g_cache.size() ? "," : "")
         << "\n";
   }
@@ -174,7 +174,7 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
           for (uint32_t pack : {0u, 64u}) {
             if (xcd == 1 && (n_kc == 1 || (8 % n_kc) != 0)) continue;
             if (pack && mean >= 64) continue;  // windows already long
-            consider(accspmm_tile{nb, wpb, sp, xcd, pack, 0});
+            consider(accspmm_tile{nb, wpb, sp, xcd, pack, 0, 0});
           }
   }
   // stage 2: column panels around the best shape, when the B columns one

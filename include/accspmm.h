@@ -139,6 +139,11 @@ typedef struct accspmm_tile {
                               ranges of B and run the pieces range by range,
                               so each range of B is gathered while it is
                               cached (0 or 1: off)                           */
+  uint32_t sweep;          /* 1: resident-window sweep: one block per CU, each
+                              wave owns a set of windows whose running sums
+                              stay in LDS and runs their pieces panel by
+                              panel (same pieces and result bits as 0; the
+                              work-unit mapping fields are ignored)          */
 } accspmm_tile;
 
 /* C[rows x K] = A * B[cols x K].  B (ldb >= K) and C (ldc >= K) are
@@ -272,6 +277,9 @@ typedef struct accspmm_server_opts {
   uint32_t max_inflight;      /* launches in flight (streams)               */
   uint32_t form;              /* accspmm_form                               */
   uint32_t one_per_launch;    /* 1: baseline mode, one launch per request   */
+  uint64_t large_work;        /* requests with nnz * K at or above this run
+                                 in launches of their own, on all slots but
+                                 one (0: 2^30); smaller ones are batched     */
 } accspmm_server_opts;
 
 void accspmm_server_opts_default(accspmm_server_opts* o);

@@ -21,10 +21,14 @@ def _pool(n, seed):
     return out
 
 
-def test_mix_equals_alone():
+@pytest.mark.parametrize("large_work", [0, 3_000_000])
+def test_mix_equals_alone(large_work):
+    # large_work = 3e6 makes the requests of the bigger graphs (nnz * K above
+    # it) run in launches of their own beside batches of the small ones
     import torch
     pool = _pool(8, 1)
-    srv = acc.Server(max_batch=8, max_inflight=2, form=acc.FORM_SPLIT_BF16)
+    srv = acc.Server(max_batch=8, max_inflight=3, form=acc.FORM_SPLIT_BF16,
+                     large_work=large_work)
     gids = [srv.register(c) for c in pool]
     graphs = [acc.DeviceGraph(c) for c in pool]
     rng = np.random.default_rng(2)

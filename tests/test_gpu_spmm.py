@@ -91,6 +91,36 @@ def test_column_panels(form, panels, split):
 
 
 @pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+@pytest.mark.parametrize("case", ["long_windows", "many_windows"])
+def test_sweep_equals_work_units(form, case):
+    # The resident-window sweep runs the work-unit schedule's pieces in
+    # another order with the same sums: C must be bit-identical.
+    # "many_windows" has more windows than one round of the sweep holds.
+    torch = _torch()
+    rng = np.random.default_rng(77 if case == "long_windows" else 78)
+    if case == "long_windows":
+        rows, cols, dens = 803, 6000, 0.04
+    else:
+        rows, cols, dens = 180001, 9000, 0.0006
+    rp, ci, va = random_csr(rng, rows, cols, dens, empty_rows=0.05, heavy_rows=3)
+    csr = acc.CSR(rows, cols, rp, ci, va)
+    for rw in (8, 16):
+        g = acc.DeviceGraph(csr, acc.default_build_opts(rw))
+        for K, nb, split, panels in ((100, 2, 0, 4), (128, 4, 24, 8), (32, 1, 0, 0)):
+            B = torch.from_numpy(acc.gen_dense(K, cols, K)).cuda()
+            outs = []
+            for sweep in (0, 1):
+                tile = acc.Tile(nb, 4, split, 0, 64, panels, sweep)
+                outs.append(g.spmm(B, None, form, tile).clone())
+            torch.cuda.synchronize()
+            assert torch.equal(outs[0], outs[1]), (rw, K, nb, split, panels)
+        if case == "long_windows":
+            mx, fr, _ = run_case(csr, 128, form, tile=acc.Tile(4, 8, 0, 0, 64, 8, 1), rw=rw,
+                                 seed=5, graph=g)
+            assert mx <= TOL[form], (rw, mx, fr)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
 def test_row_window_16(form):
     rng = np.random.default_rng(16)
     rp, ci, va = random_csr(rng, 501, 640, 0.02, empty_rows=0.1, heavy_rows=2)
