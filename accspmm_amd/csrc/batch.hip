@@ -143,6 +143,7 @@ accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
   {
     std::lock_guard<std::mutex> lock(h->mu);
     ACC_TRY(get_schedule(h, t, s, &out->sched));
+    if (form != 0) ACC_TRY(ensure_vsplit(h, s));
   }
   out->tasks = (uint64_t)out->sched->num_units * out->n_kc;
   out->bconv_bytes =
@@ -232,6 +233,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     g.bits = h->bits.as<unsigned long long>();
     g.a2b = h->a2b.as<uint32_t>();
     g.data = h->data.as<float>();
+    g.vsplit = h->vsplit.as<uint32_t>();
     g.units = q.sched->units.as<WorkUnit>();
     g.split_info = q.sched->split_info.as<uint32_t>();
     g.C = p.C;

@@ -134,9 +134,12 @@ struct accspmm_metcf {
       sweep_plans;
   mutable accspmm::DevBuf bconv;    // converted-B workspace (BF16 forms)
   mutable accspmm::DevBuf partials; // split-window partial sums
+  // A values as bf16 hi | lo << 16 (BF16 forms), kValuePad zeros after;
+  // built on first use, counted in device_bytes from then on
+  mutable accspmm::DevBuf vsplit;
   uint64_t device_bytes() const {
     return rwo.bytes + tco.bytes + bits.bytes + a2b.bytes + data.bytes +
-           grp.bytes + rix.bytes;
+           grp.bytes + rix.bytes + vsplit.bytes;
   }
 };
 
@@ -149,4 +152,6 @@ accspmm_status inclusive_scan_u32(const uint32_t* in, uint32_t* out,
 // spmm.hip
 accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
                             hipStream_t s, const Schedule** out);
+// builds h->vsplit if absent (caller holds h->mu)
+accspmm_status ensure_vsplit(const accspmm_metcf* h, hipStream_t s);
 }  // namespace accspmm

@@ -288,6 +288,8 @@ extern "C" accspmm_status accspmm_server_register(accspmm_server* s,
     std::lock_guard<std::mutex> hl(h->mu);
     const Schedule* sch = nullptr;
     st = get_schedule(h, t, s->setup, &sch);
+    // the BF16 forms' split values are part of the resident graph
+    if (st == ACCSPMM_OK && s->o.form != ACCSPMM_FORM_FP32) st = ensure_vsplit(h, s->setup);
   }
   if (st != ACCSPMM_OK) {
     accspmm_metcf_destroy(h);

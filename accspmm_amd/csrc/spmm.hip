@@ -124,6 +124,25 @@ __global__ __launch_bounds__(256) void convert_b(const float* __restrict__ B, ui
   }
 }
 
+// The METCF values as bf16 hi | lo << 16 (round to nearest even, lo = the
+// rounded remainder), the same split the kernels apply to B, followed by
+// kValuePad zeros.
+__global__ void k_split_values(const float* __restrict__ data, uint32_t nnz,
+                               uint32_t* __restrict__ out) {
+  const uint64_t n = (uint64_t)nnz + kValuePad;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t v = 0;
+    if (e < nnz) {
+      const float x = data[e];
+      const uint32_t hp = pack_bf16(x, 0.f) & 0xFFFFu;
+      const uint32_t lp = pack_bf16(x - bf16_lo_to_f32(hp), 0.f) & 0xFFFFu;
+      v = hp | (lp << 16);
+    }
+    out[e] = v;
+  }
+}
+
 // For window w and panel boundary p in [1, P): the number of the window's
 // tiles whose first condensed column lies below column p * pc.  Tiles of a
 // window are in ascending column order, so this is a lower bound search.
@@ -381,6 +400,19 @@ void default_tile(const accspmm_metcf* h, uint32_t K, int form,
   }
 }
 
+accspmm_status ensure_vsplit(const accspmm_metcf* h, hipStream_t s) {
+  if (h->vsplit.p) return ACCSPMM_OK;
+  const uint64_t n = (uint64_t)h->nnz + kValuePad;
+  ACC_TRY(h->vsplit.alloc(n * 4));
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_split_values, dim3(blocks), dim3(256), 0, s, h->data.as<float>(),
+                     h->nnz, h->vsplit.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  // other streams (batched and served launches) read it next
+  ACC_HIP(hipStreamSynchronize(s));
+  return ACCSPMM_OK;
+}
+
 // sweep.hip: the resident-window sweep over the same pieces
 accspmm_status sweep_launch(const accspmm_metcf* h, const dev::SpmmArgs& g,
                             const accspmm_tile& tile, uint32_t nb, int form,
@@ -414,6 +446,10 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   g.bits = h->bits.as<unsigned long long>();
   g.a2b = h->a2b.as<uint32_t>();
   g.data = h->data.as<float>();
+  if (form != kFP32) {
+    ACC_TRY(ensure_vsplit(h, s));
+    g.vsplit = h->vsplit.as<uint32_t>();
+  }
   g.units = sch->units.as<WorkUnit>();
   g.split_info = sch->split_info.as<uint32_t>();
   g.C = C;

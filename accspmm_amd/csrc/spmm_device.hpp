@@ -24,6 +24,7 @@ struct SpmmArgs {
   const unsigned long long* __restrict__ bits;
   const uint32_t* __restrict__ a2b;
   const float* __restrict__ data;
+  const uint32_t* __restrict__ vsplit;  // BF16 forms: data as bf16 hi | lo << 16
   const WorkUnit* __restrict__ units;
   const uint32_t* __restrict__ split_info;  // per split window: (window, first slot, count)
   float* __restrict__ partials;
@@ -178,9 +179,13 @@ __device__ inline void next_group(GroupPos& p, const uint32_t* __restrict__ rwo)
 typedef uint32_t u4a __attribute__((ext_vector_type(4), aligned(4)));
 typedef uint32_t u2a __attribute__((ext_vector_type(2), aligned(4)));
 
-// The value lane n ^ 1 holds (DPP quad permutation [1,0,3,2]).
-__device__ inline uint32_t from_pair(uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);
+// The value the even (quad permutation [0,0,2,2]) or odd ([1,1,3,3]) lane
+// of this lane's pair holds.
+__device__ inline uint32_t from_even(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xA0, 0xF, 0xF, false);
+}
+__device__ inline uint32_t from_odd(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xF5, 0xF, 0xF, false);
 }
 
 template <int RW>
@@ -215,20 +220,19 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
                                  const SpmmArgs& g, uint32_t n, bool a_rows,
                                  uint32_t feat, const void* B, uint32_t ldb,
                                  uint32_t lane) {
-  const bool odd = (n & 1u) != 0u;
+  // columns 0-3 come from the even lane of the pair, 4-7 from the odd one;
+  // the bitmap from the even lane, tcOffset[t] from the odd one
   uint32_t col[8];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const uint32_t other = from_pair(mt.half[j]);
-    col[j] = odd ? other : mt.half[j];
-    col[4 + j] = odd ? mt.half[j] : other;
+    col[j] = from_even(mt.half[j]);
+    col[4 + j] = from_odd(mt.half[j]);
   }
   unsigned long long bw;
   uint32_t off;
   if (RW == 8) {
-    const uint32_t px = from_pair(mt.x), py = from_pair(mt.y);
-    bw = odd ? ((unsigned long long)py << 32 | px) : ((unsigned long long)mt.y << 32 | mt.x);
-    off = odd ? mt.x : px;
+    bw = (unsigned long long)from_even(mt.y) << 32 | from_even(mt.x);
+    off = from_odd(mt.x);
   } else {
     const unsigned long long w0 = (unsigned long long)mt.w1 << 32 | mt.w0;
     const unsigned long long w1 = (unsigned long long)mt.w3 << 32 | mt.w2;
@@ -242,9 +246,19 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
     if (FORM == kFP32) bm |= (col[j] != kPad ? 1u : 0u) << j;
   }
   st.bmask = bm;
-  // the group's first value: tcOffset of quarter 0's tile (lane 0)
+  // the group's first value: tcOffset of quarter 0's tile (lane 0); the
+  // BF16 forms read the values already split into bf16 hi | lo << 16
   const uint32_t vbase = __builtin_amdgcn_readfirstlane(off);
-  const u4a v = *reinterpret_cast<const u4a*>(g.data + vbase + 4u * lane);
+  const uint32_t* vsrc = FORM == kFP32 ? reinterpret_cast<const uint32_t*>(g.data) : g.vsplit;
+  // RW=8: lanes past the group's values (it ends at tcOffset[t3 + 1],
+  // which the odd lane 49 of quarter 3 holds; a padding tile there gives
+  // nnz, clamped to the span) re-read the span's first 16 bytes, so only
+  // the cache lines holding values are requested.  Every lane still issues
+  // the load: the wait counts of the pipeline stay static.
+  uint32_t nv = kGroupValues;
+  if (RW == 8) nv = min(__builtin_amdgcn_readlane(mt.y, 49) - vbase, kGroupValues);
+  const uint32_t vi = 4u * lane < nv ? 4u * lane : 0u;
+  const u4a v = *reinterpret_cast<const u4a*>(vsrc + vbase + vi);
   st.vals = u4{v.x, v.y, v.z, v.w};
   const uint32_t sh = 8u * (n & 7u);
   st.byte = a_rows ? (uint32_t)(bw >> sh) & 0xFFu : 0u;
@@ -258,12 +272,12 @@ __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
   // the group's value span through the wave's LDS buffer: a set bitmap bit
   // j reads the row's next value, a clear one the zero after the span
   *reinterpret_cast<u4*>(lv + 4u * lane) = st.vals;
-  float a[8];
+  uint32_t a[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     const bool on = (st.byte >> j) & 1u;
     const uint32_t rank = __popc(st.byte & ((1u << j) - 1u));
-    a[j] = lv[on ? st.base + rank : kGroupValues];
+    a[j] = __builtin_bit_cast(uint32_t, lv[on ? st.base + rank : kGroupValues]);
   }
   if (FORM == kFP32) {
 #pragma unroll
@@ -277,25 +291,22 @@ __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
 #pragma unroll
       for (int b = 0; b < NB; ++b)
         acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-            a[j], __builtin_bit_cast(float, st.b[j].w[b]), acc[b], 0, 0, 0);
+            __builtin_bit_cast(float, a[j]), __builtin_bit_cast(float, st.b[j].w[b]),
+            acc[b], 0, 0, 0);
     return;
   }
-  // A operand: 8 bf16 along the tile's condensed columns
+  // A operand: 8 bf16 along the tile's condensed columns, picked from the
+  // split values (hi in the low half, lo in the high half); RW=8 split:
+  // rows 8-15 carry the low halves
   u4 aop, alo;
   {
-    uint32_t hi[4];
-#pragma unroll
-    for (int d = 0; d < 4; ++d) hi[d] = pack_bf16(a[2 * d], a[2 * d + 1]);
+    const uint32_t sel_hi = 0x05040100u, sel_lo = 0x07060302u;
+    const uint32_t sel = (FORM == kSPLIT && RW == 8 && m >= 8) ? sel_lo : sel_hi;
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
-      float l0 = a[2 * d] - bf16_lo_to_f32(hi[d]);
-      float l1 = a[2 * d + 1] - bf16_hi_to_f32(hi[d]);
-      alo[d] = FORM == kSPLIT ? pack_bf16(l0, l1) : 0u;
-    }
-    if (FORM == kSPLIT && RW == 8 && m >= 8) {
-      aop = alo;  // rows 8-15 carry the low halves
-    } else {
-      aop = u4{hi[0], hi[1], hi[2], hi[3]};
+      aop[d] = __builtin_amdgcn_perm(a[2 * d + 1], a[2 * d], sel);
+      alo[d] = (FORM == kSPLIT && RW == 16) ? __builtin_amdgcn_perm(a[2 * d + 1], a[2 * d], sel_lo)
+                                            : 0u;
     }
   }
   const bf16x8 A = __builtin_bit_cast(bf16x8, aop);
