@@ -52,6 +52,7 @@ class Status:
     ERR_MTX = 9
     ERR_NOT_READY = 10
     ERR_SHAPE = 11
+    ERR_FILE = 12
 
 
 class AccSpMMError(RuntimeError):
@@ -131,6 +132,8 @@ _SIGS = {
     "accspmm_metcf_destroy": (C.c_int, [C.c_void_p]),
     "accspmm_metcf_get_info": (C.c_int, [C.c_void_p, C.POINTER(MetcfInfo)]),
     "accspmm_metcf_export": (C.c_int, [C.c_void_p] + [C.c_void_p] * 7),
+    "accspmm_metcf_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "accspmm_metcf_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_void_p]),
     "accspmm_spmm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Tile),
                                C.c_void_p]),
@@ -163,6 +166,8 @@ _SIGS = {
     "accspmm_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
     "accspmm_server_register": (C.c_int, [C.c_void_p, C.POINTER(CsrView),
                                           C.POINTER(C.c_uint32)]),
+    "accspmm_server_register_file": (C.c_int, [C.c_void_p, C.c_char_p,
+                                               C.POINTER(C.c_uint32)]),
     "accspmm_server_unregister": (C.c_int, [C.c_void_p, C.c_uint32]),
     "accspmm_server_submit": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                                         C.c_void_p, C.c_uint32, C.c_uint32,
@@ -336,6 +341,26 @@ class DeviceGraph:
         self._dev = None  # the handle does not keep the CSR
         torch.cuda.synchronize()
 
+    def save(self, path) -> None:
+        """Writes the graph to a binary file (accspmm_metcf_save)."""
+        _check(lib().accspmm_metcf_save(self.handle, os.fsencode(path)), "metcf_save")
+
+    @classmethod
+    def load(cls, path, stream=None) -> "DeviceGraph":
+        """A graph from a file written by save() (accspmm_metcf_load)."""
+        import torch
+        h = C.c_void_p()
+        _check(lib().accspmm_metcf_load(os.fsencode(path), C.byref(h), _stream(stream)),
+               "metcf_load")
+        g = cls.__new__(cls)
+        g.handle = h
+        g.csr = None
+        g._dev = None
+        inf = g.info
+        g.opts = default_build_opts(inf.row_window, build_adpbme=1 if inf.num_groups else 0)
+        torch.cuda.synchronize()
+        return g
+
     def close(self):
         if getattr(self, "handle", None):
             lib().accspmm_metcf_destroy(self.handle)
@@ -392,7 +417,7 @@ class DeviceGraph:
         assert B.stride(1) == 1
         K = B.shape[1]
         if C_out is None:
-            C_out = torch.empty((self.csr.rows, K), dtype=torch.float32, device=B.device)
+            C_out = torch.empty((self.info.rows, K), dtype=torch.float32, device=B.device)
         st = lib().accspmm_spmm(self.handle, B.data_ptr(), B.stride(0),
                                 C_out.data_ptr(), C_out.stride(0), K, form,
                                 C.byref(tile) if tile is not None else None,
@@ -554,6 +579,13 @@ class Server:
         torch.cuda.synchronize()
         _check(lib().accspmm_server_register(self.handle, C.byref(view), C.byref(gid)),
                "server_register")
+        return gid.value
+
+    def register_file(self, path) -> int:
+        """Registers a graph saved by DeviceGraph.save (warm start)."""
+        gid = C.c_uint32()
+        _check(lib().accspmm_server_register_file(self.handle, os.fsencode(path),
+                                                  C.byref(gid)), "server_register_file")
         return gid.value
 
     def unregister(self, gid: int) -> None:

@@ -272,16 +272,11 @@ extern "C" accspmm_status accspmm_server_create(const accspmm_server_opts* opts,
   return ACCSPMM_OK;
 }
 
-extern "C" accspmm_status accspmm_server_register(accspmm_server* s,
-                                                  const accspmm_csr_view* csr,
-                                                  uint32_t* graph_id) {
-  if (!s || !csr || !graph_id) return ACCSPMM_ERR_INVALID_ARG;
-  accspmm_build_opts bo;
-  accspmm_build_opts_default(&bo);
-  bo.build_adpbme = 0;
-  accspmm_metcf* h = nullptr;
-  ACC_TRY(accspmm_metcf_build(csr, &bo, &h, s->setup));
-  // prepare the default schedule now so the first request does not
+namespace {
+// Takes ownership of h: prepares its default schedule (and the BF16 forms'
+// split values) so the first request does not, then charges it against the
+// budget.
+accspmm_status register_handle(accspmm_server* s, accspmm_metcf* h, uint32_t* graph_id) {
   accspmm_tile t;
   accspmm_status st = accspmm_tile_select(h, 128, (accspmm_form)s->o.form, &t);
   if (st == ACCSPMM_OK) {
@@ -307,6 +302,27 @@ extern "C" accspmm_status accspmm_server_register(accspmm_server* s,
   s->graphs[id] = accspmm_server::Graph{h, bytes, 0};
   *graph_id = id;
   return ACCSPMM_OK;
+}
+}  // namespace
+
+extern "C" accspmm_status accspmm_server_register(accspmm_server* s,
+                                                  const accspmm_csr_view* csr,
+                                                  uint32_t* graph_id) {
+  if (!s || !csr || !graph_id) return ACCSPMM_ERR_INVALID_ARG;
+  accspmm_build_opts bo;
+  accspmm_build_opts_default(&bo);
+  bo.build_adpbme = 0;
+  accspmm_metcf* h = nullptr;
+  ACC_TRY(accspmm_metcf_build(csr, &bo, &h, s->setup));
+  return register_handle(s, h, graph_id);
+}
+
+extern "C" accspmm_status accspmm_server_register_file(accspmm_server* s, const char* path,
+                                                       uint32_t* graph_id) {
+  if (!s || !path || !graph_id) return ACCSPMM_ERR_INVALID_ARG;
+  accspmm_metcf* h = nullptr;
+  ACC_TRY(accspmm_metcf_load(path, &h, s->setup));
+  return register_handle(s, h, graph_id);
 }
 
 extern "C" accspmm_status accspmm_server_unregister(accspmm_server* s,

@@ -223,6 +223,7 @@ extern "C" const char* accspmm_status_string(accspmm_status s) {
     case ACCSPMM_ERR_MTX: return "Matrix Market parse error";
     case ACCSPMM_ERR_NOT_READY: return "not ready";
     case ACCSPMM_ERR_SHAPE: return "operand shapes do not agree";
+    case ACCSPMM_ERR_FILE: return "METCF file unreadable, corrupt or of another format";
   }
   return "unknown status";
 }

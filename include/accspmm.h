@@ -35,7 +35,8 @@ typedef enum accspmm_status {
   ACCSPMM_ERR_QUEUE_FULL = 8,     /* serving: request queue full              */
   ACCSPMM_ERR_MTX = 9,            /* Matrix Market parse error                */
   ACCSPMM_ERR_NOT_READY = 10,     /* serving: request still in flight         */
-  ACCSPMM_ERR_SHAPE = 11          /* operand shapes do not agree              */
+  ACCSPMM_ERR_SHAPE = 11,         /* operand shapes do not agree              */
+  ACCSPMM_ERR_FILE = 12           /* METCF file unreadable, corrupt or wrong  */
 } accspmm_status;
 
 const char* accspmm_status_string(accspmm_status s);
@@ -113,6 +114,15 @@ accspmm_status accspmm_metcf_export(const accspmm_metcf* h,
                                     uint32_t* sparseA2B, float* data,
                                     uint32_t* groupOffset,
                                     uint32_t* rowIndices);
+
+/* Binary save / load of a built graph (no reference counterpart: the
+ * reference rebuilds the format from the .mtx file on every run,
+ * src/mma_tf32.cu:195-226).  The file holds the build options, the arrays
+ * accspmm_metcf_export returns and a checksum; load validates all of it
+ * (ACCSPMM_ERR_FILE otherwise) and gives a handle equal to the saved one. */
+accspmm_status accspmm_metcf_save(const accspmm_metcf* h, const char* path);
+accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** out,
+                                  void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* SpMM                                                                     */
@@ -290,6 +300,9 @@ accspmm_status accspmm_server_create(const accspmm_server_opts* opts,
 accspmm_status accspmm_server_register(accspmm_server* s,
                                        const accspmm_csr_view* csr,
                                        uint32_t* graph_id);
+/* Registers a resident graph saved by accspmm_metcf_save (warm start). */
+accspmm_status accspmm_server_register_file(accspmm_server* s, const char* path,
+                                            uint32_t* graph_id);
 accspmm_status accspmm_server_unregister(accspmm_server* s, uint32_t graph_id);
 /* Submits C = A_g * B (device buffers, caller-owned until completion). */
 accspmm_status accspmm_server_submit(accspmm_server* s, uint32_t graph_id,

@@ -132,3 +132,52 @@ def test_empty_matrix_rows_without_nonzeros():
     exp = O.adpbme_build(8, 19, rp, [], [])
     _assert_same(got, exp, ALL)
     assert g.info.num_tiles == 0
+
+
+@pytest.mark.parametrize("rw,adpbme", [(8, 1), (8, 0), (16, 1)])
+def test_save_load_round_trip(tmp_path, rw, adpbme):
+    # a saved graph loads back to the same arrays and the same SpMM bits
+    import torch
+    rng = np.random.default_rng(40 + rw)
+    rows, cols = 2003, 1500
+    rp, ci, va = random_csr(rng, rows, cols, 0.01, empty_rows=0.1, heavy_rows=2)
+    g = acc.DeviceGraph(acc.CSR(rows, cols, rp, ci, va),
+                        acc.default_build_opts(rw, build_adpbme=adpbme))
+    path = tmp_path / "g.metcf"
+    g.save(path)
+    h = acc.DeviceGraph.load(path)
+    a, b = g.export(), h.export()
+    assert sorted(a) == sorted(b)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    ia, ib = g.info, h.info
+    for f in ("rows", "cols", "nnz", "row_window", "num_windows", "num_tiles", "num_groups",
+              "ibd", "balanced", "max_window_tiles"):
+        assert getattr(ia, f) == getattr(ib, f), f
+    assert g.tune_key(64) == h.tune_key(64)
+    B = torch.from_numpy(acc.gen_dense(3, cols, 64)).cuda()
+    for form in (acc.FORM_FP32, acc.FORM_SPLIT_BF16, acc.FORM_BF16):
+        assert torch.equal(g.spmm(B, None, form), h.spmm(B, None, form))
+
+
+def test_load_rejects_bad_files(tmp_path):
+    rng = np.random.default_rng(41)
+    rp, ci, va = random_csr(rng, 300, 300, 0.02)
+    g = acc.DeviceGraph(acc.CSR(300, 300, rp, ci, va))
+    good = tmp_path / "g.metcf"
+    g.save(good)
+    raw = bytearray(good.read_bytes())
+    cases = {
+        "flipped": raw[:100] + bytes([raw[100] ^ 1]) + raw[101:],
+        "truncated": raw[:-9],
+        "extended": raw + b"\0",
+        "magic": b"XCCSPMTC" + raw[8:],
+    }
+    for name, data in cases.items():
+        bad = tmp_path / f"{name}.metcf"
+        bad.write_bytes(bytes(data))
+        with pytest.raises(acc.AccSpMMError) as e:
+            acc.DeviceGraph.load(bad)
+        assert e.value.status == acc.Status.ERR_FILE, name
+    with pytest.raises(acc.AccSpMMError):
+        acc.DeviceGraph.load(tmp_path / "missing.metcf")

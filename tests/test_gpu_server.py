@@ -123,3 +123,23 @@ def test_budget_queue_and_unknown_graph():
     srv.unregister(gid)
     assert srv.stats().resident_bytes == 0
     srv.close()
+
+
+def test_register_file_warm_start(tmp_path):
+    # a graph saved once is served from the file without a rebuild
+    import torch
+    c = _pool(1, 9)[0]
+    g = acc.DeviceGraph(c)
+    path = tmp_path / "g.metcf"
+    g.save(path)
+    srv = acc.Server(form=acc.FORM_SPLIT_BF16)
+    gid = srv.register_file(path)
+    B = torch.from_numpy(acc.gen_dense(11, c.cols, 64)).cuda()
+    Cm = torch.zeros((c.rows, 64), device="cuda")
+    rid = srv.submit(gid, B, Cm)
+    srv.drain()
+    assert srv.poll(rid) is not None
+    assert torch.equal(Cm, g.spmm(B, None, acc.FORM_SPLIT_BF16))
+    with pytest.raises(acc.AccSpMMError):
+        srv.register_file(tmp_path / "missing.metcf")
+    srv.close()
