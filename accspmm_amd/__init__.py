@@ -158,6 +158,7 @@ _SIGS = {
     "accspmm_read_mtx": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(HostCsr),
                                    C.POINTER(C.c_int)]),
     "accspmm_csr_is_canonical": (C.c_int, [C.POINTER(HostCsr)]),
+    "accspmm_read_mtx_threads_used": (C.c_int, []),
     "accspmm_gen_chung_lu": (C.c_int, [C.c_uint32, C.c_double, C.c_double, C.c_uint64,
                                        C.c_int, C.POINTER(HostCsr)]),
     "accspmm_gen_stencil27": (C.c_int, [C.c_uint32, C.c_uint64, C.POINTER(HostCsr)]),

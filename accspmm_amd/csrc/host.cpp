@@ -1,11 +1,19 @@
 // host.cpp — host-side utilities of libaccspmm: Matrix Market ingest, CSR
 // construction and deterministic graph generators.  No GPU code here.
+#include <fcntl.h>
+#include <sched.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -27,14 +35,47 @@ extern "C" void accspmm_host_csr_free(accspmm_host_csr* c) {
 
 namespace {
 
+// malloc-compatible allocation (released with std::free); blocks of 4 MiB and
+// more are 2 MiB aligned and marked for transparent huge pages, which cuts
+// the first-touch page faults that dominate filling fresh ingest arrays.
+void* host_alloc(size_t bytes) {
+  constexpr size_t kHuge = 2u << 20;
+  if (bytes < 2 * kHuge) return std::malloc(bytes ? bytes : 1);
+  void* p = nullptr;
+  if (posix_memalign(&p, kHuge, bytes) != 0) return nullptr;
+  madvise(p, bytes, MADV_HUGEPAGE);
+  return p;
+}
+
+template <class T>
+struct HostAllocator {
+  using value_type = T;
+  HostAllocator() = default;
+  template <class U>
+  HostAllocator(const HostAllocator<U>&) {}
+  T* allocate(size_t n) {
+    void* p = host_alloc(n * sizeof(T));
+    if (!p) throw std::bad_alloc();
+    return static_cast<T*>(p);
+  }
+  void deallocate(T* p, size_t) { std::free(p); }
+  template <class U>
+  bool operator==(const HostAllocator<U>&) const { return true; }
+  template <class U>
+  bool operator!=(const HostAllocator<U>&) const { return false; }
+};
+
+template <class T>
+using HostVec = std::vector<T, HostAllocator<T>>;
+
 bool alloc_csr(accspmm_host_csr* out, uint32_t rows, uint32_t cols,
                uint64_t nnz) {
   out->rows = rows;
   out->cols = cols;
   out->nnz = (uint32_t)nnz;
-  out->row_ptr = (uint32_t*)std::malloc(sizeof(uint32_t) * (rows + 1ull));
-  out->col_idx = (uint32_t*)std::malloc(sizeof(uint32_t) * (nnz ? nnz : 1));
-  out->val = (float*)std::malloc(sizeof(float) * (nnz ? nnz : 1));
+  out->row_ptr = (uint32_t*)host_alloc(sizeof(uint32_t) * (rows + 1ull));
+  out->col_idx = (uint32_t*)host_alloc(sizeof(uint32_t) * (nnz ? nnz : 1));
+  out->val = (float*)host_alloc(sizeof(float) * (nnz ? nnz : 1));
   if (!out->row_ptr || !out->col_idx || !out->val) {
     accspmm_host_csr_free(out);
     return false;
@@ -42,40 +83,119 @@ bool alloc_csr(accspmm_host_csr* out, uint32_t rows, uint32_t cols,
   return true;
 }
 
-// Sorts each row's (col, val) pairs by column; duplicates are summed.
-void canonicalize_rows(accspmm_host_csr* c) {
-  std::vector<uint32_t> perm;
-  std::vector<uint32_t> ctmp;
-  std::vector<float> vtmp;
-  uint32_t out = 0;
-  uint32_t start = 0;
-  for (uint32_t r = 0; r < c->rows; ++r) {
-    uint32_t b = start, e = c->row_ptr[r + 1];
-    start = e;
-    uint32_t n = e - b;
-    perm.resize(n);
-    std::iota(perm.begin(), perm.end(), 0u);
-    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t x, uint32_t y) {
-      return c->col_idx[b + x] < c->col_idx[b + y];
-    });
-    ctmp.resize(n);
-    vtmp.resize(n);
-    for (uint32_t i = 0; i < n; ++i) {
-      ctmp[i] = c->col_idx[b + perm[i]];
-      vtmp[i] = c->val[b + perm[i]];
-    }
-    c->row_ptr[r] = out;
-    for (uint32_t i = 0; i < n; ++i) {
-      if (out > c->row_ptr[r] && c->col_idx[out - 1] == ctmp[i]) {
-        c->val[out - 1] += vtmp[i];
-      } else {
-        c->col_idx[out] = ctmp[i];
-        c->val[out] = vtmp[i];
-        ++out;
+// Worker threads for host ingest: ACCSPMM_HOST_THREADS when set, else the
+// CPUs this process may run on, at most 16.
+int host_threads() {
+  if (const char* e = std::getenv("ACCSPMM_HOST_THREADS")) {
+    const int v = std::atoi(e);
+    if (v >= 1) return std::min(v, 256);
+  }
+  int n = 0;
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) n = CPU_COUNT(&set);
+  if (n <= 0) n = (int)std::thread::hardware_concurrency();
+  return std::max(1, std::min(n, 16));
+}
+
+// Runs f(i) for i in [0, n) on n threads (the caller's thread takes i = 0).
+template <class F>
+void run_parallel(int n, F&& f) {
+  std::vector<std::thread> ts;
+  ts.reserve(n > 1 ? n - 1 : 0);
+  for (int i = 1; i < n; ++i) ts.emplace_back([&f, i] { f(i); });
+  f(0);
+  for (auto& t : ts) t.join();
+}
+
+// Row blocks of about equal entry counts: block i is rows [b[i], b[i+1]).
+std::vector<uint32_t> row_blocks(const uint32_t* row_ptr, uint32_t rows, int parts) {
+  std::vector<uint32_t> b(parts + 1, rows);
+  b[0] = 0;
+  const uint64_t nnz = row_ptr[rows];
+  for (int i = 1; i < parts; ++i) {
+    const uint64_t target = nnz * (uint64_t)i / (uint64_t)parts;
+    b[i] = (uint32_t)(std::lower_bound(row_ptr, row_ptr + rows + 1, (uint32_t)target) - row_ptr);
+    b[i] = std::min(std::max(b[i], b[i - 1]), rows);
+  }
+  return b;
+}
+
+struct ColVal {
+  uint32_t c;
+  float v;
+};
+
+// Sorts one row's (col, val) pairs stably by column and sums duplicates left
+// to right, in place; returns the row's new length.
+uint32_t canonicalize_row(uint32_t* col, float* val, uint32_t n, std::vector<ColVal>& tmp) {
+  bool sorted = true;
+  for (uint32_t i = 1; i < n && sorted; ++i) sorted = col[i - 1] < col[i];
+  if (sorted) return n;
+  if (n <= 32) {
+    for (uint32_t i = 1; i < n; ++i) {
+      const uint32_t c = col[i];
+      const float v = val[i];
+      uint32_t j = i;
+      for (; j > 0 && col[j - 1] > c; --j) {
+        col[j] = col[j - 1];
+        val[j] = val[j - 1];
       }
+      col[j] = c;
+      val[j] = v;
+    }
+  } else {
+    tmp.resize(n);
+    for (uint32_t i = 0; i < n; ++i) tmp[i] = ColVal{col[i], val[i]};
+    std::stable_sort(tmp.begin(), tmp.end(),
+                     [](const ColVal& a, const ColVal& b) { return a.c < b.c; });
+    for (uint32_t i = 0; i < n; ++i) {
+      col[i] = tmp[i].c;
+      val[i] = tmp[i].v;
     }
   }
-  c->row_ptr[c->rows] = out;
+  uint32_t out = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (out > 0 && col[out - 1] == col[i]) {
+      val[out - 1] += val[i];
+    } else {
+      col[out] = col[i];
+      val[out] = val[i];
+      ++out;
+    }
+  }
+  return out;
+}
+
+// Sorts each row's (col, val) pairs by column; duplicates are summed in file
+// order.  Rows are sorted in parallel blocks, then compacted left to right
+// when duplicates shortened any row.
+void canonicalize_rows(accspmm_host_csr* c) {
+  const uint32_t rows = c->rows;
+  if (rows == 0) return;
+  const int parts = c->nnz >= (1u << 16) ? host_threads() : 1;
+  const std::vector<uint32_t> blk = row_blocks(c->row_ptr, rows, parts);
+  std::vector<uint32_t> len(rows);
+  std::vector<uint8_t> shrunk(parts, 0);
+  run_parallel(parts, [&](int t) {
+    std::vector<ColVal> tmp;
+    for (uint32_t r = blk[t]; r < blk[t + 1]; ++r) {
+      const uint32_t b = c->row_ptr[r], n = c->row_ptr[r + 1] - b;
+      len[r] = canonicalize_row(c->col_idx + b, c->val + b, n, tmp);
+      if (len[r] != n) shrunk[t] = 1;
+    }
+  });
+  if (std::find(shrunk.begin(), shrunk.end(), 1) == shrunk.end()) return;
+  uint32_t out = 0;
+  for (uint32_t r = 0; r < rows; ++r) {
+    const uint32_t b = c->row_ptr[r];
+    if (out != b) {
+      std::memmove(c->col_idx + out, c->col_idx + b, sizeof(uint32_t) * len[r]);
+      std::memmove(c->val + out, c->val + b, sizeof(float) * len[r]);
+    }
+    c->row_ptr[r] = out;
+    out += len[r];
+  }
+  c->row_ptr[rows] = out;
   c->nnz = out;
 }
 
@@ -150,6 +270,303 @@ struct Cursor {
   }
 };
 
+// ---- parallel entry parser ------------------------------------------------
+//
+// The entry section is cut at line starts into one chunk per thread.  Each
+// chunk is parsed as whole lines "row col [value [imag]]"; a line of any other
+// shape (an entry split over lines, a comment, extra tokens, an index out of
+// range) or an entry count other than the size line's makes the caller parse
+// the file with the sequential tokenizer instead, so the result and the error
+// behaviour are those of the sequential reader on every file.
+
+constexpr size_t kParallelMinBytes = 1u << 20;
+constexpr uint64_t kHistogramBudget = 512ull << 20;  // per-group row counters
+
+// 10^0 .. 10^22: the powers of ten that are exact doubles
+constexpr double kPow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,
+                               1e8,  1e9,  1e10, 1e11, 1e12, 1e13, 1e14, 1e15,
+                               1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+
+inline bool is_hspace(char ch) { return ch == ' ' || ch == '\t' || ch == '\r'; }
+inline bool is_ws(char ch) { return is_hspace(ch) || ch == '\n'; }
+
+// A decimal token [sign] digits [. digits] [(e|E) [sign] digits] whose
+// significand is below 2^53 and whose decimal exponent is within +-22 is an
+// exact double times or divided by an exact power of ten, so one IEEE
+// multiply or divide gives the correctly rounded double that strtod returns
+// (Clinger's fast path: W. D. Clinger, "How to read floating point numbers
+// accurately", PLDI 1990).  Anything else returns false for strtod to parse.
+bool fast_decimal(const char* p, const char* e, double* out) {
+  bool neg = false;
+  if (p < e && (*p == '+' || *p == '-')) neg = *p++ == '-';
+  uint64_t mant = 0;
+  int exp10 = 0, digits = 0;
+  for (; p < e && *p >= '0' && *p <= '9'; ++p, ++digits) {
+    mant = mant * 10 + (uint64_t)(*p - '0');
+    if (mant >= (1ull << 53)) return false;
+  }
+  if (p < e && *p == '.') {
+    for (++p; p < e && *p >= '0' && *p <= '9'; ++p, ++digits, --exp10) {
+      mant = mant * 10 + (uint64_t)(*p - '0');
+      if (mant >= (1ull << 53)) return false;
+    }
+  }
+  if (digits == 0) return false;
+  if (p < e && (*p == 'e' || *p == 'E')) {
+    bool eneg = false;
+    if (++p < e && (*p == '+' || *p == '-')) eneg = *p++ == '-';
+    int x = 0, ed = 0;
+    for (; p < e && *p >= '0' && *p <= '9'; ++p) {
+      if (++ed > 4) return false;
+      x = x * 10 + (*p - '0');
+    }
+    if (ed == 0) return false;
+    exp10 += eneg ? -x : x;
+  }
+  if (p != e) return false;
+  double v = 0.0;
+  if (mant != 0) {
+    if (exp10 < -22 || exp10 > 22) return false;
+    v = exp10 < 0 ? (double)mant / kPow10[-exp10] : (double)mant * kPow10[exp10];
+  }
+  *out = neg ? -v : v;
+  return true;
+}
+
+// strtod / strtol over one whitespace-delimited token, as Cursor reads it;
+// false unless the whole token is consumed.
+bool token_double(const char* p, const char* e, double* out) {
+  if (fast_decimal(p, e, out)) return true;
+  char buf[128];
+  const size_t n = (size_t)(e - p);
+  if (n == 0 || n >= sizeof(buf)) return false;
+  std::memcpy(buf, p, n);
+  buf[n] = 0;
+  char* end = nullptr;
+  *out = std::strtod(buf, &end);
+  return end == buf + n;
+}
+
+bool token_long(const char* p, const char* e, long* out) {
+  const char* q = p;
+  bool neg = false;
+  if (q < e && (*q == '+' || *q == '-')) neg = *q++ == '-';
+  if (q < e && e - q <= 18) {
+    long x = 0;
+    for (; q < e && *q >= '0' && *q <= '9'; ++q) x = x * 10 + (*q - '0');
+    if (q == e) {
+      *out = neg ? -x : x;
+      return true;
+    }
+  }
+  char buf[64];
+  const size_t n = (size_t)(e - p);
+  if (n == 0 || n >= sizeof(buf)) return false;
+  std::memcpy(buf, p, n);
+  buf[n] = 0;
+  char* end = nullptr;
+  *out = std::strtol(buf, &end, 10);
+  return end == buf + n;
+}
+
+inline bool line_u32(const char*& p, const char* e, uint32_t* v) {
+  if (p >= e || *p < '0' || *p > '9') return false;
+  uint64_t x = 0;
+  for (; p < e && *p >= '0' && *p <= '9'; ++p) {
+    x = x * 10 + (uint64_t)(*p - '0');
+    if (x > 0xFFFFFFFFull) return false;
+  }
+  *v = (uint32_t)x;
+  return true;
+}
+
+struct Part {
+  HostVec<uint32_t> r, c;  // 0-based, file order
+  HostVec<float> v;
+  uint64_t expanded = 0;  // entries after symmetric expansion
+};
+
+bool parse_lines(const char* p, const char* e, char field, bool sym, uint32_t m,
+                 uint32_t n, Part* out) {
+  const size_t guess = (size_t)(e - p) / 16;
+  out->r.reserve(guess);
+  out->c.reserve(guess);
+  out->v.reserve(guess);
+  const int nvals = field == 'P' ? 0 : field == 'C' ? 2 : 1;
+  while (p < e) {
+    while (p < e && is_hspace(*p)) ++p;
+    if (p == e) break;
+    if (*p == '\n') {
+      ++p;
+      continue;
+    }
+    uint32_t r, c;
+    if (!line_u32(p, e, &r) || p == e || !is_hspace(*p)) return false;
+    while (p < e && is_hspace(*p)) ++p;
+    if (!line_u32(p, e, &c)) return false;
+    double v = 1.0;
+    for (int k = 0; k < nvals; ++k) {
+      if (p == e || !is_hspace(*p)) return false;
+      while (p < e && is_hspace(*p)) ++p;
+      const char* tb = p;
+      while (p < e && !is_ws(*p)) ++p;
+      if (field == 'I') {
+        long iv;
+        if (!token_long(tb, p, &iv)) return false;
+        v = (double)iv;
+      } else {
+        double x;
+        if (!token_double(tb, p, &x)) return false;
+        if (k == 0) v = x;
+      }
+    }
+    while (p < e && is_hspace(*p)) ++p;
+    if (p < e) {
+      if (*p != '\n') return false;
+      ++p;
+    }
+    if (r == 0 || c == 0 || r > m || c > n) return false;
+    out->r.push_back(r - 1);
+    out->c.push_back(c - 1);
+    out->v.push_back((float)v);
+    out->expanded += (sym && r != c) ? 2 : 1;
+  }
+  return true;
+}
+
+// Parses [p, end) on `threads` threads and builds the CSR with the order of
+// coo_to_csr over the sequentially expanded entries: entries are counted per
+// row by groups of consecutive chunks, and group g writes row r's entries
+// after those of groups before it.  Returns false (nothing allocated) when a
+// chunk is irregular or the entry count differs from nz.
+bool read_entries_parallel(const char* p, const char* end, char field, bool sym, uint32_t m,
+                           uint32_t n, uint32_t nz, int threads, accspmm_host_csr* out,
+                           accspmm_status* st) {
+  std::vector<const char*> cut(threads + 1, end);
+  cut[0] = p;
+  const size_t len = (size_t)(end - p);
+  for (int i = 1; i < threads; ++i) {
+    const char* s = p + len * (size_t)i / (size_t)threads;
+    s = std::max(s, cut[i - 1]);
+    const char* nl = s < end ? (const char*)std::memchr(s, '\n', (size_t)(end - s)) : nullptr;
+    cut[i] = nl ? nl + 1 : end;
+  }
+  std::vector<Part> parts(threads);
+  std::vector<uint8_t> ok(threads, 0);
+  run_parallel(threads, [&](int t) {
+    ok[t] = parse_lines(cut[t], cut[t + 1], field, sym, m, n, &parts[t]) ? 1 : 0;
+  });
+  uint64_t entries = 0, total = 0;
+  for (int t = 0; t < threads; ++t) {
+    if (!ok[t]) return false;
+    entries += parts[t].r.size();
+    total += parts[t].expanded;
+  }
+  if (entries != nz) return false;
+  if (total > 0xFFFFFFFFull) {
+    *st = ACCSPMM_ERR_UNSUPPORTED;
+    return true;
+  }
+  if (!alloc_csr(out, m, n, total)) {
+    *st = ACCSPMM_ERR_OUT_OF_MEMORY;
+    return true;
+  }
+  // groups of consecutive chunks, each with a counter per row
+  const uint64_t per_group = 4ull * m + 4;
+  const int groups =
+      (int)std::max<uint64_t>(1, std::min<uint64_t>(threads, kHistogramBudget / per_group));
+  std::vector<HostVec<uint32_t>> cnt(groups);
+  auto group_parts = [&](int g, int* t0, int* t1) {
+    *t0 = g * threads / groups;
+    *t1 = (g + 1) * threads / groups;
+  };
+  run_parallel(groups, [&](int g) {
+    HostVec<uint32_t>& h = cnt[g];
+    h.assign(m, 0u);
+    int t0, t1;
+    group_parts(g, &t0, &t1);
+    for (int t = t0; t < t1; ++t) {
+      const Part& P = parts[t];
+      for (size_t i = 0; i < P.r.size(); ++i) {
+        ++h[P.r[i]];
+        if (sym && P.r[i] != P.c[i]) ++h[P.c[i]];
+      }
+    }
+  });
+  // per-row group offsets and row lengths, then the row prefix
+  const int rthreads = m >= (1u << 16) ? threads : 1;
+  run_parallel(rthreads, [&](int t) {
+    const uint32_t r0 = (uint32_t)((uint64_t)m * t / rthreads);
+    const uint32_t r1 = (uint32_t)((uint64_t)m * (t + 1) / rthreads);
+    for (uint32_t r = r0; r < r1; ++r) {
+      uint32_t s = 0;
+      for (int g = 0; g < groups; ++g) {
+        const uint32_t x = cnt[g][r];
+        cnt[g][r] = s;
+        s += x;
+      }
+      out->row_ptr[r + 1] = s;
+    }
+  });
+  out->row_ptr[0] = 0;
+  for (uint32_t r = 0; r < m; ++r) out->row_ptr[r + 1] += out->row_ptr[r];
+  run_parallel(groups, [&](int g) {
+    HostVec<uint32_t>& h = cnt[g];
+    const uint32_t* rp = out->row_ptr;
+    int t0, t1;
+    group_parts(g, &t0, &t1);
+    for (int t = t0; t < t1; ++t) {
+      const Part& P = parts[t];
+      for (size_t i = 0; i < P.r.size(); ++i) {
+        const uint32_t r = P.r[i], c = P.c[i];
+        const uint32_t d = rp[r] + h[r]++;
+        out->col_idx[d] = c;
+        out->val[d] = P.v[i];
+        if (sym && r != c) {
+          const uint32_t d2 = rp[c] + h[c]++;
+          out->col_idx[d2] = r;
+          out->val[d2] = P.v[i];
+        }
+      }
+    }
+  });
+  *st = ACCSPMM_OK;
+  return true;
+}
+
+thread_local int g_threads_used = 0;
+
+// The whole file, memory-mapped read-only.
+struct MappedFile {
+  const char* data = nullptr;
+  size_t size = 0;
+  void* map = nullptr;
+  ~MappedFile() {
+    if (map) munmap(map, size);
+  }
+  bool open_file(const char* path) {
+    const int fd = ::open(path, O_RDONLY);
+    if (fd < 0) return false;
+    struct stat sb;
+    if (fstat(fd, &sb) != 0 || !S_ISREG(sb.st_mode)) {
+      ::close(fd);
+      return false;
+    }
+    size = (size_t)sb.st_size;
+    if (size > 0) {
+      void* q = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+      if (q == MAP_FAILED) {
+        ::close(fd);
+        return false;
+      }
+      map = q;
+      data = static_cast<const char*>(q);
+    }
+    ::close(fd);
+    return true;
+  }
+};
+
 }  // namespace
 
 // Matrix Market reader.  Accepts what read_from_mtx accepts
@@ -159,30 +576,17 @@ struct Cursor {
 // (include/mmio.hpp:133-158), gives pattern entries 1.0 and casts each value
 // double -> float.  Divergences: integer values are parsed as integers (the
 // reference reads %d into a double); a malformed entry line is an error
-// rather than a repeat of the previous entry.
-extern "C" accspmm_status accspmm_read_mtx(const char* path, int canonicalize,
-                                           accspmm_host_csr* out,
-                                           int* mtx_code) {
-  if (!path || !out) return ACCSPMM_ERR_INVALID_ARG;
-  std::memset(out, 0, sizeof(*out));
-  if (mtx_code) *mtx_code = 0;
-  FILE* f = std::fopen(path, "rb");
-  if (!f) {
+// rather than a repeat of the previous entry.  Files with at least 1 MiB of
+// entries are parsed on several threads (same result; see read_entries_parallel).
+static accspmm_status read_mtx_impl(const char* path, int canonicalize,
+                                    accspmm_host_csr* out, int* mtx_code) {
+  MappedFile file;
+  if (!file.open_file(path)) {
     if (mtx_code) *mtx_code = -1;
     return ACCSPMM_ERR_MTX;
   }
-  std::fseek(f, 0, SEEK_END);
-  long sz = std::ftell(f);
-  std::fseek(f, 0, SEEK_SET);
-  std::string text((size_t)(sz > 0 ? sz : 0), '\0');
-  if (sz > 0 && std::fread(&text[0], 1, (size_t)sz, f) != (size_t)sz) {
-    std::fclose(f);
-    if (mtx_code) *mtx_code = -1;
-    return ACCSPMM_ERR_MTX;
-  }
-  std::fclose(f);
-  const char* p = text.data();
-  const char* end = p + text.size();
+  const char* p = file.data;
+  const char* end = p + file.size;
   auto next_line = [&](std::string* line) -> bool {
     if (p >= end) return false;
     const char* nl = (const char*)std::memchr(p, '\n', (size_t)(end - p));
@@ -255,9 +659,21 @@ extern "C" accspmm_status accspmm_read_mtx(const char* path, int canonicalize,
     nz = got[2];
   }
   bool symmetric = sym == 'S' || sym == 'H';
+  const int threads = host_threads();
+  if (threads > 1 && (size_t)(end - p) >= kParallelMinBytes) {
+    accspmm_status st = ACCSPMM_OK;
+    if (read_entries_parallel(p, end, field, symmetric, m, n, nz, threads, out, &st)) {
+      g_threads_used = threads;
+      if (st == ACCSPMM_OK && canonicalize) canonicalize_rows(out);
+      return st;
+    }
+  }
+  g_threads_used = 1;
   std::vector<uint32_t> R, C;
   std::vector<float> V;
-  R.reserve(symmetric ? 2ull * nz : nz);
+  // an entry takes at least 4 bytes ("1 1\n"): do not trust nz for the reserve
+  const uint64_t fit = std::min<uint64_t>(nz, (uint64_t)(end - p) / 4 + 1);
+  R.reserve(symmetric ? 2 * fit : fit);
   C.reserve(R.capacity());
   V.reserve(R.capacity());
   Cursor cur{p, end};
@@ -294,11 +710,30 @@ extern "C" accspmm_status accspmm_read_mtx(const char* path, int canonicalize,
       V.push_back((float)v);
     }
   }
+  if (R.size() > 0xFFFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
   if (!alloc_csr(out, m, n, R.size())) return ACCSPMM_ERR_OUT_OF_MEMORY;
   coo_to_csr(m, R, C, V, out);
   if (canonicalize) canonicalize_rows(out);
   return ACCSPMM_OK;
 }
+
+extern "C" accspmm_status accspmm_read_mtx(const char* path, int canonicalize,
+                                           accspmm_host_csr* out, int* mtx_code) {
+  if (!path || !out) return ACCSPMM_ERR_INVALID_ARG;
+  std::memset(out, 0, sizeof(*out));
+  if (mtx_code) *mtx_code = 0;
+  g_threads_used = 0;
+  try {
+    const accspmm_status st = read_mtx_impl(path, canonicalize, out, mtx_code);
+    if (st != ACCSPMM_OK) accspmm_host_csr_free(out);
+    return st;
+  } catch (const std::exception&) {  // allocation or thread creation failed
+    accspmm_host_csr_free(out);
+    return ACCSPMM_ERR_OUT_OF_MEMORY;
+  }
+}
+
+extern "C" int accspmm_read_mtx_threads_used(void) { return g_threads_used; }
 
 extern "C" int accspmm_csr_is_canonical(const accspmm_host_csr* c) {
   if (!c || !c->row_ptr) return 0;

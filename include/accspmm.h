@@ -252,9 +252,18 @@ void accspmm_host_csr_free(accspmm_host_csr* c);
  * (r,c) then (c,r), pattern => 1.0) and builds the CSR by stable counting
  * sort (include/format.hpp:110-142).  If `canonicalize` is non-zero each row
  * is then sorted by column and duplicates are summed.  `mtx_code` receives
- * the reference's read_from_mtx code (0, -1 open failure, -2 banner). */
+ * the reference's read_from_mtx code (0, -1 open failure, -2 banner).
+ * Files with at least 1 MiB of entries are parsed line-wise on several
+ * threads (ACCSPMM_HOST_THREADS, default: the usable CPUs, at most 16); a
+ * file whose entries are not one per line falls back to the sequential
+ * tokenizer, so the result never depends on the thread count. */
 accspmm_status accspmm_read_mtx(const char* path, int canonicalize,
                                 accspmm_host_csr* out, int* mtx_code);
+
+/* Threads that parsed the entries in this thread's last accspmm_read_mtx
+ * call (1: the sequential tokenizer).  No reference counterpart: an ingest
+ * diagnostic. */
+int accspmm_read_mtx_threads_used(void);
 
 /* 1 if every row is strictly increasing in column and all columns < cols. */
 int accspmm_csr_is_canonical(const accspmm_host_csr* c);
