@@ -27,7 +27,7 @@ import numpy as np
 
 __all__ = [
     "lib", "Status", "AccSpMMError", "FORM_FP32", "FORM_SPLIT_BF16", "FORM_BF16",
-    "BuildOpts", "Tile", "CSR", "read_mtx", "METCFBit", "AdpBME",
+    "BuildOpts", "Tile", "CSR", "read_mtx", "canonicalize_device", "METCFBit", "AdpBME",
     "balanceStrategy", "DeviceGraph", "Batch", "Server", "gen_chung_lu", "gen_stencil27",
     "gen_dense", "MetcfInfo",
 ]
@@ -129,6 +129,9 @@ _SIGS = {
                                       C.POINTER(C.c_void_p), C.c_void_p]),
     "accspmm_metcf_build_host": (C.c_int, [C.POINTER(CsrView), C.POINTER(BuildOpts),
                                            C.POINTER(C.c_void_p)]),
+    "accspmm_csr_canonicalize": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32),
+                                           C.c_void_p]),
     "accspmm_metcf_destroy": (C.c_int, [C.c_void_p]),
     "accspmm_metcf_get_info": (C.c_int, [C.c_void_p, C.POINTER(MetcfInfo)]),
     "accspmm_metcf_export": (C.c_int, [C.c_void_p] + [C.c_void_p] * 7),
@@ -159,6 +162,7 @@ _SIGS = {
                                    C.POINTER(C.c_int)]),
     "accspmm_csr_is_canonical": (C.c_int, [C.POINTER(HostCsr)]),
     "accspmm_read_mtx_threads_used": (C.c_int, []),
+    "accspmm_write_mtx": (C.c_int, [C.POINTER(HostCsr), C.c_char_p, C.c_int]),
     "accspmm_gen_chung_lu": (C.c_int, [C.c_uint32, C.c_double, C.c_double, C.c_uint64,
                                        C.c_int, C.POINTER(HostCsr)]),
     "accspmm_gen_stencil27": (C.c_int, [C.c_uint32, C.c_uint64, C.POINTER(HostCsr)]),
@@ -258,12 +262,19 @@ class CSR:
         lib().accspmm_host_csr_free(C.byref(h))
         return out
 
+    def _host_view(self) -> HostCsr:
+        return HostCsr(self.rows, self.cols, self.nnz,
+                       self.row_ptr.ctypes.data_as(C.POINTER(C.c_uint32)),
+                       self.col_idx.ctypes.data_as(C.POINTER(C.c_uint32)),
+                       self.data.ctypes.data_as(C.POINTER(C.c_float)))
+
     def is_canonical(self) -> bool:
-        h = HostCsr(self.rows, self.cols, self.nnz,
-                    self.row_ptr.ctypes.data_as(C.POINTER(C.c_uint32)),
-                    self.col_idx.ctypes.data_as(C.POINTER(C.c_uint32)),
-                    self.data.ctypes.data_as(C.POINTER(C.c_float)))
-        return bool(lib().accspmm_csr_is_canonical(C.byref(h)))
+        return bool(lib().accspmm_csr_is_canonical(C.byref(self._host_view())))
+
+    def write_mtx(self, path, symmetric: bool = False) -> None:
+        """Writes a Matrix Market file (accspmm_write_mtx)."""
+        _check(lib().accspmm_write_mtx(C.byref(self._host_view()), os.fsencode(path),
+                                       int(symmetric)), "write_mtx")
 
     def to_device(self, device="cuda"):
         import torch
@@ -322,16 +333,31 @@ def default_build_opts(row_window: int = 8, **kw) -> BuildOpts:
     return o
 
 
+def canonicalize_device(rows: int, cols: int, row_ptr, col_idx, val, stream=None) -> int:
+    """Makes a device CSR (torch int32/float32 tensors) canonical in place
+    (accspmm_csr_canonicalize); returns the new nonzero count."""
+    nnz = C.c_uint32(0)
+    _check(lib().accspmm_csr_canonicalize(rows, cols, col_idx.numel(), row_ptr.data_ptr(),
+                                          col_idx.data_ptr(), val.data_ptr(), C.byref(nnz),
+                                          _stream(stream)), "csr_canonicalize")
+    return nnz.value
+
+
 class DeviceGraph:
-    """A METCF(+AdpBME) graph resident on the GPU (opaque C handle)."""
+    """A METCF(+AdpBME) graph resident on the GPU (opaque C handle).
+    canonicalize=True sorts and de-duplicates the rows on the device first
+    (input rows may then be in any order, with repeated columns)."""
 
     def __init__(self, csr: CSR, opts: BuildOpts | None = None, stream=None,
-                 device="cuda"):
+                 device="cuda", canonicalize: bool = False):
         import torch
         self.csr = csr
         self._dev = csr.to_device(device)
         rp, ci, va = self._dev
-        view = CsrView(csr.rows, csr.cols, csr.nnz, rp.data_ptr(), ci.data_ptr(),
+        nnz = csr.nnz
+        if canonicalize:
+            nnz = canonicalize_device(csr.rows, csr.cols, rp, ci, va, stream)
+        view = CsrView(csr.rows, csr.cols, nnz, rp.data_ptr(), ci.data_ptr(),
                        va.data_ptr())
         self.opts = opts or default_build_opts()
         h = C.c_void_p()

@@ -1,0 +1,188 @@
+// canon.hip — canonical CSR on the device: every row sorted by column
+// (stable), duplicate columns summed left to right in float32, rows
+// compacted.  The result equals the host loader's canonicalisation
+// (accspmm_read_mtx with canonicalize = 1) bit for bit.
+//
+// Why it exists: the METCF builder stores a tile's values in CSR order while
+// the SpMM kernels index them by bitmap popcount (include/format.hpp:238-269
+// against include/mma_tf32.cuh:74-78), so both agree only on rows that are
+// column-sorted and duplicate-free.  Graphs that arrive as device arrays
+// (a caller's own pipeline) are made canonical here without a host round
+// trip.
+//
+// Steps: validate row_ptr; key every entry by (row, column) — rows take the
+// high bits, so one stable LSD radix sort over the row and column bits
+// orders each row by column and keeps equal columns in their input order —
+// then mark the first entry of each (row, column) run, scan the marks into
+// output positions, and let the run heads add their runs in order.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "internal.hpp"
+
+using namespace accspmm;
+
+namespace {
+
+constexpr uint32_t kThreads = 256;
+enum : uint32_t { kBadRowPtr = 1u, kBadCol = 2u, kUnsorted = 4u };
+
+inline uint32_t bit_width(uint32_t x) { return x ? 32u - (uint32_t)__builtin_clz(x) : 0u; }
+
+inline unsigned blocks_for(uint64_t n) {
+  return (unsigned)std::max<uint64_t>(1, (n + kThreads - 1) / kThreads);
+}
+
+__global__ __launch_bounds__(kThreads) void k_check_rows(const uint32_t* rp, uint32_t rows,
+                                                         uint32_t nnz, uint32_t* flags) {
+  const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  bool bad = false;
+  if (r <= rows) {
+    const uint32_t x = rp[r];
+    if (r == 0 && x != 0u) bad = true;
+    if (r == rows && x != nnz) bad = true;
+    if (r < rows && rp[r + 1] < x) bad = true;
+  }
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(flags, kBadRowPtr);
+}
+
+// Largest r in [0, rows) with rp[r] <= i: the row holding entry i when rp is
+// monotone (reads stay inside rp[0, rows) whatever rp holds).
+__device__ inline uint32_t find_row(const uint32_t* rp, uint32_t rows, uint32_t i) {
+  uint32_t lo = 0, hi = rows;
+  while (hi - lo > 1u) {
+    const uint32_t mid = lo + (hi - lo) / 2u;
+    if (rp[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <class Key>
+__global__ __launch_bounds__(kThreads) void k_make_keys(const uint32_t* rp, const uint32_t* col,
+                                                        uint32_t rows, uint32_t cols, uint32_t nnz,
+                                                        uint32_t col_bits, Key* keys,
+                                                        uint32_t* flags) {
+  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  bool bad_col = false, unsorted = false;
+  if (i < nnz) {
+    const uint32_t r = find_row(rp, rows, (uint32_t)i);
+    const uint32_t c = col[i];
+    bad_col = c >= cols;
+    unsorted = i > rp[r] && col[i - 1] >= c;
+    keys[i] = ((Key)r << col_bits) | (Key)c;
+  }
+  const bool any_bad = __syncthreads_or(bad_col);
+  const bool any_unsorted = __syncthreads_or(unsorted);
+  if (threadIdx.x == 0 && (any_bad || any_unsorted))
+    atomicOr(flags, (any_bad ? kBadCol : 0u) | (any_unsorted ? kUnsorted : 0u));
+}
+
+// heads[i] = 1 where a (row, column) run starts; heads[nnz] = 0 so that the
+// exclusive scan's last element is the new nonzero count.
+template <class Key>
+__global__ __launch_bounds__(kThreads) void k_heads(const Key* keys, uint32_t nnz,
+                                                    uint32_t* heads) {
+  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < nnz) heads[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+  else if (i == nnz) heads[i] = 0u;
+}
+
+// Each run head adds its run left to right and writes one entry.
+template <class Key>
+__global__ __launch_bounds__(kThreads) void k_compact(const Key* keys, const float* vals,
+                                                      const uint32_t* heads, const uint32_t* pos,
+                                                      uint32_t nnz, Key col_mask, uint32_t* col,
+                                                      float* val) {
+  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= nnz || !heads[i]) return;
+  float s = vals[i];
+  for (uint64_t j = i + 1; j < nnz && !heads[j]; ++j) s += vals[j];
+  const uint32_t d = pos[i];
+  col[d] = (uint32_t)(keys[i] & col_mask);
+  val[d] = s;
+}
+
+// Rows keep their places in the sorted order, so a row's new start is the
+// number of run heads before its old start.
+__global__ __launch_bounds__(kThreads) void k_remap_rows(uint32_t* rp, uint32_t rows,
+                                                         const uint32_t* pos) {
+  const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (r <= rows) rp[r] = pos[rp[r]];
+}
+
+template <class Key>
+accspmm_status sort_and_compact(uint32_t rows, uint32_t nnz, uint32_t col_bits,
+                                uint32_t key_bits, uint32_t* rp, uint32_t* col, float* val,
+                                const DevBuf& keys_in, uint32_t* nnz_out, hipStream_t s) {
+  DevBuf keys_out, vals_out, heads, pos, temp;
+  ACC_TRY(keys_out.alloc(sizeof(Key) * (size_t)nnz));
+  ACC_TRY(vals_out.alloc(sizeof(float) * (size_t)nnz));
+  size_t temp_bytes = 0;
+  ACC_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys_in.as<Key>(), keys_out.as<Key>(),
+                                    val, vals_out.as<float>(), nnz, 0u, key_bits, s));
+  ACC_TRY(temp.alloc(temp_bytes));
+  ACC_HIP(rocprim::radix_sort_pairs(temp.p, temp_bytes, keys_in.as<Key>(), keys_out.as<Key>(),
+                                    val, vals_out.as<float>(), nnz, 0u, key_bits, s));
+  ACC_TRY(heads.alloc(sizeof(uint32_t) * ((size_t)nnz + 1)));
+  ACC_TRY(pos.alloc(sizeof(uint32_t) * ((size_t)nnz + 1)));
+  k_heads<Key><<<blocks_for((uint64_t)nnz + 1), kThreads, 0, s>>>(keys_out.as<Key>(), nnz,
+                                                                   heads.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  ACC_TRY(exclusive_scan_u32(heads.as<uint32_t>(), pos.as<uint32_t>(), (uint64_t)nnz + 1, s));
+  const Key col_mask = col_bits ? (((Key)1 << col_bits) - 1) : (Key)0;
+  k_compact<Key><<<blocks_for(nnz), kThreads, 0, s>>>(
+      keys_out.as<Key>(), vals_out.as<float>(), heads.as<uint32_t>(), pos.as<uint32_t>(), nnz,
+      col_mask, col, val);
+  ACC_HIP(hipGetLastError());
+  k_remap_rows<<<blocks_for((uint64_t)rows + 1), kThreads, 0, s>>>(rp, rows,
+                                                                    pos.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  ACC_HIP(hipMemcpyAsync(nnz_out, pos.as<uint32_t>() + nnz, sizeof(uint32_t),
+                         hipMemcpyDeviceToHost, s));
+  ACC_HIP(hipStreamSynchronize(s));
+  return ACCSPMM_OK;
+}
+
+template <class Key>
+accspmm_status canonicalize(uint32_t rows, uint32_t cols, uint32_t nnz, uint32_t* rp,
+                            uint32_t* col, float* val, uint32_t* nnz_out, hipStream_t s) {
+  const uint32_t col_bits = bit_width(cols - 1u);
+  const uint32_t key_bits = std::max(1u, col_bits + bit_width(rows - 1u));
+  DevBuf flags, keys;
+  ACC_TRY(flags.alloc(sizeof(uint32_t)));
+  ACC_HIP(hipMemsetAsync(flags.p, 0, sizeof(uint32_t), s));
+  k_check_rows<<<blocks_for((uint64_t)rows + 1), kThreads, 0, s>>>(rp, rows, nnz,
+                                                                    flags.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  ACC_TRY(keys.alloc(sizeof(Key) * (size_t)std::max(nnz, 1u)));
+  if (nnz) {
+    k_make_keys<Key><<<blocks_for(nnz), kThreads, 0, s>>>(rp, col, rows, cols, nnz, col_bits,
+                                                          keys.as<Key>(), flags.as<uint32_t>());
+    ACC_HIP(hipGetLastError());
+  }
+  uint32_t h_flags = 0;
+  ACC_HIP(hipMemcpyAsync(&h_flags, flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  ACC_HIP(hipStreamSynchronize(s));
+  if (h_flags & (kBadRowPtr | kBadCol)) return ACCSPMM_ERR_INVALID_ARG;
+  if (!(h_flags & kUnsorted)) {
+    *nnz_out = nnz;
+    return ACCSPMM_OK;
+  }
+  return sort_and_compact<Key>(rows, nnz, col_bits, key_bits, rp, col, val, keys, nnz_out, s);
+}
+
+}  // namespace
+
+extern "C" accspmm_status accspmm_csr_canonicalize(uint32_t rows, uint32_t cols, uint32_t nnz,
+                                                   uint32_t* row_ptr, uint32_t* col_idx,
+                                                   float* val, uint32_t* nnz_out,
+                                                   void* stream) {
+  if (!row_ptr || !nnz_out || rows == 0 || cols == 0 || (nnz && (!col_idx || !val)))
+    return ACCSPMM_ERR_INVALID_ARG;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t bits = bit_width(cols - 1u) + bit_width(rows - 1u);
+  if (bits <= 32u) return canonicalize<uint32_t>(rows, cols, nnz, row_ptr, col_idx, val, nnz_out, s);
+  return canonicalize<uint64_t>(rows, cols, nnz, row_ptr, col_idx, val, nnz_out, s);
+}

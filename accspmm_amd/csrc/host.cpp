@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <new>
 #include <numeric>
 #include <string>
@@ -746,6 +747,80 @@ extern "C" int accspmm_csr_is_canonical(const accspmm_host_csr* c) {
     }
   }
   return 1;
+}
+
+namespace {
+
+inline void put_u32(std::string& s, uint32_t x) {
+  char buf[10];
+  int n = 0;
+  do {
+    buf[n++] = (char)('0' + x % 10u);
+    x /= 10u;
+  } while (x);
+  while (n) s.push_back(buf[--n]);
+}
+
+// One entry line "row col value" (1-based), the value with 9 significant
+// digits, which reads back to the same float.
+inline void put_entry(std::string& s, uint32_t r, uint32_t c, float v) {
+  put_u32(s, r + 1u);
+  s.push_back(' ');
+  put_u32(s, c + 1u);
+  char buf[32];
+  const int n = std::snprintf(buf, sizeof(buf), " %.9g\n", (double)v);
+  s.append(buf, (size_t)n);
+}
+
+}  // namespace
+
+// Matrix Market writer (coordinate real).  general: every entry, row-major.
+// symmetric: the lower triangle in column-major order — for column c the
+// entries (r, c), r >= c, taken from row c of the CSR (so the CSR must hold a
+// symmetric matrix) — the order in which accspmm_read_mtx's expansion yields
+// sorted rows.  Rows are formatted on several threads and written in order.
+extern "C" accspmm_status accspmm_write_mtx(const accspmm_host_csr* c, const char* path,
+                                            int symmetric) {
+  if (!c || !path || !c->row_ptr || (c->nnz && (!c->col_idx || !c->val)))
+    return ACCSPMM_ERR_INVALID_ARG;
+  if (symmetric && c->rows != c->cols) return ACCSPMM_ERR_SHAPE;
+  try {
+    const int parts = c->nnz >= (1u << 16) ? host_threads() : 1;
+    const std::vector<uint32_t> blk = row_blocks(c->row_ptr, c->rows, parts);
+    std::vector<std::string> text(parts);
+    std::vector<uint64_t> count(parts, 0);
+    run_parallel(parts, [&](int t) {
+      std::string& s = text[t];
+      s.reserve((size_t)(c->row_ptr[blk[t + 1]] - c->row_ptr[blk[t]]) * 24);
+      for (uint32_t r = blk[t]; r < blk[t + 1]; ++r) {
+        for (uint32_t e = c->row_ptr[r]; e < c->row_ptr[r + 1]; ++e) {
+          const uint32_t col = c->col_idx[e];
+          if (symmetric) {
+            if (col < r) continue;
+            put_entry(s, col, r, c->val[e]);  // (col, r) below the diagonal
+          } else {
+            put_entry(s, r, col, c->val[e]);
+          }
+          ++count[t];
+        }
+      }
+    });
+    uint64_t total = 0;
+    for (uint64_t x : count) total += x;
+    std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "wb"), &std::fclose);
+    if (!f) return ACCSPMM_ERR_FILE;
+    std::string head = std::string("%%MatrixMarket matrix coordinate real ") +
+                       (symmetric ? "symmetric\n" : "general\n");
+    head += std::to_string(c->rows) + " " + std::to_string(c->cols) + " " +
+            std::to_string(total) + "\n";
+    bool ok = std::fwrite(head.data(), 1, head.size(), f.get()) == head.size();
+    for (const std::string& s : text)
+      ok = ok && std::fwrite(s.data(), 1, s.size(), f.get()) == s.size();
+    ok = (std::fclose(f.release()) == 0) && ok;
+    return ok ? ACCSPMM_OK : ACCSPMM_ERR_FILE;
+  } catch (const std::exception&) {
+    return ACCSPMM_ERR_OUT_OF_MEMORY;
+  }
 }
 
 // ------------------------------------------------------------------------

@@ -116,6 +116,100 @@ def check_errors(csr, B, results):
     return errs
 
 
+def ingest_bench(workload: str) -> dict:
+    """File to resident graph: the workload graph's pattern with symmetric
+    values (a hash of (min(r,c), max(r,c)) mapped to 24-bit fractions in
+    (0,1], printed with 9 digits) written as a symmetric Matrix Market file
+    (lower triangle, column-major), read back canonicalised by the sequential
+    tokenizer and by the line-parallel reader (both must return that CSR bit
+    for bit), then the METCF build; and the device canonicaliser on the same
+    CSR with every row reversed (must restore it) and on the canonical CSR
+    (check pass only)."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    import accspmm_amd as acc
+    pattern = make_graph(WORKLOADS[workload])
+    rows_of = np.repeat(np.arange(pattern.rows, dtype=np.uint64),
+                        np.diff(pattern.row_ptr.astype(np.int64)))
+    cols64 = pattern.col_idx.astype(np.uint64)
+    lo, hi = np.minimum(rows_of, cols64), np.maximum(rows_of, cols64)
+    del rows_of, cols64
+    h = (lo * np.uint64(0x9E3779B1) + hi * np.uint64(0x85EBCA77)) & np.uint64(0xFFFFFF)
+    del lo, hi
+    vals = ((h + np.uint64(1)).astype(np.float64) / float(1 << 24)).astype(np.float32)
+    del h
+    csr = acc.CSR(pattern.rows, pattern.cols, pattern.row_ptr, pattern.col_idx, vals)
+    del pattern
+    tmp = tempfile.mkdtemp(prefix="accspmm_ingest_")
+    out = {"metric": "mtx_ingest_seconds", "unit": "s", "higher_is_better": False,
+           "data": "synthetic", "config": {"workload": workload, "rows": csr.rows,
+                                           "nnz": csr.nnz}}
+    try:
+        path = os.path.join(tmp, "graph.mtx")
+        t0 = time.perf_counter()
+        csr.write_mtx(path, symmetric=True)
+        out["write_s"] = round(time.perf_counter() - t0, 3)
+        out["file_bytes"] = os.path.getsize(path)
+        log(f"bench: wrote {out['file_bytes']} bytes in {out['write_s']} s")
+        reads, exact = {}, True
+        saved = os.environ.get("ACCSPMM_HOST_THREADS")
+        for th in ("1", ""):
+            if th:
+                os.environ["ACCSPMM_HOST_THREADS"] = th
+            else:
+                os.environ.pop("ACCSPMM_HOST_THREADS", None)
+            t0 = time.perf_counter()
+            got = acc.read_mtx(path, canonicalize=True)
+            dt = time.perf_counter() - t0
+            used = acc.lib().accspmm_read_mtx_threads_used()
+            reads[str(used)] = round(dt, 3)
+            exact = exact and np.array_equal(got.row_ptr, csr.row_ptr) and \
+                np.array_equal(got.col_idx, csr.col_idx) and \
+                np.array_equal(got.data.view(np.uint32), csr.data.view(np.uint32))
+            log(f"bench: read_mtx on {used} thread(s): {dt:.3f} s, exact={exact}")
+            del got
+        if saved is None:
+            os.environ.pop("ACCSPMM_HOST_THREADS", None)
+        else:
+            os.environ["ACCSPMM_HOST_THREADS"] = saved
+        out["read_canonical_s"] = reads
+        out["round_trip_exact"] = bool(exact)
+        torch.cuda.init()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g = acc.DeviceGraph(csr)
+        torch.cuda.synchronize()
+        out["metcf_build_s"] = round(time.perf_counter() - t0, 3)
+        out["file_to_graph_s"] = round(min(reads.values()) + out["metcf_build_s"], 3)
+        del g
+        rp, ci, va = csr.to_device()
+        lens = (rp[1:] - rp[:-1]).long()
+        row_of = torch.repeat_interleave(torch.arange(csr.rows, device="cuda"), lens)
+        pos = torch.arange(csr.nnz, device="cuda")
+        rev = rp.long()[row_of] + rp.long()[row_of + 1] - 1 - pos
+        ci_r, va_r, rp_r = ci[rev].contiguous(), va[rev].contiguous(), rp.clone()
+        del row_of, pos, rev, lens
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n = acc.canonicalize_device(csr.rows, csr.cols, rp_r, ci_r, va_r)
+        out["gpu_canonicalize_reversed_s"] = round(time.perf_counter() - t0, 4)
+        restored = n == csr.nnz and torch.equal(rp_r, rp) and torch.equal(ci_r, ci) and \
+            torch.equal(va_r, va)
+        out["gpu_canonicalize_exact"] = bool(restored)
+        t0 = time.perf_counter()
+        acc.canonicalize_device(csr.rows, csr.cols, rp_r, ci_r, va_r)
+        out["gpu_canonicalize_check_s"] = round(time.perf_counter() - t0, 4)
+        log(f"bench: device canonicalize {out['gpu_canonicalize_reversed_s']} s "
+            f"(reversed rows, exact={restored}), {out['gpu_canonicalize_check_s']} s (check)")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
 def batch_bench(ngraphs: int, steps: int, warmup: int):
     """Batching: ngraphs Chung-Lu graphs (500-50,000 nodes log-uniform, mean
     degree U[4,64], exponent U[2.0,2.5], U(0,1] values, graph seed 1000+i),
@@ -276,6 +370,9 @@ def main():
                     help="requests per serving run (0 skips the serving benchmark)")
     ap.add_argument("--batch-graphs", type=int, default=256,
                     help="graphs in the batched-launch benchmark (0 skips it)")
+    ap.add_argument("--ingest", action="store_true",
+                    help="only measure Matrix Market ingest and canonicalisation of the "
+                         "workload graph (prints its own JSON line)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -291,6 +388,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log(f"bench: copied {4 * n} bytes three times")
+        return
+
+    if args.ingest:
+        print(json.dumps(ingest_bench(args.workload)), flush=True)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

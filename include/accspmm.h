@@ -36,7 +36,8 @@ typedef enum accspmm_status {
   ACCSPMM_ERR_MTX = 9,            /* Matrix Market parse error                */
   ACCSPMM_ERR_NOT_READY = 10,     /* serving: request still in flight         */
   ACCSPMM_ERR_SHAPE = 11,         /* operand shapes do not agree              */
-  ACCSPMM_ERR_FILE = 12           /* METCF file unreadable, corrupt or wrong  */
+  ACCSPMM_ERR_FILE = 12           /* file unwritable; METCF file unreadable,
+                                     corrupt or of another version           */
 } accspmm_status;
 
 const char* accspmm_status_string(accspmm_status s);
@@ -99,6 +100,19 @@ accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
 accspmm_status accspmm_metcf_build_host(const accspmm_csr_view* host_csr,
                                         const accspmm_build_opts* opts,
                                         accspmm_metcf** out);
+/* Makes a DEVICE CSR canonical in place: each row sorted by column (stable),
+ * duplicate columns summed in their input order in float32, rows compacted,
+ * row_ptr rewritten; *nnz_out receives the new count (col_idx/val keep their
+ * length, entries past *nnz_out are unspecified).  The result equals
+ * accspmm_read_mtx's host canonicalisation bit for bit.  ERR_INVALID_ARG if
+ * a column is >= cols or row_ptr is not non-decreasing from 0 to nnz.
+ * Blocks until done.  No reference counterpart: the reference hands its
+ * builder rows as they come (include/format.hpp:110-142, 238-269), which is
+ * right only for sorted, duplicate-free rows; this makes device-resident
+ * graphs valid input for accspmm_metcf_build. */
+accspmm_status accspmm_csr_canonicalize(uint32_t rows, uint32_t cols, uint32_t nnz,
+                                        uint32_t* row_ptr, uint32_t* col_idx, float* val,
+                                        uint32_t* nnz_out, void* stream);
 accspmm_status accspmm_metcf_destroy(accspmm_metcf* h);
 accspmm_status accspmm_metcf_get_info(const accspmm_metcf* h,
                                       accspmm_metcf_info* info);
@@ -259,6 +273,15 @@ void accspmm_host_csr_free(accspmm_host_csr* c);
  * tokenizer, so the result never depends on the thread count. */
 accspmm_status accspmm_read_mtx(const char* path, int canonicalize,
                                 accspmm_host_csr* out, int* mtx_code);
+
+/* Writes a Matrix Market file (coordinate real, values with 9 significant
+ * digits, which read back to the same floats).  symmetric = 0: every entry,
+ * row-major.  symmetric = 1: a square CSR holding a symmetric matrix is
+ * written as its lower triangle in column-major order, the usual layout of
+ * symmetric files (ERR_SHAPE if rows != cols).  No reference counterpart
+ * (the reference only reads, include/mmio.hpp:171-210). */
+accspmm_status accspmm_write_mtx(const accspmm_host_csr* csr, const char* path,
+                                 int symmetric);
 
 /* Threads that parsed the entries in this thread's last accspmm_read_mtx
  * call (1: the sequential tokenizer).  No reference counterpart: an ingest

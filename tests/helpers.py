@@ -53,3 +53,51 @@ def random_csr(rng: np.random.Generator, rows: int, cols: int, density: float,
         rp.append(len(ci))
     return (np.array(rp, np.uint32), np.array(ci, np.uint32),
             np.array(va, np.float32))
+
+
+def canonical_np(rows: int, cols: int, row_ptr, col_idx, val):
+    """Canonical form of a CSR, restated in numpy: entries ordered by (row,
+    column) keeping input order among equal columns (a stable sort), each run
+    of equal columns summed left to right in float32."""
+    rp = np.asarray(row_ptr, np.int64)
+    r = np.repeat(np.arange(rows, dtype=np.int64), np.diff(rp))
+    c = np.asarray(col_idx, np.int64)
+    v = np.asarray(val, np.float32)
+    order = np.lexsort((c, r))          # stable: ties keep input order
+    key = r[order] * cols + c[order]
+    v = v[order]
+    head = np.r_[True, key[1:] != key[:-1]] if key.size else np.zeros(0, bool)
+    starts = np.flatnonzero(head)
+    group = np.cumsum(head) - 1
+    rank = np.arange(key.size) - starts[group] if key.size else np.zeros(0, np.int64)
+    sums = v[starts].copy()
+    for k in range(1, int(rank.max()) + 1 if rank.size else 1):
+        sel = rank == k                 # the k-th member of every run, in order
+        sums[group[sel]] = (sums[group[sel]] + v[sel]).astype(np.float32)
+    ukey = key[head]
+    out_rp = np.zeros(rows + 1, np.int64)
+    np.add.at(out_rp, ukey // cols + 1, 1)
+    return (np.cumsum(out_rp).astype(np.uint32), (ukey % cols).astype(np.uint32),
+            sums.astype(np.float32))
+
+
+def scrambled_csr(rng: np.random.Generator, rows: int, cols: int, per_row_max: int,
+                  dup_frac: float = 0.2, heavy=(), empty_frac: float = 0.1):
+    """CSR with rows in random column order and repeated columns (not
+    canonical); `heavy` lists (row, length) pairs of long rows."""
+    lens = rng.integers(0, per_row_max + 1, rows)
+    lens[rng.random(rows) < empty_frac] = 0
+    for r_, n_ in heavy:
+        lens[r_] = n_
+    rp = np.r_[0, np.cumsum(lens)].astype(np.uint32)
+    nnz = int(rp[-1])
+    ci = rng.integers(0, cols, nnz).astype(np.uint32)
+    ndup = int(dup_frac * nnz)
+    if ndup and nnz > 1:
+        # copy a column from elsewhere in the same row
+        r_of = np.repeat(np.arange(rows), lens)
+        idx = rng.integers(0, nnz, ndup)
+        src = rp[r_of[idx]] + (rng.random(ndup) * lens[r_of[idx]]).astype(np.int64)
+        ci[idx] = ci[src]
+    va = rng.uniform(-1, 1, nnz).astype(np.float32)
+    return rp, ci, va

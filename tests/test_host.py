@@ -125,3 +125,69 @@ def test_mma_driver_exit_codes(tmp_path):
     bad = _write(tmp_path, "bad.mtx", K["mtx"]["bad_banners"][0])
     r = subprocess.run([mma, str(bad), "32"], capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode == 3 and "banner" in r.stdout
+
+
+def test_canonical_restatement_brute_force():
+    # the numpy restatement the canonicalisation tests check against, pinned
+    # by a per-row dictionary walk on small scrambled matrices
+    from helpers import canonical_np, scrambled_csr
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        rows, cols = int(rng.integers(1, 9)), int(rng.integers(1, 9))
+        rp, ci, va = scrambled_csr(rng, rows, cols, 6, 0.3)
+        orp, oci, ova = canonical_np(rows, cols, rp, ci, va)
+        exp_rp, exp_c, exp_v = [0], [], []
+        for r in range(rows):
+            runs = {}
+            for c, v in zip(ci[rp[r]:rp[r + 1]].tolist(), va[rp[r]:rp[r + 1]].tolist()):
+                runs.setdefault(c, []).append(np.float32(v))
+            for c in sorted(runs):
+                s = runs[c][0]
+                for v in runs[c][1:]:
+                    s = np.float32(s + v)
+                exp_c.append(c)
+                exp_v.append(s)
+            exp_rp.append(len(exp_c))
+        assert orp.tolist() == exp_rp and oci.tolist() == exp_c
+        assert np.array_equal(ova.view(np.uint32), np.array(exp_v, np.float32).view(np.uint32))
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+@pytest.mark.parametrize("n", [50, 60000])
+def test_write_mtx_round_trip(tmp_path, symmetric, n):
+    # written files read back to the same CSR bit for bit (9-digit values),
+    # and match the oracle's fscanf reader
+    csr = acc.gen_chung_lu(n, 12.0, 2.1, 3, 0)
+    p = tmp_path / "w.mtx"
+    csr.write_mtx(p, symmetric=symmetric)
+    head = p.read_text()[:80].splitlines()
+    assert head[0] == "%%MatrixMarket matrix coordinate real " + \
+        ("symmetric" if symmetric else "general")
+    back = acc.read_mtx(p, canonicalize=True)
+    assert np.array_equal(back.row_ptr, csr.row_ptr)
+    assert np.array_equal(back.col_idx, csr.col_idx)
+    assert np.array_equal(back.data.view(np.uint32), csr.data.view(np.uint32))
+    if n <= 50:
+        code, m, nn, rr, cc, vv = O.read_mtx(str(p))
+        rp, ci, va = O.coo_to_csr(m, rr, cc, vv)
+        assert code == 0 and np.array_equal(rp, csr.row_ptr) and np.array_equal(ci, csr.col_idx)
+        assert np.array_equal(va.view(np.uint32), csr.data.view(np.uint32))
+
+
+def test_write_mtx_rejects_non_square_symmetric(tmp_path):
+    rng = np.random.default_rng(1)
+    rp, ci, va = random_csr_small(rng)
+    csr = acc.CSR(4, 6, rp, ci, va)
+    with pytest.raises(acc.AccSpMMError) as e:
+        csr.write_mtx(tmp_path / "x.mtx", symmetric=True)
+    assert e.value.status == acc.Status.ERR_SHAPE
+    csr.write_mtx(tmp_path / "x.mtx", symmetric=False)
+    back = acc.read_mtx(tmp_path / "x.mtx")
+    assert np.array_equal(back.col_idx, ci) and np.array_equal(back.data, va)
+
+
+def random_csr_small(rng):
+    rp = np.array([0, 2, 2, 5, 6], np.uint32)
+    ci = np.array([1, 4, 0, 3, 5, 2], np.uint32)
+    va = rng.uniform(-1, 1, 6).astype(np.float32)
+    return rp, ci, va

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import accspmm_amd as acc
+from helpers import canonical_np
 from oracle import oracle as O
 
 PAR = "4"   # threads for the parallel path
@@ -141,26 +142,14 @@ def test_parallel_canonicalize_sorts_and_sums(tmp_path, monkeypatch):
     seq = _read(p, monkeypatch, "1", canonicalize=True)
     assert par.is_canonical()
     _same(par, seq.row_ptr, seq.col_idx, seq.data)
-    # independent restatement: stable order by (row, col), duplicates summed
-    # left to right in float32
+    # independent restatement (numpy): stable (row, col) order, duplicates
+    # summed left to right in float32
     vals = np.array([np.float32(float(f"{x:.9g}")) for x in v])
-    order = np.lexsort((c, r))
-    keys = (r[order].astype(np.int64) - 1) * n + (c[order] - 1)
-    first = np.r_[True, keys[1:] != keys[:-1]]
-    starts = np.flatnonzero(first)
-    sums = vals[order][starts].copy()
-    for gi, s in enumerate(starts):
-        e = starts[gi + 1] if gi + 1 < len(starts) else len(keys)
-        acc_ = np.float32(vals[order][s])
-        for j in range(s + 1, e):
-            acc_ = np.float32(acc_ + vals[order][j])
-        sums[gi] = acc_
-    ukeys = keys[first]
-    rp = np.zeros(n + 1, np.int64)
-    np.add.at(rp, ukeys // n + 1, 1)
-    assert np.array_equal(par.row_ptr, np.cumsum(rp).astype(np.uint32))
-    assert np.array_equal(par.col_idx, (ukeys % n).astype(np.uint32))
-    assert np.array_equal(par.data.view(np.uint32), sums.view(np.uint32))
+    rr = np.sort(r - 1, kind="stable")
+    order = np.argsort(r, kind="stable")
+    raw_rp = np.r_[0, np.cumsum(np.bincount(rr, minlength=n))]
+    crp, cci, cva = canonical_np(n, n, raw_rp, c[order] - 1, vals[order])
+    _same(par, crp, cci, cva)
 
 
 def _big_general(n_entries, extra_lines=(), split_entry=None, nz=None):
