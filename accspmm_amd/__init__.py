@@ -72,7 +72,8 @@ class BuildOpts(C.Structure):
     _fields_ = [("row_window", C.c_uint32), ("col_window", C.c_uint32),
                 ("t_load", C.c_uint32), ("t_comp", C.c_uint32),
                 ("t_write", C.c_uint32), ("target_num", C.c_uint32),
-                ("imbalance_threshold", C.c_uint32), ("build_adpbme", C.c_uint32)]
+                ("imbalance_threshold", C.c_uint32), ("build_adpbme", C.c_uint32),
+                ("reorder_rows", C.c_uint32)]
 
 
 class MetcfInfo(C.Structure):
@@ -80,7 +81,8 @@ class MetcfInfo(C.Structure):
                 ("row_window", C.c_uint32), ("num_windows", C.c_uint32),
                 ("num_tiles", C.c_uint32), ("num_groups", C.c_uint32),
                 ("ibd", C.c_uint32), ("balanced", C.c_uint32),
-                ("max_window_tiles", C.c_uint32), ("device_bytes", C.c_uint64)]
+                ("max_window_tiles", C.c_uint32), ("device_bytes", C.c_uint64),
+                ("reordered", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class Tile(C.Structure):
@@ -135,6 +137,7 @@ _SIGS = {
     "accspmm_metcf_destroy": (C.c_int, [C.c_void_p]),
     "accspmm_metcf_get_info": (C.c_int, [C.c_void_p, C.POINTER(MetcfInfo)]),
     "accspmm_metcf_export": (C.c_int, [C.c_void_p] + [C.c_void_p] * 7),
+    "accspmm_metcf_row_order": (C.c_int, [C.c_void_p, C.c_void_p]),
     "accspmm_metcf_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "accspmm_metcf_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_void_p]),
     "accspmm_spmm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
@@ -384,7 +387,8 @@ class DeviceGraph:
         g.csr = None
         g._dev = None
         inf = g.info
-        g.opts = default_build_opts(inf.row_window, build_adpbme=1 if inf.num_groups else 0)
+        g.opts = default_build_opts(inf.row_window, build_adpbme=1 if inf.num_groups else 0,
+                                    reorder_rows=inf.reordered)
         torch.cuda.synchronize()
         return g
 
@@ -424,6 +428,12 @@ class DeviceGraph:
             self.handle, _ptr(out["rowWindowOffset"]), _ptr(out["tcOffset"]),
             _ptr(out["tcLocalBit"]), _ptr(out["sparseA2B"]), _ptr(out["data"]),
             _ptr(grp), _ptr(rix)), "metcf_export")
+        return out
+
+    def row_order(self) -> np.ndarray:
+        """Input row of each format row (identity unless reorder_rows)."""
+        out = np.empty(self.info.rows, np.uint32)
+        _check(lib().accspmm_metcf_row_order(self.handle, _ptr(out)), "metcf_row_order")
         return out
 
     def tile_select(self, K: int, form: int = FORM_FP32) -> Tile:

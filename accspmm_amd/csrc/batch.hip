@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void k_batch_reduce(
   const uint32_t p = last_le(sprefix, nprob, blockIdx.x);
   const SpmmArgs& g = descs[p];
   reduce_split_window<RW>(g.split_info, blockIdx.x - sprefix[p], g.partials,
-                          g.kc_padded, g.K, g.C, g.ldc, g.rows);
+                          g.kc_padded, g.K, g.C, g.ldc, g.rows, g.row_map);
 }
 
 inline uint64_t align256(uint64_t x) { return (x + 255) / 256 * 256; }
@@ -237,6 +237,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     g.units = q.sched->units.as<WorkUnit>();
     g.split_info = q.sched->split_info.as<uint32_t>();
     g.C = p.C;
+    g.row_map = h->row_perm.p ? h->row_perm.as<uint32_t>() : nullptr;
     g.ldc = p.ldc;
     g.rows = h->rows;
     g.K = p.K;

@@ -326,6 +326,7 @@ extern "C" void accspmm_build_opts_default(accspmm_build_opts* o) {
   o->target_num = 16;
   o->imbalance_threshold = 8;
   o->build_adpbme = 1;
+  o->reorder_rows = 0;
 }
 
 namespace {
@@ -572,12 +573,25 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
   if (o.row_window != 8 && o.row_window != 16) return ACCSPMM_ERR_UNSUPPORTED;
   if (o.col_window != 8) return ACCSPMM_ERR_UNSUPPORTED;
   if (o.build_adpbme && o.t_load + o.t_comp == 0) return ACCSPMM_ERR_UNSUPPORTED;
+  if (o.reorder_rows > 1) return ACCSPMM_ERR_UNSUPPORTED;
   if (csr->rows == 0 || csr->cols == 0) return ACCSPMM_ERR_INVALID_ARG;
   if (!csr->row_ptr || (csr->nnz && (!csr->col_idx || !csr->val)))
     return ACCSPMM_ERR_INVALID_ARG;
   auto* h = new (std::nothrow) accspmm_metcf();
   if (!h) return ACCSPMM_ERR_OUT_OF_MEMORY;
-  accspmm_status st = build_impl(csr, o, h, (hipStream_t)stream);
+  // reorder_rows: build from the rows in order of decreasing length; the
+  // handle keeps the permutation and the kernels map C rows through it
+  DevBuf prp, pci, pva;
+  accspmm_csr_view pv;
+  const accspmm_csr_view* src = csr;
+  accspmm_status st = ACCSPMM_OK;
+  if (o.reorder_rows) {
+    st = reorder_rows_by_length(csr, prp, pci, pva, h->row_perm, (hipStream_t)stream);
+    pv = accspmm_csr_view{csr->rows, csr->cols, csr->nnz, prp.as<uint32_t>(),
+                          pci.as<uint32_t>(), pva.as<float>()};
+    src = &pv;
+  }
+  if (st == ACCSPMM_OK) st = build_impl(src, o, h, (hipStream_t)stream);
   if (st != ACCSPMM_OK) {
     (void)hipStreamSynchronize((hipStream_t)stream);
     delete h;
@@ -627,6 +641,20 @@ extern "C" accspmm_status accspmm_metcf_get_info(const accspmm_metcf* h,
   info->balanced = h->balanced;
   info->max_window_tiles = h->max_window_tiles;
   info->device_bytes = h->device_bytes();
+  info->reordered = h->row_perm.p ? 1u : 0u;
+  info->reserved = 0;
+  return ACCSPMM_OK;
+}
+
+extern "C" accspmm_status accspmm_metcf_row_order(const accspmm_metcf* h,
+                                                  uint32_t* row_order) {
+  if (!h || !row_order) return ACCSPMM_ERR_INVALID_ARG;
+  if (!h->row_perm.p) {
+    for (uint32_t i = 0; i < h->rows; ++i) row_order[i] = i;
+    return ACCSPMM_OK;
+  }
+  ACC_HIP(hipSetDevice(h->device));
+  ACC_HIP(hipMemcpy(row_order, h->row_perm.p, (size_t)h->rows * 4, hipMemcpyDeviceToHost));
   return ACCSPMM_OK;
 }
 

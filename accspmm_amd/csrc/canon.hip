@@ -15,6 +15,9 @@
 // orders each row by column and keeps equal columns in their input order —
 // then mark the first entry of each (row, column) run, scan the marks into
 // output positions, and let the run heads add their runs in order.
+//
+// Also here: the row reordering the builder applies on request (rows by
+// decreasing length, a stable radix sort of the complemented lengths).
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -173,7 +176,92 @@ accspmm_status canonicalize(uint32_t rows, uint32_t cols, uint32_t nnz, uint32_t
   return sort_and_compact<Key>(rows, nnz, col_bits, key_bits, rp, col, val, keys, nnz_out, s);
 }
 
+// keys[r] = ~length(r) (ascending order = decreasing length), ids[r] = r
+__global__ __launch_bounds__(kThreads) void k_length_keys(const uint32_t* rp, uint32_t rows,
+                                                          uint32_t* keys, uint32_t* ids) {
+  const uint64_t r = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (r < rows) {
+    keys[r] = ~(rp[r + 1] - rp[r]);
+    ids[r] = (uint32_t)r;
+  }
+}
+
+// len[i] = length of input row perm[i]; len[rows] = 0 (scanned into row_ptr)
+__global__ __launch_bounds__(kThreads) void k_perm_lengths(const uint32_t* rp,
+                                                           const uint32_t* perm, uint32_t rows,
+                                                           uint32_t* len) {
+  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < rows) len[i] = rp[perm[i] + 1] - rp[perm[i]];
+  else if (i == rows) len[i] = 0u;
+}
+
+// One wave per output row copies the input row's columns and values.
+__global__ __launch_bounds__(kThreads) void k_copy_rows(const uint32_t* rp, const uint32_t* ci,
+                                                        const float* va, const uint32_t* perm,
+                                                        const uint32_t* nrp, uint32_t rows,
+                                                        uint32_t* nci, float* nva) {
+  const uint64_t i = (uint64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64;
+  if (i >= rows) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t src = rp[perm[i]], dst = nrp[i], len = nrp[i + 1] - dst;
+  for (uint32_t j = lane; j < len; j += 64u) {
+    nci[dst + j] = ci[src + j];
+    nva[dst + j] = va[src + j];
+  }
+}
+
 }  // namespace
+
+namespace accspmm {
+
+accspmm_status reorder_rows_by_length(const accspmm_csr_view* in, DevBuf& rp, DevBuf& ci,
+                                      DevBuf& va, DevBuf& perm, hipStream_t s) {
+  const uint32_t rows = in->rows, nnz = in->nnz;
+  // the copies below index by row_ptr: it must run from 0 to nnz, in order
+  DevBuf flags, keys, keys_out, ids, len, temp;
+  ACC_TRY(flags.alloc(sizeof(uint32_t)));
+  ACC_HIP(hipMemsetAsync(flags.p, 0, sizeof(uint32_t), s));
+  k_check_rows<<<blocks_for((uint64_t)rows + 1), kThreads, 0, s>>>(in->row_ptr, rows, nnz,
+                                                                    flags.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  uint32_t h_flags = 0;
+  ACC_HIP(hipMemcpyAsync(&h_flags, flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  ACC_HIP(hipStreamSynchronize(s));
+  if (h_flags) return ACCSPMM_ERR_INVALID_ARG;
+
+  ACC_TRY(keys.alloc(sizeof(uint32_t) * (size_t)rows));
+  ACC_TRY(keys_out.alloc(sizeof(uint32_t) * (size_t)rows));
+  ACC_TRY(ids.alloc(sizeof(uint32_t) * (size_t)rows));
+  ACC_TRY(perm.alloc(sizeof(uint32_t) * (size_t)rows));
+  k_length_keys<<<blocks_for(rows), kThreads, 0, s>>>(in->row_ptr, rows, keys.as<uint32_t>(),
+                                                      ids.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  size_t temp_bytes = 0;
+  ACC_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys.as<uint32_t>(),
+                                    keys_out.as<uint32_t>(), ids.as<uint32_t>(),
+                                    perm.as<uint32_t>(), rows, 0u, 32u, s));
+  ACC_TRY(temp.alloc(temp_bytes));
+  ACC_HIP(rocprim::radix_sort_pairs(temp.p, temp_bytes, keys.as<uint32_t>(),
+                                    keys_out.as<uint32_t>(), ids.as<uint32_t>(),
+                                    perm.as<uint32_t>(), rows, 0u, 32u, s));
+  ACC_TRY(len.alloc(sizeof(uint32_t) * ((size_t)rows + 1)));
+  ACC_TRY(rp.alloc(sizeof(uint32_t) * ((size_t)rows + 1)));
+  k_perm_lengths<<<blocks_for((uint64_t)rows + 1), kThreads, 0, s>>>(
+      in->row_ptr, perm.as<uint32_t>(), rows, len.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  ACC_TRY(exclusive_scan_u32(len.as<uint32_t>(), rp.as<uint32_t>(), (uint64_t)rows + 1, s));
+  ACC_TRY(ci.alloc(sizeof(uint32_t) * (size_t)std::max(nnz, 1u)));
+  ACC_TRY(va.alloc(sizeof(float) * (size_t)std::max(nnz, 1u)));
+  const uint64_t waves_per_block = kThreads / 64;
+  k_copy_rows<<<(unsigned)std::max<uint64_t>(1, (rows + waves_per_block - 1) / waves_per_block),
+                kThreads, 0, s>>>(in->row_ptr, in->col_idx, in->val, perm.as<uint32_t>(),
+                                  rp.as<uint32_t>(), rows, ci.as<uint32_t>(), va.as<float>());
+  ACC_HIP(hipGetLastError());
+  ACC_HIP(hipStreamSynchronize(s));
+  return ACCSPMM_OK;
+}
+
+}  // namespace accspmm
 
 extern "C" accspmm_status accspmm_csr_canonicalize(uint32_t rows, uint32_t cols, uint32_t nnz,
                                                    uint32_t* row_ptr, uint32_t* col_idx,

@@ -120,6 +120,9 @@ struct accspmm_metcf {
   uint32_t ibd = 0, balanced = 0, max_window_tiles = 0;
   accspmm_build_opts opts{};
   accspmm::DevBuf rwo, tco, bits, a2b, data, grp, rix;
+  // rows reordered at build (opts.reorder_rows): format row i is input row
+  // row_perm[i]; empty otherwise
+  accspmm::DevBuf row_perm;
   std::vector<uint32_t> h_rwo;  // host copy of rowWindowOffset
   std::string hist_key;         // degree-histogram part of the tuning key
   // lazily built state (guarded by mu; handles are otherwise immutable)
@@ -139,11 +142,16 @@ struct accspmm_metcf {
   mutable accspmm::DevBuf vsplit;
   uint64_t device_bytes() const {
     return rwo.bytes + tco.bytes + bits.bytes + a2b.bytes + data.bytes +
-           grp.bytes + rix.bytes + vsplit.bytes;
+           grp.bytes + rix.bytes + vsplit.bytes + row_perm.bytes;
   }
 };
 
 namespace accspmm {
+// canon.hip: validates the CSR's row_ptr and writes the CSR with its rows
+// in order of decreasing length (stable) into rp/ci/va; perm[i] = input row
+// of row i
+accspmm_status reorder_rows_by_length(const accspmm_csr_view* in, DevBuf& rp, DevBuf& ci,
+                                      DevBuf& va, DevBuf& perm, hipStream_t s);
 // scan.hip
 accspmm_status exclusive_scan_u32(const uint32_t* in, uint32_t* out,
                                   uint64_t n, hipStream_t s);

@@ -5,13 +5,15 @@
 // lets a server warm-start its resident set without rebuilding.
 //
 // File layout (little-endian):
-//   "ACCSPMTC" (8 bytes), format version (u32, = 1)
+//   "ACCSPMTC" (8 bytes), format version (u32, = 2; version 1 is read too)
 //   header (u32 each): rows, cols, nnz, row_window, W, T, G, ibd, balanced,
-//     max_window_tiles, has_adpbme, the 8 build options, tuning-key length
+//     max_window_tiles, has_adpbme, the 8 build options, tuning-key length,
+//     and from version 2 the reorder_rows option
 //   tuning key (bytes)
 //   rowWindowOffset[W+1] u32, tcOffset[T+1] u32, tcLocalBit[T*RW/8] u64,
 //   sparseA2B[8T] u32, data[nnz] f32, and with AdpBME groupOffset[G+1] u32,
-//   rowIndices[T] u32 — the arrays accspmm_metcf_export returns
+//   rowIndices[T] u32 — the arrays accspmm_metcf_export returns — and with
+//   reorder_rows the row order[rows] u32 (accspmm_metcf_row_order)
 //   FNV-1a 64 of every byte before it (u64)
 // Loading checks the header, every size, the checksum and the offsets'
 // invariants (monotone, popcounts match tcOffset), then rebuilds the
@@ -30,8 +32,9 @@ using namespace accspmm;
 namespace {
 
 constexpr char kMagic[8] = {'A', 'C', 'C', 'S', 'P', 'M', 'T', 'C'};
-constexpr uint32_t kVersion = 1;
-constexpr uint32_t kHeaderWords = 19;
+constexpr uint32_t kVersion = 2;
+constexpr uint32_t kHeaderWordsV1 = 19;
+constexpr uint32_t kHeaderWords = 20;
 
 struct Fnv {
   uint64_t h = 1469598103934665603ull;
@@ -95,9 +98,10 @@ extern "C" accspmm_status accspmm_metcf_save(const accspmm_metcf* h, const char*
   if (!h || !path) return ACCSPMM_ERR_INVALID_ARG;
   const uint32_t T = h->T, W = h->W, words = h->words;
   const bool adp = h->grp.p != nullptr;
-  std::vector<uint32_t> rwo, tco, a2b, grp, rix;
+  std::vector<uint32_t> rwo, tco, a2b, grp, rix, order;
   std::vector<unsigned long long> bits;
   std::vector<float> data;
+  const bool reordered = h->row_perm.p != nullptr;
   ACC_HIP(hipSetDevice(h->device));
   ACC_TRY(to_host(rwo, h->rwo, (size_t)W + 1));
   ACC_TRY(to_host(tco, h->tco, (size_t)T + 1));
@@ -108,6 +112,7 @@ extern "C" accspmm_status accspmm_metcf_save(const accspmm_metcf* h, const char*
     ACC_TRY(to_host(grp, h->grp, (size_t)h->G + 1));
     ACC_TRY(to_host(rix, h->rix, (size_t)T));
   }
+  if (reordered) ACC_TRY(to_host(order, h->row_perm, (size_t)h->rows));
   std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "wb"), &std::fclose);
   if (!f) return ACCSPMM_ERR_FILE;
   Writer w;
@@ -116,7 +121,7 @@ extern "C" accspmm_status accspmm_metcf_save(const accspmm_metcf* h, const char*
       h->rows, h->cols, h->nnz, h->rw, W, T, h->G, h->ibd, h->balanced,
       h->max_window_tiles, adp ? 1u : 0u, h->opts.row_window, h->opts.col_window,
       h->opts.t_load, h->opts.t_comp, h->opts.t_write, h->opts.target_num,
-      h->opts.imbalance_threshold, (uint32_t)h->hist_key.size()};
+      h->opts.imbalance_threshold, (uint32_t)h->hist_key.size(), reordered ? 1u : 0u};
   w.put(kMagic, sizeof(kMagic));
   w.put(&kVersion, 4);
   w.put(hdr, sizeof(hdr));
@@ -130,6 +135,7 @@ extern "C" accspmm_status accspmm_metcf_save(const accspmm_metcf* h, const char*
     w.put(grp.data(), grp.size() * 4);
     w.put(rix.data(), rix.size() * 4);
   }
+  if (reordered) w.put(order.data(), order.size() * 4);
   const uint64_t sum = w.sum.h;
   if (w.ok) w.ok = std::fwrite(&sum, 1, 8, f.get()) == 8;
   return w.ok ? ACCSPMM_OK : ACCSPMM_ERR_FILE;
@@ -145,16 +151,18 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
   Reader r;
   r.f = f.get();
   char magic[8];
-  uint32_t version = 0, hdr[kHeaderWords];
+  uint32_t version = 0, hdr[kHeaderWords] = {};
   r.get(magic, 8);
   r.get(&version, 4);
-  r.get(hdr, sizeof(hdr));
-  if (!r.ok || std::memcmp(magic, kMagic, 8) != 0 || version != kVersion)
+  if (!r.ok || std::memcmp(magic, kMagic, 8) != 0 || (version != 1 && version != kVersion))
     return ACCSPMM_ERR_FILE;
+  r.get(hdr, 4 * (version == 1 ? kHeaderWordsV1 : kHeaderWords));
   const uint32_t rows = hdr[0], cols = hdr[1], nnz = hdr[2], rw = hdr[3], W = hdr[4],
-                 T = hdr[5], G = hdr[6], adp = hdr[10], key_len = hdr[18];
-  if ((rw != 8 && rw != 16) || rows == 0 || cols == 0 || W != (rows + rw - 1) / rw ||
-      adp > 1 || key_len > 4096 || (uint64_t)T * kColWindow > 0xFFFFFFFFull)
+                 T = hdr[5], G = hdr[6], adp = hdr[10], key_len = hdr[18],
+                 reorder = hdr[19];
+  if (!r.ok || (rw != 8 && rw != 16) || rows == 0 || cols == 0 ||
+      W != (rows + rw - 1) / rw || adp > 1 || reorder > 1 || key_len > 4096 ||
+      (uint64_t)T * kColWindow > 0xFFFFFFFFull)
     return ACCSPMM_ERR_FILE;
   const uint32_t words = rw / 8;
   std::string key(key_len, '\0');
@@ -162,7 +170,7 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
   std::vector<uint32_t> rwo((size_t)W + 1), tco((size_t)T + 1), a2b((size_t)T * kColWindow);
   std::vector<unsigned long long> bits((size_t)T * words);
   std::vector<float> data(nnz);
-  std::vector<uint32_t> grp, rix;
+  std::vector<uint32_t> grp, rix, order;
   r.get(rwo.data(), rwo.size() * 4);
   r.get(tco.data(), tco.size() * 4);
   r.get(bits.data(), bits.size() * 8);
@@ -174,6 +182,10 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
     r.get(grp.data(), grp.size() * 4);
     r.get(rix.data(), rix.size() * 4);
   }
+  if (reorder) {
+    order.resize(rows);
+    r.get(order.data(), order.size() * 4);
+  }
   uint64_t sum = 0;
   const uint64_t expect = r.sum.h;
   const bool tail_ok = r.ok && std::fread(&sum, 1, 8, f.get()) == 8;
@@ -183,6 +195,13 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
   // invariants of the format
   if (!monotone(rwo, T) || !monotone(tco, nnz)) return ACCSPMM_ERR_FILE;
   if (adp && !monotone(grp, T)) return ACCSPMM_ERR_FILE;
+  if (reorder) {  // a permutation of the rows
+    std::vector<uint8_t> seen(rows, 0);
+    for (uint32_t x : order) {
+      if (x >= rows || seen[x]) return ACCSPMM_ERR_FILE;
+      seen[x] = 1;
+    }
+  }
   for (uint32_t t = 0; t < T; ++t) {
     uint32_t pc = 0;
     for (uint32_t k = 0; k < words; ++k) pc += (uint32_t)__builtin_popcountll(bits[(size_t)t * words + k]);
@@ -206,7 +225,7 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
   h->balanced = hdr[8];
   h->max_window_tiles = hdr[9];
   h->opts = accspmm_build_opts{hdr[11], hdr[12], hdr[13], hdr[14], hdr[15], hdr[16], hdr[17],
-                               adp};
+                               adp, reorder};
   h->hist_key = key;
   h->h_rwo = rwo;
   // device arrays with the kernels' padding: tcOffset[T+1] repeats nnz, the
@@ -224,6 +243,7 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
     ACC_TRY(to_device(h->grp, grp, grp.size()));
     ACC_TRY(to_device(h->rix, rix, rix.size()));
   }
+  if (reorder) ACC_TRY(to_device(h->row_perm, order, order.size()));
   ACC_HIP(hipDeviceSynchronize());
   *out = h.release();
   return ACCSPMM_OK;

@@ -167,8 +167,9 @@ __global__ void k_panel_bounds(const uint32_t* __restrict__ rwo,
 template <int RW>
 __global__ __launch_bounds__(256) void k_reduce_split(
     const uint32_t* __restrict__ sinfo, const float* __restrict__ partials,
-    uint32_t kcp, uint32_t K, float* __restrict__ C, uint32_t ldc, uint32_t rows) {
-  reduce_split_window<RW>(sinfo, blockIdx.x, partials, kcp, K, C, ldc, rows);
+    uint32_t kcp, uint32_t K, float* __restrict__ C, uint32_t ldc, uint32_t rows,
+    const uint32_t* __restrict__ rmap) {
+  reduce_split_window<RW>(sinfo, blockIdx.x, partials, kcp, K, C, ldc, rows, rmap);
 }
 
 // Shortest piece a column panel cut may leave (shorter ones are merged
@@ -461,6 +462,7 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   g.units = sch->units.as<WorkUnit>();
   g.split_info = sch->split_info.as<uint32_t>();
   g.C = C;
+  g.row_map = h->row_perm.p ? h->row_perm.as<uint32_t>() : nullptr;
   g.ldc = ldc;
   g.rows = h->rows;
   g.K = K;
@@ -521,11 +523,11 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
     if (h->rw == 8)
       hipLaunchKernelGGL(k_reduce_split<8>, dim3(sch->num_split_windows), dim3(256), 0, s,
                          sch->split_info.as<uint32_t>(), g.partials, g.kc_padded, K,
-                         C, ldc, h->rows);
+                         C, ldc, h->rows, g.row_map);
     else
       hipLaunchKernelGGL(k_reduce_split<16>, dim3(sch->num_split_windows), dim3(256), 0, s,
                          sch->split_info.as<uint32_t>(), g.partials, g.kc_padded, K,
-                         C, ldc, h->rows);
+                         C, ldc, h->rows, g.row_map);
     ACC_HIP(hipGetLastError());
   }
   return ACCSPMM_OK;

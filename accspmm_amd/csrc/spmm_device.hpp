@@ -30,6 +30,9 @@ struct SpmmArgs {
   float* __restrict__ partials;
   const void* __restrict__ B;  // FP32: float; SPLIT: u32 (hi|lo<<16); BF16: u16
   float* __restrict__ C;
+  // C row of format row r: row_map[r] when the graph was built with its
+  // rows reordered, r when row_map is null
+  const uint32_t* __restrict__ row_map;
   // what padded columns read: BF16 forms: row zero_row of the converted B
   // (its last row, all zero); FP32 form: row 0 of the caller's B, masked
   // after the load (measured faster on gfx950 than reading a separate zero
@@ -363,13 +366,18 @@ __device__ inline void store_vec(float* p, const float* v, uint32_t nvalid) {
   }
 }
 
+__device__ inline uint64_t out_row(const uint32_t* __restrict__ rmap, uint32_t r) {
+  return rmap ? rmap[r] : r;
+}
+
 // Writes the accumulated rows of window w to C (split form, RW=8: rows r
 // and r+8 of the MFMA tile hold the A_hi and A_lo halves and are added in
 // place; the caller resets the accumulators afterwards).
 template <int FORM, int NB, int RW>
 __device__ inline void store_window(f4* acc, uint32_t w, uint32_t q,
                                     uint32_t feat, uint32_t nvalid, float* C,
-                                    uint32_t ldc, uint32_t rows) {
+                                    uint32_t ldc, uint32_t rows,
+                                    const uint32_t* rmap) {
   if (FORM == kSPLIT && RW == 8) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
@@ -384,7 +392,7 @@ __device__ inline void store_window(f4* acc, uint32_t w, uint32_t q,
       float v[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) v[b] = acc[b][i];
-      store_vec<NB>(C + (uint64_t)r * ldc + feat, v, nvalid);
+      store_vec<NB>(C + out_row(rmap, r) * ldc + feat, v, nvalid);
     }
   }
 }
@@ -392,7 +400,7 @@ __device__ inline void store_window(f4* acc, uint32_t w, uint32_t q,
 template <int NB, int RW>
 __device__ inline void zero_window(uint32_t w, uint32_t q, uint32_t feat,
                                    uint32_t nvalid, float* C, uint32_t ldc,
-                                   uint32_t rows) {
+                                   uint32_t rows, const uint32_t* rmap) {
   if (RW == 8 && q >= 2) return;
   float v[NB];
 #pragma unroll
@@ -400,7 +408,7 @@ __device__ inline void zero_window(uint32_t w, uint32_t q, uint32_t feat,
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const uint32_t r = w * RW + 4 * q + i;
-    if (r < rows && nvalid) store_vec<NB>(C + (uint64_t)r * ldc + feat, v, nvalid);
+    if (r < rows && nvalid) store_vec<NB>(C + out_row(rmap, r) * ldc + feat, v, nvalid);
   }
 }
 
@@ -414,6 +422,7 @@ struct Accum {
   bool direct;  // packed unit: flush to C; split piece: keep for the partial
   uint32_t q, feat, nvalid, ldc, rows;
   float* C;
+  const uint32_t* rmap;
   __device__ inline void reset() {
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
@@ -422,7 +431,7 @@ struct Accum {
   // empty windows except as a whole unit, so no zero-fill is needed here)
   __device__ inline void move_to(uint32_t w) {
     if (w == cur_w) return;
-    store_window<FORM, NB, RW>(acc, cur_w, q, feat, nvalid, C, ldc, rows);
+    store_window<FORM, NB, RW>(acc, cur_w, q, feat, nvalid, C, ldc, rows, rmap);
     reset();
     cur_w = w;
   }
@@ -528,6 +537,7 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
   out.C = C;
   out.ldc = ldc;
   out.rows = rows;
+  out.rmap = g.row_map;
 
   const uint32_t ngroups = u.ngroups;
   if (ngroups > 0) {
@@ -548,9 +558,9 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
 
   if (u.slot < 0) {
     // last window of the unit; a unit of empty windows writes zero rows
-    store_window<FORM, NB, RW>(out.acc, out.cur_w, q, feat, nvalid, C, ldc, rows);
+    store_window<FORM, NB, RW>(out.acc, out.cur_w, q, feat, nvalid, C, ldc, rows, g.row_map);
     for (uint32_t e = out.cur_w + 1; e < u.window + u.nwin; ++e)
-      zero_window<NB, RW>(e, q, feat, nvalid, C, ldc, rows);
+      zero_window<NB, RW>(e, q, feat, nvalid, C, ldc, rows, g.row_map);
     return;
   }
   // split piece: write partial (RW rows x kc_padded floats per slot)
@@ -587,7 +597,8 @@ __device__ inline void reduce_split_window(const uint32_t* __restrict__ sinfo,
                                            const float* __restrict__ partials,
                                            uint32_t kcp, uint32_t K,
                                            float* __restrict__ C, uint32_t ldc,
-                                           uint32_t rows) {
+                                           uint32_t rows,
+                                           const uint32_t* __restrict__ rmap) {
   const uint32_t w = sinfo[3ull * sw], first = sinfo[3ull * sw + 1],
                  count = sinfo[3ull * sw + 2];
   for (uint32_t i = threadIdx.x; i < RW * K; i += blockDim.x) {
@@ -597,7 +608,7 @@ __device__ inline void reduce_split_window(const uint32_t* __restrict__ sinfo,
     const float* src = partials + ((uint64_t)first * RW + r) * kcp + k;
     float v = 0.f;
     for (uint32_t p = 0; p < count; ++p) v += src[(uint64_t)p * RW * kcp];
-    C[row * ldc + k] = v;
+    C[out_row(rmap, (uint32_t)row) * ldc + k] = v;
   }
 }
 

@@ -72,7 +72,8 @@ __device__ inline void finish_piece(f4* acc, f4* slot, uint32_t lane, bool first
 template <int NB, int RW>
 __device__ inline void store_slot(const f4* slot, uint32_t lane, uint32_t q,
                                   uint32_t w, uint32_t feat, uint32_t nvalid,
-                                  float* C, uint32_t ldc, uint32_t rows) {
+                                  float* C, uint32_t ldc, uint32_t rows,
+                                  const uint32_t* rmap) {
   constexpr uint32_t NL = slot_lanes<RW>();
   if (lane >= NL || !nvalid) return;
   f4 acc[NB];
@@ -85,7 +86,7 @@ __device__ inline void store_slot(const f4* slot, uint32_t lane, uint32_t q,
       float v[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) v[b] = acc[b][i];
-      store_vec<NB>(C + (uint64_t)r * ldc + feat, v, nvalid);
+      store_vec<NB>(C + out_row(rmap, r) * ldc + feat, v, nvalid);
     }
   }
 }
@@ -221,9 +222,9 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
         if (w == kPad) continue;
         if ((started >> s) & 1u)
           store_slot<NB, RW>(slots + (size_t)s * NB * NL, lane, q, w, feat, nvalid, g.C,
-                             g.ldc, g.rows);
+                             g.ldc, g.rows, g.row_map);
         else
-          zero_window<NB, RW>(w, q, feat, nvalid, g.C, g.ldc, g.rows);
+          zero_window<NB, RW>(w, q, feat, nvalid, g.C, g.ldc, g.rows, g.row_map);
       }
     }
   }

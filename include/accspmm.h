@@ -66,6 +66,13 @@ typedef struct accspmm_build_opts {
   uint32_t t_load, t_comp, t_write, target_num; /* AdpBME cost model        */
   uint32_t imbalance_threshold; /* balanceStrategy threshold                */
   uint32_t build_adpbme;        /* 1: also build AdpBME groups              */
+  /* 0 (default): rows kept in input order, the reference's format.  1: the
+   * format is built from the rows in order of decreasing length (stable),
+   * so 8-row windows of like rows share more columns: fewer tiles and fewer
+   * B rows gathered.  C is still written in input row order.  No reference
+   * counterpart (the reference reads an externally reordered file, named
+   * ddi.my_reorder.mtx in its tree). */
+  uint32_t reorder_rows;
 } accspmm_build_opts;
 
 void accspmm_build_opts_default(accspmm_build_opts* o);
@@ -84,6 +91,8 @@ typedef struct accspmm_metcf_info {
   uint32_t balanced;      /* ibd > imbalance_threshold                */
   uint32_t max_window_tiles;
   uint64_t device_bytes;  /* device memory held by the handle         */
+  uint32_t reordered;     /* built with reorder_rows = 1              */
+  uint32_t reserved;
 } accspmm_metcf_info;
 
 /* GPU METCF builder.  Replaces METCFBit::convertFromCSR
@@ -128,6 +137,11 @@ accspmm_status accspmm_metcf_export(const accspmm_metcf* h,
                                     uint32_t* sparseA2B, float* data,
                                     uint32_t* groupOffset,
                                     uint32_t* rowIndices);
+
+/* Row order of the format: row i of the exported arrays is input row
+ * row_order[i] (HOST buffer of `rows` entries; the identity unless built
+ * with reorder_rows = 1).  No reference counterpart. */
+accspmm_status accspmm_metcf_row_order(const accspmm_metcf* h, uint32_t* row_order);
 
 /* Binary save / load of a built graph (no reference counterpart: the
  * reference rebuilds the format from the .mtx file on every run,

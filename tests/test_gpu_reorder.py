@@ -1,0 +1,130 @@
+"""Graphs built with reorder_rows = 1 (rows by decreasing length): the
+format equals the format of the explicitly permuted CSR (oracle builder),
+C comes back in input row order equal, bit for bit, to the permuted graph's
+C scattered through the permutation, and within each form's tolerance of
+the FP64 oracle — on the single launch, sweep, batched and saved paths."""
+import numpy as np
+import pytest
+
+import accspmm_amd as acc
+from helpers import random_csr
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {acc.FORM_FP32: 1e-5, acc.FORM_SPLIT_BF16: 2.0 ** -15 + 1e-6, acc.FORM_BF16: 1e-2}
+FORMS = [acc.FORM_FP32, acc.FORM_SPLIT_BF16, acc.FORM_BF16]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _permuted(csr):
+    """Rows in order of decreasing length, ties in input order."""
+    lens = np.diff(csr.row_ptr.astype(np.int64))
+    order = np.argsort(-lens, kind="stable")
+    rp = np.r_[0, np.cumsum(lens[order])].astype(np.uint32)
+    idx = np.concatenate([np.arange(csr.row_ptr[r], csr.row_ptr[r + 1]) for r in order]
+                         + [np.zeros(0, np.int64)]).astype(np.int64)
+    return order.astype(np.uint32), acc.CSR(csr.rows, csr.cols, rp, csr.col_idx[idx],
+                                            csr.data[idx])
+
+
+def _graphs():
+    rng = np.random.default_rng(5)
+    rp, ci, va = random_csr(rng, 1203, 900, 0.02, empty_rows=0.15, heavy_rows=3)
+    return {"random": acc.CSR(1203, 900, rp, ci, va),
+            "chung_lu": acc.gen_chung_lu(9000, 20.0, 2.1, 8, 1)}
+
+
+@pytest.mark.parametrize("rw", [8, 16])
+@pytest.mark.parametrize("name", ["random", "chung_lu"])
+def test_format_and_results(name, rw):
+    torch = _torch()
+    csr = _graphs()[name]
+    order, pcsr = _permuted(csr)
+    g = acc.DeviceGraph(csr, acc.default_build_opts(rw, reorder_rows=1))
+    gp = acc.DeviceGraph(pcsr, acc.default_build_opts(rw))
+    assert g.info.reordered == 1 and gp.info.reordered == 0
+    assert np.array_equal(g.row_order(), order)
+    assert np.array_equal(gp.row_order(), np.arange(csr.rows))
+    ea, eb = g.export(), gp.export()
+    for k in eb:
+        assert np.array_equal(ea[k], eb[k]), k
+    ref = O.metcf_build(rw, pcsr.rows, pcsr.row_ptr, pcsr.col_idx, pcsr.data)
+    for k in ("rowWindowOffset", "tcOffset", "tcLocalBit", "sparseA2B"):
+        assert np.array_equal(ea[k], ref[k]), k
+    for K in (32, 100):
+        Bh = acc.gen_dense(K, csr.cols, K)
+        B = torch.from_numpy(Bh).cuda()
+        C64 = O.spmm_f64(csr.row_ptr, csr.col_idx, csr.data, Bh)
+        ab = O.spmm_abs_f64(csr.row_ptr, csr.col_idx, csr.data, Bh)
+        for form in FORMS:
+            for tile in (None, acc.Tile(2, 4, 12, 0, 64, 4), acc.Tile(4, 4, 0, 1, 64, 0)):
+                C = torch.full((csr.rows, K), float("nan"), device="cuda")
+                g.spmm(B, C, form, tile)
+                Cp = gp.spmm(B, None, form, tile)
+                torch.cuda.synchronize()
+                Cn = C.cpu().numpy()
+                assert np.array_equal(Cn[order], Cp.cpu().numpy()), (form, K)
+                mx, fr = O.errors(Cn, C64, ab)
+                assert mx <= TOL[form], (form, K, mx)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+def test_sweep_and_batch_and_saved(form, tmp_path):
+    torch = _torch()
+    graphs = _graphs()
+    built = []
+    for i, (name, csr) in enumerate(graphs.items()):
+        g = acc.DeviceGraph(csr, acc.default_build_opts(8, reorder_rows=1))
+        B = torch.from_numpy(acc.gen_dense(40 + i, csr.cols, 64)).cuda()
+        built.append((csr, g, B))
+    # a graph in input order in the same batch
+    csr0 = graphs["random"]
+    g0 = acc.DeviceGraph(csr0)
+    B0 = torch.from_numpy(acc.gen_dense(7, csr0.cols, 64)).cuda()
+    built.append((csr0, g0, B0))
+    alone = [g.spmm(B, None, form).clone() for _, g, B in built]
+    # sweep mode equals work units
+    for (_, g, B), C0 in zip(built, alone):
+        Cs = g.spmm(B, None, form, acc.Tile(2, 4, 0, 0, 64, 4, 1)).clone()
+        Cw = g.spmm(B, None, form, acc.Tile(2, 4, 0, 0, 64, 4, 0)).clone()
+        torch.cuda.synchronize()
+        assert torch.equal(Cs, Cw)
+    # batched launch equals standalone
+    Cs = [torch.full_like(C0, float("nan")) for C0 in alone]
+    batch = acc.Batch([(g, B, C) for (_, g, B), C in zip(built, Cs)], form)
+    batch.run()
+    torch.cuda.synchronize()
+    for C1, C0 in zip(Cs, alone):
+        assert torch.equal(C1, C0)
+    batch.close()
+    # save / load keeps the row order
+    csr, g, B = built[1]
+    path = tmp_path / "r.metcf"
+    g.save(path)
+    h = acc.DeviceGraph.load(path)
+    assert h.info.reordered == 1
+    assert np.array_equal(h.row_order(), g.row_order())
+    assert torch.equal(h.spmm(B, None, form), alone[1])
+
+
+def test_reorder_rejects_bad_row_ptr():
+    torch = _torch()
+    import ctypes
+    rp = torch.tensor([0, 3, 2, 4], dtype=torch.int32, device="cuda")   # decreasing
+    ci = torch.tensor([0, 1, 2, 0], dtype=torch.int32, device="cuda")
+    va = torch.ones(4, device="cuda")
+    view = acc.CsrView(3, 3, 4, rp.data_ptr(), ci.data_ptr(), va.data_ptr())
+    h = ctypes.c_void_p()
+    st = acc.lib().accspmm_metcf_build(ctypes.byref(view),
+                                       ctypes.byref(acc.default_build_opts(8, reorder_rows=1)),
+                                       ctypes.byref(h), None)
+    assert st == acc.Status.ERR_INVALID_ARG
+    st = acc.lib().accspmm_metcf_build(ctypes.byref(view),
+                                       ctypes.byref(acc.default_build_opts(8, reorder_rows=2)),
+                                       ctypes.byref(h), None)
+    assert st == acc.Status.ERR_UNSUPPORTED

@@ -19,19 +19,26 @@ def _fnv(data: bytes) -> int:
     return h
 
 
-def _file(rows=61, cols=50, seed=3):
+def _file(rows=61, cols=50, seed=3, version=1, order=None):
+    """Version 1: 19 header words.  Version 2: a 20th word (reorder_rows)
+    and, when it is 1, the row order after the arrays."""
     rng = np.random.default_rng(seed)
     rp, ci, va = random_csr(rng, rows, cols, 0.08, empty_rows=0.1)
     m = O.metcf_build(8, rows, rp, ci, va)
     W, T, nnz = len(m["rowWindowOffset"]) - 1, len(m["tcOffset"]) - 1, len(va)
     key = b"rw8-test"
     hdr = [rows, cols, nnz, 8, W, T, 0, 0, 0, 0, 0, 8, 8, 3, 1, 2, 16, 8, len(key)]
-    body = (b"ACCSPMTC" + struct.pack("<I", 1) + struct.pack("<19I", *hdr) + key +
+    if version == 2:
+        hdr.append(0 if order is None else 1)
+    body = (b"ACCSPMTC" + struct.pack("<I", version) + struct.pack(f"<{len(hdr)}I", *hdr) +
+            key +
             m["rowWindowOffset"].astype("<u4").tobytes() +
             m["tcOffset"].astype("<u4").tobytes() +
             m["tcLocalBit"].astype("<u8").tobytes() +
             m["sparseA2B"].astype("<u4").tobytes() +
             np.asarray(m["data"], "<f4").tobytes())
+    if order is not None:
+        body += np.asarray(order, "<u4").tobytes()
     return body + struct.pack("<Q", _fnv(body))
 
 
@@ -45,6 +52,18 @@ def _load(tmp_path, name, data) -> int:
 def test_valid_file_passes_format_checks(tmp_path):
     st = _load(tmp_path, "ok.metcf", _file())
     assert st in (acc.Status.OK, acc.Status.ERR_HIP), st
+    for order in (None, np.random.default_rng(4).permutation(61)):
+        st = _load(tmp_path, "ok2.metcf", _file(version=2, order=order))
+        assert st in (acc.Status.OK, acc.Status.ERR_HIP), st
+
+
+def test_row_order_must_be_a_permutation(tmp_path):
+    order = np.arange(61)
+    order[5] = 7                        # row 7 twice, row 5 never
+    assert _load(tmp_path, "dup.metcf", _file(version=2, order=order)) == acc.Status.ERR_FILE
+    order = np.arange(61)
+    order[0] = 61                       # out of range
+    assert _load(tmp_path, "oob.metcf", _file(version=2, order=order)) == acc.Status.ERR_FILE
 
 
 def test_corruptions_rejected(tmp_path):
@@ -64,7 +83,8 @@ def test_corruptions_rejected(tmp_path):
         "truncated": good[:-20],
         "trailing": good + b"x",
         "magic": b"ACCSPMTX" + good[8:],
-        "version": good[:8] + struct.pack("<I", 2) + good[12:],
+        "version": good[:8] + struct.pack("<I", 3) + good[12:],
+        "v2_header_on_v1_body": good[:8] + struct.pack("<I", 2) + good[12:],
         # header inconsistent with the arrays (re-summed, so only the
         # structural checks can catch it)
         "window_count": resum(set_word(body, 4, 3)),
