@@ -381,7 +381,11 @@ accspmm_status launch_form(int form, uint32_t rw, uint32_t nb,
 //     (split form: feature chunks pinned to XCDs and twice the panels);
 //   * short windows (banded matrices): one contiguous range of windows per
 //     XCD, so neighbouring windows share B rows in that XCD's L2;
-//   * windows much longer than the mean are split, short ones packed.
+//   * windows much longer than the mean are split, short ones packed;
+//   * rows reordered by length (long windows first): fewer panels and a
+//     split threshold near the mean window (split-BF16: 4 panels, feature
+//     chunks pinned to XCDs; FP32: no panels, pieces of 2x the mean; BF16:
+//     2 panels) — measured on the power-law graph at RW 8 and 16.
 void default_tile(const accspmm_metcf* h, uint32_t K, int form,
                   accspmm_tile* t) {
   uint32_t nb = K >= 128 && form == kBF16 ? 8 : K >= 64 ? 4 : K >= 32 ? 2 : 1;
@@ -393,7 +397,21 @@ void default_tile(const accspmm_metcf* h, uint32_t K, int form,
   t->pack_tiles = 64;
   const uint64_t kp = ((uint64_t)K + 16 * nb - 1) / (16 * nb) * (16 * nb);
   const uint64_t b_bytes = ((uint64_t)h->cols + 1) * kp * (form == kBF16 ? 2 : 4);
-  if (mean >= 64) {
+  if (mean >= 64 && h->row_perm.p) {
+    const uint32_t n_kc = (uint32_t)(kp / (16 * nb));
+    const uint32_t panels = b_bytes > (32ull << 20) ? 4 : b_bytes > (8ull << 20) ? 2 : 0;
+    t->xcd_slice = 0;
+    t->split_tiles = h->max_window_tiles > mean ? std::max<uint32_t>(64, mean) : 0;
+    if (form == kSPLIT) {
+      t->col_panels = panels;
+      if (n_kc >= 2 && 8 % n_kc == 0) t->xcd_slice = 1;
+    } else if (form == kFP32) {
+      t->col_panels = 0;
+      t->split_tiles = h->max_window_tiles > 2 * mean ? std::max<uint32_t>(64, 2 * mean) : 0;
+    } else {
+      t->col_panels = std::min<uint32_t>(panels, 2);
+    }
+  } else if (mean >= 64) {
     t->xcd_slice = 0;
     t->col_panels = b_bytes > (32ull << 20) ? 4 : b_bytes > (8ull << 20) ? 2 : 0;
     // split form: pin feature chunks to XCDs (each L2 then holds half of a

@@ -130,7 +130,7 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
   hipStream_t s = (hipStream_t)stream;
   const uint32_t mean = h->W ? std::max<uint32_t>(1, h->T / h->W) : 1;
   std::vector<uint32_t> splits = {0};
-  for (uint32_t f : {2u, 4u, 8u}) {
+  for (uint32_t f : {1u, 2u, 4u, 8u}) {
     uint32_t sp = std::max<uint32_t>(32, mean * f);
     if (h->max_window_tiles > sp) splits.push_back(sp);
   }
@@ -223,7 +223,7 @@ extern "C" const char* accspmm_status_string(accspmm_status s) {
     case ACCSPMM_ERR_MTX: return "Matrix Market parse error";
     case ACCSPMM_ERR_NOT_READY: return "not ready";
     case ACCSPMM_ERR_SHAPE: return "operand shapes do not agree";
-    case ACCSPMM_ERR_FILE: return "METCF file unreadable, corrupt or of another format";
+    case ACCSPMM_ERR_FILE: return "file not writable, or METCF file unreadable, corrupt or of another format";
   }
   return "unknown status";
 }

@@ -364,6 +364,11 @@ def main():
     ap.add_argument("--K", type=int, default=128)
     ap.add_argument("--forms", default="split_bf16,fp32,bf16")
     ap.add_argument("--tile", default="", help="nb,wpb,split,xcd (default: library pick)")
+    ap.add_argument("--row-window", type=int, default=16, choices=(8, 16),
+                    help="METCF row window of the timed graph")
+    ap.add_argument("--reorder", type=int, default=1, choices=(0, 1),
+                    help="build option reorder_rows (rows by decreasing length; C is "
+                         "still produced in input row order)")
     ap.add_argument("--tune", action="store_true", help="autotune each form first")
     ap.add_argument("--no-check", action="store_true", help="skip errors and CPU baseline")
     ap.add_argument("--serve-requests", type=int, default=10000,
@@ -419,11 +424,13 @@ def main():
     torch.cuda.init()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    g = acc.DeviceGraph(csr)
+    opts = acc.default_build_opts(args.row_window, reorder_rows=args.reorder)
+    g = acc.DeviceGraph(csr, opts)
     torch.cuda.synchronize()
     build_ms = (time.perf_counter() - t0) * 1e3
     inf = g.info
-    log(f"bench: METCF built on GPU in {build_ms:.1f} ms: {inf.num_windows} windows, "
+    log(f"bench: METCF built on GPU in {build_ms:.1f} ms (row window {inf.row_window}, "
+        f"reorder_rows {inf.reordered}): {inf.num_windows} windows, "
         f"{inf.num_tiles} tiles, {inf.num_groups} AdpBME groups, ibd={inf.ibd}")
     B = torch.from_numpy(Bh).cuda()
     Cd = torch.empty((csr.rows, K), dtype=torch.float32, device="cuda")
@@ -467,6 +474,24 @@ def main():
                          f"median of {cpu.pop('runs')} runs")
         log(f"bench: CPU baseline {cpu['value']:.1f} GFLOP/s on {cpu['cores']} threads")
 
+    # the head form on the reference's format options (row window 8, rows in
+    # input order), for comparison
+    input_order = None
+    if args.reorder or args.row_window != 8:
+        del g
+        torch.cuda.empty_cache()
+        g = acc.DeviceGraph(csr)
+        form = name_to_form[forms[0]]
+        tile = g.tile_select(K, form)
+        r = time_form(g, B, Cd, form, tile, args.steps, args.warmup)
+        input_order = {"row_window": 8, "reorder_rows": 0, "form": forms[0],
+                       "tiles": g.info.num_tiles, "tile": list(tile.as_tuple()),
+                       "ms_per_step": round(r["ms_per_step"], 4),
+                       "kernel_ms": round(r["kernel_ms_mean"], 4),
+                       "gflops": round(2.0 * csr.nnz * K / (r["ms_per_step"] * 1e-3) / 1e9, 2)}
+        log(f"bench: {forms[0]} on input-order rows, row window 8: "
+            f"{r['ms_per_step']:.3f} ms/step, {input_order['gflops']:.1f} GFLOP/s")
+
     del g, B, Cd, outputs
     torch.cuda.empty_cache()
     batching = None
@@ -483,7 +508,9 @@ def main():
     ab = alg_bytes(csr.rows, csr.cols, csr.nnz, K, head_form)
     achieved = ab / (hr["kernel_ms_mean"] * 1e-3) / 1e9
     traffic = None
-    tfile = REPO / "profiles" / f"traffic_{args.workload}_k{K}_{head}.json"
+    variant = "" if (inf.row_window == 8 and not inf.reordered) else \
+        f"_rw{inf.row_window}" + ("_reordered" if inf.reordered else "")
+    tfile = REPO / "profiles" / f"traffic_{args.workload}_k{K}_{head}{variant}.json"
     if tfile.exists():
         traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
     out = {
@@ -501,6 +528,7 @@ def main():
         "data": "synthetic (seeded generators, A and B resident in HBM)",
         "config": {"workload": args.workload, "rows": csr.rows, "nnz": csr.nnz, "K": K,
                    "form": head, "row_window": inf.row_window,
+                   "reorder_rows": inf.reordered,
                    "tiles": inf.num_tiles, "tile": hr["tile"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -513,6 +541,7 @@ def main():
         "batching": batching,
         "serving": serving,
         "build_ms": round(build_ms, 1),
+        "input_order": input_order,
         "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv)
                       for kk, vv in v.items()} for k, v in per_form.items()},
     }
