@@ -89,7 +89,7 @@ class Tile(C.Structure):
     _fields_ = [("feat_per_lane", C.c_uint32), ("waves_per_block", C.c_uint32),
                 ("split_tiles", C.c_uint32), ("xcd_slice", C.c_uint32),
                 ("pack_tiles", C.c_uint32), ("col_panels", C.c_uint32),
-                ("sweep", C.c_uint32)]
+                ("sweep", C.c_uint32), ("b_split", C.c_uint32)]
 
     def as_tuple(self):
         return (self.feat_per_lane, self.waves_per_block, self.split_tiles, self.xcd_slice,

@@ -361,10 +361,12 @@ accspmm_status launch_form(int form, uint32_t rw, uint32_t nb,
   if (rw == 8) {
     if (form == kFP32) return launch_nb<kFP32, 8>(nb, g, blocks, wpb, packed, s);
     if (form == kSPLIT) return launch_nb<kSPLIT, 8>(nb, g, blocks, wpb, packed, s);
+    if (form == kSPLITF) return launch_nb<kSPLITF, 8>(nb, g, blocks, wpb, packed, s);
     return launch_nb<kBF16, 8>(nb, g, blocks, wpb, packed, s);
   }
   if (form == kFP32) return launch_nb<kFP32, 16>(nb, g, blocks, wpb, packed, s);
   if (form == kSPLIT) return launch_nb<kSPLIT, 16>(nb, g, blocks, wpb, packed, s);
+  if (form == kSPLITF) return launch_nb<kSPLITF, 16>(nb, g, blocks, wpb, packed, s);
   return launch_nb<kBF16, 16>(nb, g, blocks, wpb, packed, s);
 }
 
@@ -427,6 +429,18 @@ void default_tile(const accspmm_metcf* h, uint32_t K, int form,
   }
 }
 
+// Split-BF16 B path when the tile leaves it open.  The conversion pass costs
+// one read and one write of B; splitting in registers costs a few VALU
+// operations per gathered element instead, i.e. per tile column.  With few
+// gathers per B row (banded matrices: ~14 on the 27-point stencil) the
+// in-register split wins (0.84 -> 0.67 ms per step at 100^3, K=128); with
+// many (~380 on the 232,965-node power-law graph) the conversion pass does
+// (3.08 vs 3.28 ms).  Threshold: 64 gathered tile columns per B row.
+uint32_t default_b_split(const accspmm_metcf* h, uint32_t K) {
+  (void)K;  // both costs scale with K
+  return (uint64_t)h->T * kColWindow < 64ull * h->cols ? 2u : 1u;
+}
+
 accspmm_status ensure_vsplit(const accspmm_metcf* h, hipStream_t s) {
   if (h->vsplit.p) return ACCSPMM_OK;
   const uint64_t n = (uint64_t)h->nnz + kValuePad;
@@ -456,8 +470,11 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   if (wpb > 8) return ACCSPMM_ERR_UNSUPPORTED;
   const uint32_t KC = 16 * nb;
   const uint32_t n_kc = (K + KC - 1) / KC;
-  // FP32 reads the caller's B with NB-wide vector loads: needs alignment
-  if (form == kFP32) {
+  // split-BF16 with b_split 2: the kernel splits the caller's B itself
+  const bool fused = form == kSPLIT && tile.b_split == 2;
+  const int kform = fused ? (int)kSPLITF : form;
+  // FP32 B is read with NB-wide vector loads: needs alignment
+  if (raw_b(kform)) {
     if ((ldb % nb) != 0 || ((uintptr_t)B % (4u * nb)) != 0) return ACCSPMM_ERR_UNSUPPORTED;
   }
   if ((phases & 2) && ((ldc % nb) != 0 || ((uintptr_t)C % (4u * nb)) != 0))
@@ -498,7 +515,7 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
     g.partials = h->partials.as<float>();
   }
 
-  if (form == kFP32) {
+  if (raw_b(kform)) {
     g.B = B;
     g.ldb = ldb;
   } else {
@@ -521,7 +538,7 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
     g.ldb = ldx;
   }
   if (!(phases & 2)) return ACCSPMM_OK;
-  if (tile.sweep) return sweep_launch(h, g, tile, nb, form, sch, s);
+  if (tile.sweep) return sweep_launch(h, g, tile, nb, kform, sch, s);
 
   uint64_t blocks;
   if (g.xcd_slice == 1) {
@@ -535,7 +552,7 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   }
   if (blocks == 0) return ACCSPMM_OK;
   if (blocks > 0x7FFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
-  ACC_TRY(launch_form(form, h->rw, nb, g, (uint32_t)blocks, wpb, sch->packed, s));
+  ACC_TRY(launch_form(kform, h->rw, nb, g, (uint32_t)blocks, wpb, sch->packed, s));
   ACC_HIP(hipGetLastError());
   if (sch->num_split_windows) {
     if (h->rw == 8)
@@ -570,11 +587,16 @@ accspmm_status spmm_fit_tile(const accspmm_metcf* h, const float* B,
   } else {
     ACC_TRY(accspmm_tile_select(h, K, (accspmm_form)form, &t));
   }
+  if (form != kSPLIT) t.b_split = 0;
+  else if (t.b_split == 0) t.b_split = default_b_split(h, K);
   while (t.feat_per_lane > 1 &&
          ((form == kFP32 &&
            ((ldb % t.feat_per_lane) || ((uintptr_t)B % (4u * t.feat_per_lane)))) ||
           (C && ((ldc % t.feat_per_lane) || ((uintptr_t)C % (4u * t.feat_per_lane))))))
     t.feat_per_lane /= 2;
+  // the in-register split reads B like the FP32 form; otherwise convert
+  if (t.b_split == 2 && ((ldb % t.feat_per_lane) || ((uintptr_t)B % (4u * t.feat_per_lane))))
+    t.b_split = 1;
   *out = t;
   return ACCSPMM_OK;
 }

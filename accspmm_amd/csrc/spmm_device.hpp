@@ -16,7 +16,13 @@ typedef float f2 __attribute__((__vector_size__(2 * sizeof(float))));
 typedef float f4 __attribute__((__vector_size__(4 * sizeof(float))));
 typedef uint32_t u4 __attribute__((__vector_size__(4 * sizeof(uint32_t))));
 
-enum : int { kFP32 = 0, kSPLIT = 1, kBF16 = 2 };
+// kSPLITF: the split-BF16 form reading the caller's FP32 B and splitting it
+// into bf16 hi and lo in registers, with the rounding the conversion pass
+// uses (the same C bits, no converted copy of B)
+enum : int { kFP32 = 0, kSPLIT = 1, kBF16 = 2, kSPLITF = 3 };
+constexpr bool is_split(int f) { return f == kSPLIT || f == kSPLITF; }
+// forms that gather the caller's FP32 B (padded columns read row 0, masked)
+constexpr bool raw_b(int f) { return f == kFP32 || f == kSPLITF; }
 
 struct SpmmArgs {
   const uint32_t* __restrict__ rwo;
@@ -246,7 +252,7 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     load_brow<FORM, NB, FULL>(st.b[j], B, ldb, col[j], feat, g.K, g.zero_row);
-    if (FORM == kFP32) bm |= (col[j] != kPad ? 1u : 0u) << j;
+    if (raw_b(FORM)) bm |= (col[j] != kPad ? 1u : 0u) << j;
   }
   st.bmask = bm;
   // the group's first value: tcOffset of quarter 0's tile (lane 0); the
@@ -304,25 +310,42 @@ __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
   u4 aop, alo;
   {
     const uint32_t sel_hi = 0x05040100u, sel_lo = 0x07060302u;
-    const uint32_t sel = (FORM == kSPLIT && RW == 8 && m >= 8) ? sel_lo : sel_hi;
+    const uint32_t sel = (is_split(FORM) && RW == 8 && m >= 8) ? sel_lo : sel_hi;
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
       aop[d] = __builtin_amdgcn_perm(a[2 * d + 1], a[2 * d], sel);
-      alo[d] = (FORM == kSPLIT && RW == 16) ? __builtin_amdgcn_perm(a[2 * d + 1], a[2 * d], sel_lo)
+      alo[d] = (is_split(FORM) && RW == 16) ? __builtin_amdgcn_perm(a[2 * d + 1], a[2 * d], sel_lo)
                                             : 0u;
     }
   }
   const bf16x8 A = __builtin_bit_cast(bf16x8, aop);
   const bf16x8 AL = __builtin_bit_cast(bf16x8, alo);
+  if (FORM == kSPLITF) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const bool bv = (st.bmask >> j) & 1u;
+#pragma unroll
+      for (int i = 0; i < NB; ++i) st.b[j].w[i] = bv ? st.b[j].w[i] : 0u;
+    }
+  }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    if (FORM == kSPLIT) {
+    if (is_split(FORM)) {
       u4 bh, bl;
 #pragma unroll
       for (int d = 0; d < 4; ++d) {
         uint32_t x = st.b[2 * d].w[b], y = st.b[2 * d + 1].w[b];
-        bh[d] = __builtin_amdgcn_perm(y, x, 0x05040100u);
-        bl[d] = __builtin_amdgcn_perm(y, x, 0x07060302u);
+        if (FORM == kSPLITF) {
+          // hi = bf16(x), lo = bf16(x - hi), both rounded to nearest even,
+          // for the two columns of the pair at once
+          const float fx = __builtin_bit_cast(float, x), fy = __builtin_bit_cast(float, y);
+          const uint32_t hp = pack_bf16(fx, fy);
+          bh[d] = hp;
+          bl[d] = pack_bf16(fx - bf16_lo_to_f32(hp), fy - bf16_hi_to_f32(hp));
+        } else {
+          bh[d] = __builtin_amdgcn_perm(y, x, 0x05040100u);
+          bl[d] = __builtin_amdgcn_perm(y, x, 0x07060302u);
+        }
       }
       const bf16x8 BH = __builtin_bit_cast(bf16x8, bh);
       const bf16x8 BL = __builtin_bit_cast(bf16x8, bl);
@@ -378,7 +401,7 @@ __device__ inline void store_window(f4* acc, uint32_t w, uint32_t q,
                                     uint32_t feat, uint32_t nvalid, float* C,
                                     uint32_t ldc, uint32_t rows,
                                     const uint32_t* rmap) {
-  if (FORM == kSPLIT && RW == 8) {
+  if (is_split(FORM) && RW == 8) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
@@ -520,11 +543,11 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
   const uint32_t feat = k0 + NB * n;
   // B loads are full-width unless this chunk straddles K (FP32 form; the
   // converted BF16-form buffers are zero-padded to a multiple of KC)
-  const bool full = FORM != kFP32 || k0 + KC <= K;
+  const bool full = !raw_b(FORM) || k0 + KC <= K;
   // A rows: lanes m = n carry tile row m (RW=16) or m%8 (RW=8; rows 8-15
   // only in the split form, as the low halves)
   const uint32_t m = n;
-  const bool a_rows = RW == 16 || m < 8 || FORM == kSPLIT;
+  const bool a_rows = RW == 16 || m < 8 || is_split(FORM);
   const uint32_t nvalid = feat < K ? min((uint32_t)NB, K - feat) : 0u;
 
   Accum<FORM, NB, RW> out;
@@ -565,7 +588,7 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
   }
   // split piece: write partial (RW rows x kc_padded floats per slot)
   f4* acc = out.acc;
-  if (FORM == kSPLIT && RW == 8) {
+  if (is_split(FORM) && RW == 8) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll

@@ -47,7 +47,7 @@ constexpr uint32_t slot_lanes() { return RW == 8 ? 32u : 64u; }
 template <int FORM, int NB, int RW>
 __device__ inline void finish_piece(f4* acc, f4* slot, uint32_t lane, bool first,
                                     bool split) {
-  if (FORM == kSPLIT && RW == 8) {
+  if (is_split(FORM) && RW == 8) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
   if (v >= a.NW) return;
   const uint32_t lane = threadIdx.x & 63u, q = lane >> 4, n = lane & 15u;
   const uint32_t m = n;
-  const bool a_rows = RW == 16 || m < 8 || FORM == kSPLIT;
+  const bool a_rows = RW == 16 || m < 8 || is_split(FORM);
   f4* slots = lds_slots + (size_t)wv * a.M * NB * NL;
   float* lv = lds_vals + wv * kWaveLds;
   if (lane == 0) lv[kGroupValues] = 0.f;
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
     for (uint32_t kc = 0; kc < g.n_kc; ++kc) {
       const uint32_t k0 = kc * KC, feat = k0 + NB * n;
       // B loads are full-width unless the chunk straddles K (FP32 form)
-      const bool full = FORM != kFP32 || k0 + KC <= g.K;
+      const bool full = !raw_b(FORM) || k0 + KC <= g.K;
       const uint32_t nvalid = feat < g.K ? min((uint32_t)NB, g.K - feat) : 0u;
       uint32_t started = 0;
       if (ng) {
@@ -389,10 +389,12 @@ accspmm_status sweep_launch(const accspmm_metcf* h, const dev::SpmmArgs& g,
   if (h->rw == 8) {
     if (form == kFP32) return launch_sweep_nb<kFP32, 8>(nb, a, cus, wpb, lds, s);
     if (form == kSPLIT) return launch_sweep_nb<kSPLIT, 8>(nb, a, cus, wpb, lds, s);
+    if (form == kSPLITF) return launch_sweep_nb<kSPLITF, 8>(nb, a, cus, wpb, lds, s);
     return launch_sweep_nb<kBF16, 8>(nb, a, cus, wpb, lds, s);
   }
   if (form == kFP32) return launch_sweep_nb<kFP32, 16>(nb, a, cus, wpb, lds, s);
   if (form == kSPLIT) return launch_sweep_nb<kSPLIT, 16>(nb, a, cus, wpb, lds, s);
+  if (form == kSPLITF) return launch_sweep_nb<kSPLITF, 16>(nb, a, cus, wpb, lds, s);
   return launch_sweep_nb<kBF16, 16>(nb, a, cus, wpb, lds, s);
 }
 

@@ -52,7 +52,7 @@ void load_file(const char* path) {
   std::ifstream in(path);
   if (!in) return;
   std::string line;
-  // one entry per line: "key": [nb, wpb, split, xcd, pack, panels, sweep],
+  // one entry per line: "key": [nb, wpb, split, xcd, pack, panels, sweep, b_split],
   while (std::getline(in, line)) {
     size_t a = line.find('"'), b = line.find('"', a + 1);
     size_t l = line.find('['), r = line.find(']');
@@ -60,11 +60,11 @@ void load_file(const char* path) {
         l == std::string::npos || r == std::string::npos)
       continue;
     std::string key = line.substr(a + 1, b - a - 1);
-    unsigned v[7] = {0, 0, 0, 0, 0, 0, 0};
-    const int got = std::sscanf(line.c_str() + l + 1, "%u , %u , %u , %u , %u , %u , %u",
-                                &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6]);
-    if (got >= 5)  // older entries lack col_panels and sweep (both 0)
-      g_cache[key] = accspmm_tile{v[0], v[1], v[2], v[3], v[4], v[5], v[6]};
+    unsigned v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int got = std::sscanf(line.c_str() + l + 1, "%u , %u , %u , %u , %u , %u , %u , %u",
+                                &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7]);
+    if (got >= 5)  // older entries lack col_panels, sweep and b_split (all 0)
+      g_cache[key] = accspmm_tile{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
   }
 }
 
@@ -77,7 +77,8 @@ void save_file(const char* path) {
     out << "  \"" << kv.first << "\": [" << kv.second.feat_per_lane << ", "
         << kv.second.waves_per_block << ", " << kv.second.split_tiles << ", "
         << kv.second.xcd_slice << ", " << kv.second.pack_tiles << ", "
-        << kv.second.col_panels << ", " << kv.second.sweep << "]"
+        << kv.second.col_panels << ", " << kv.second.sweep << ", " << kv.second.b_split
+        << "]"
         << (++i < g_cache.size() ? "," : "")
         << "\n";
   }
@@ -174,7 +175,7 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
           for (uint32_t pack : {0u, 64u}) {
             if (xcd == 1 && (n_kc == 1 || (8 % n_kc) != 0)) continue;
             if (pack && mean >= 64) continue;  // windows already long
-            consider(accspmm_tile{nb, wpb, sp, xcd, pack, 0, 0});
+            consider(accspmm_tile{nb, wpb, sp, xcd, pack, 0, 0, 0});
           }
   }
   // stage 2: column panels around the best shape, when the B columns one
@@ -193,6 +194,15 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
           c.split_tiles = sp;
           consider(c);
         }
+    }
+  }
+  // stage 3 (split-BF16): both ways of splitting B, around the best shape
+  if (bms < 1e30f && form == ACCSPMM_FORM_SPLIT_BF16) {
+    const accspmm_tile base = bt;
+    for (uint32_t bs : {1u, 2u}) {
+      accspmm_tile c = base;
+      c.b_split = bs;
+      consider(c);
     }
   }
   (void)hipEventDestroy(e0);

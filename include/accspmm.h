@@ -182,6 +182,12 @@ typedef struct accspmm_tile {
                               stay in LDS and runs their pieces panel by
                               panel (same pieces and result bits as 0; the
                               work-unit mapping fields are ignored)          */
+  uint32_t b_split;        /* split-BF16 form, how B is split into bf16 hi and
+                              lo: 1 a conversion pass writes a split copy of B
+                              that the kernel gathers; 2 the kernel gathers
+                              the FP32 B and splits it in registers (no copy,
+                              the same C bits); 0 the library's choice.  The
+                              batched and served paths use 1.              */
 } accspmm_tile;
 
 /* C[rows x K] = A * B[cols x K].  B (ldb >= K) and C (ldc >= K) are

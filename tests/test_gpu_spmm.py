@@ -211,3 +211,51 @@ def test_invalid_arguments():
                           None) == acc.Status.ERR_SHAPE
     assert L.accspmm_spmm(g.handle, B.data_ptr(), 8, Cm.data_ptr(), 8, 8, 7, None,
                           None) == acc.Status.ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("rw", [8, 16])
+def test_split_in_registers_equals_conversion(rw):
+    # b_split 2 (the kernel splits the FP32 B) and 1 (a conversion pass)
+    # round B the same way: C must be bit-identical, K tails included
+    torch = _torch()
+    rng = np.random.default_rng(60 + rw)
+    rows, cols = 1501, 2300
+    rp, ci, va = random_csr(rng, rows, cols, 0.03, empty_rows=0.1, heavy_rows=3)
+    csr = acc.CSR(rows, cols, rp, ci, va)
+    g = acc.DeviceGraph(csr, acc.default_build_opts(rw))
+    for K in (8, 17, 64, 100, 128):
+        Bh = acc.gen_dense(K, cols, K)
+        Bh[5, :] = np.inf if K == 17 else Bh[5, :]   # specials split the same way too
+        B = torch.from_numpy(Bh).cuda()
+        for nb, split, panels, sweep in ((1, 0, 0, 0), (2, 16, 0, 0), (4, 0, 4, 0),
+                                         (8, 24, 2, 0), (4, 0, 4, 1)):
+            outs = []
+            for bs in (1, 2):
+                tile = acc.Tile(nb, 4, split, 0, 64, panels, sweep, bs)
+                outs.append(g.spmm(B, None, acc.FORM_SPLIT_BF16, tile).clone())
+            torch.cuda.synchronize()
+            assert torch.equal(outs[0], outs[1]) or (
+                torch.equal(torch.isnan(outs[0]), torch.isnan(outs[1])) and
+                torch.equal(torch.nan_to_num(outs[0]), torch.nan_to_num(outs[1]))), \
+                (K, nb, split, panels, sweep)
+        if K != 17:
+            C64 = O.spmm_f64(rp, ci, va, Bh)
+            ab = O.spmm_abs_f64(rp, ci, va, Bh)
+            mx, fr = O.errors(outs[1].cpu().numpy(), C64, ab)
+            assert mx <= TOL[acc.FORM_SPLIT_BF16], (K, mx)
+
+
+def test_split_in_registers_unaligned_b_falls_back():
+    # a B view with ldb % NB != 0 cannot take the in-register path: the
+    # library converts instead, with the same result
+    torch = _torch()
+    rng = np.random.default_rng(61)
+    rp, ci, va = random_csr(rng, 400, 500, 0.05)
+    g = acc.DeviceGraph(acc.CSR(400, 500, rp, ci, va))
+    Bh = acc.gen_dense(3, 500, 66)
+    Bfull = torch.from_numpy(Bh).cuda()
+    Bv = Bfull[:, :64]                                  # ldb = 66
+    ref = g.spmm(Bv.contiguous(), None, acc.FORM_SPLIT_BF16, acc.Tile(4, 4, 0, 0, 64, 0, 0, 1))
+    got = g.spmm(Bv, None, acc.FORM_SPLIT_BF16, acc.Tile(4, 4, 0, 0, 64, 0, 0, 2))
+    torch.cuda.synchronize()
+    assert torch.equal(ref, got)
