@@ -174,6 +174,8 @@ _SIGS = {
     "accspmm_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
     "accspmm_server_register": (C.c_int, [C.c_void_p, C.POINTER(CsrView),
                                           C.POINTER(C.c_uint32)]),
+    "accspmm_server_register_opts": (C.c_int, [C.c_void_p, C.POINTER(CsrView),
+                                               C.POINTER(BuildOpts), C.POINTER(C.c_uint32)]),
     "accspmm_server_register_file": (C.c_int, [C.c_void_p, C.c_char_p,
                                                C.POINTER(C.c_uint32)]),
     "accspmm_server_unregister": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -607,15 +609,18 @@ class Server:
         self.handle = h
         self._keep = {}
 
-    def register(self, csr: CSR) -> int:
+    def register(self, csr: CSR, opts: BuildOpts | None = None) -> int:
+        """Builds and registers a resident graph (build options: row window,
+        reorder_rows; None = defaults)."""
         import torch
         rp, ci, va = csr.to_device()
         view = CsrView(csr.rows, csr.cols, csr.nnz, rp.data_ptr(), ci.data_ptr(),
                        va.data_ptr())
         gid = C.c_uint32()
         torch.cuda.synchronize()
-        _check(lib().accspmm_server_register(self.handle, C.byref(view), C.byref(gid)),
-               "server_register")
+        _check(lib().accspmm_server_register_opts(self.handle, C.byref(view),
+                                                  C.byref(opts) if opts is not None else None,
+                                                  C.byref(gid)), "server_register")
         return gid.value
 
     def register_file(self, path) -> int:

@@ -305,16 +305,24 @@ accspmm_status register_handle(accspmm_server* s, accspmm_metcf* h, uint32_t* gr
 }
 }  // namespace
 
-extern "C" accspmm_status accspmm_server_register(accspmm_server* s,
-                                                  const accspmm_csr_view* csr,
-                                                  uint32_t* graph_id) {
+extern "C" accspmm_status accspmm_server_register_opts(accspmm_server* s,
+                                                       const accspmm_csr_view* csr,
+                                                       const accspmm_build_opts* opts,
+                                                       uint32_t* graph_id) {
   if (!s || !csr || !graph_id) return ACCSPMM_ERR_INVALID_ARG;
   accspmm_build_opts bo;
-  accspmm_build_opts_default(&bo);
-  bo.build_adpbme = 0;
+  if (opts) bo = *opts;
+  else accspmm_build_opts_default(&bo);
+  bo.build_adpbme = 0;  // the served kernels use work units, not AdpBME groups
   accspmm_metcf* h = nullptr;
   ACC_TRY(accspmm_metcf_build(csr, &bo, &h, s->setup));
   return register_handle(s, h, graph_id);
+}
+
+extern "C" accspmm_status accspmm_server_register(accspmm_server* s,
+                                                  const accspmm_csr_view* csr,
+                                                  uint32_t* graph_id) {
+  return accspmm_server_register_opts(s, csr, nullptr, graph_id);
 }
 
 extern "C" accspmm_status accspmm_server_register_file(accspmm_server* s, const char* path,

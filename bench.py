@@ -301,17 +301,20 @@ def batch_bench(ngraphs: int, steps: int, warmup: int):
     return out
 
 
-def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000):
+def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000,
+                row_window: int = 0, reorder: int = 2):
     """Serving: Poisson arrivals over a pool of 64 resident Chung-Lu graphs
     (500-50,000 nodes, mean degree U[4,64], exponent U[2.0,2.5]) plus two
     2M-node mean-degree-32 graphs; each request is one graph (uniform) and K
     uniform in {32, 128}, split-BF16.  Runs the continuous-batching server and
     the one-launch-per-request baseline: saturation throughput of each, then
-    p50/p99 latency of both at the same fixed QPS."""
+    p50/p99 latency of both at the same fixed QPS.  Graphs are registered
+    with the given row window and reorder_rows build options."""
     import numpy as np
     import torch
 
     import accspmm_amd as acc
+    opts = acc.default_build_opts(row_window, reorder_rows=reorder)
     rng = np.random.default_rng(3000)
     pool = []
     for i in range(64):
@@ -354,7 +357,7 @@ def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000):
     for mode in ("one_per_launch", "batched"):
         srv = acc.Server(hbm_budget_bytes=200 << 30, max_batch=64, max_inflight=3,
                          form=acc.FORM_SPLIT_BF16, one_per_launch=(mode == "one_per_launch"))
-        gids = [srv.register(c) for c in pool]
+        gids = [srv.register(c, opts) for c in pool]
         run(srv, 0)  # warm-up: schedules, workspaces
         wall, lats = run(srv, 0)
         st = srv.stats()
@@ -368,7 +371,7 @@ def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000):
     for mode in ("one_per_launch", "batched"):
         srv = acc.Server(hbm_budget_bytes=200 << 30, max_batch=64, max_inflight=3,
                          form=acc.FORM_SPLIT_BF16, one_per_launch=(mode == "one_per_launch"))
-        gids = [srv.register(c) for c in pool]
+        gids = [srv.register(c, opts) for c in pool]
         run(srv, 0)
         wall, lats = run(srv, qps)
         srv.close()
@@ -380,6 +383,8 @@ def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000):
     out["throughput_ratio"] = (out["batched"]["saturation_req_per_s"] /
                                out["one_per_launch"]["saturation_req_per_s"])
     out["requests"] = requests
+    out["row_window"] = row_window
+    out["reorder_rows"] = reorder
     return out
 
 
@@ -392,11 +397,12 @@ def main():
     ap.add_argument("--K", type=int, default=128)
     ap.add_argument("--forms", default="split_bf16,fp32,bf16")
     ap.add_argument("--tile", default="", help="nb,wpb,split,xcd (default: library pick)")
-    ap.add_argument("--row-window", type=int, default=16, choices=(8, 16),
-                    help="METCF row window of the timed graph")
-    ap.add_argument("--reorder", type=int, default=1, choices=(0, 1),
-                    help="build option reorder_rows (rows by decreasing length; C is "
-                         "still produced in input row order)")
+    ap.add_argument("--row-window", type=int, default=0, choices=(0, 8, 16),
+                    help="METCF row window of the timed graph (0: the library's automatic "
+                         "choice)")
+    ap.add_argument("--reorder", type=int, default=2, choices=(0, 1, 2),
+                    help="build option reorder_rows (1: rows by decreasing length, C still "
+                         "in input row order; 2: the library's automatic choice)")
     ap.add_argument("--tune", action="store_true", help="autotune each form first")
     ap.add_argument("--no-check", action="store_true", help="skip errors and CPU baseline")
     ap.add_argument("--serve-requests", type=int, default=10000,
@@ -507,7 +513,7 @@ def main():
     # the head form on the reference's format options (row window 8, rows in
     # input order), for comparison
     input_order = None
-    if args.reorder or args.row_window != 8:
+    if inf.reordered or inf.row_window != 8:
         del g
         torch.cuda.empty_cache()
         g = acc.DeviceGraph(csr)
@@ -532,7 +538,8 @@ def main():
         torch.cuda.empty_cache()
     serving = None
     if args.serve_requests > 0:
-        serving = serve_bench(args.serve_requests)
+        serving = serve_bench(args.serve_requests, row_window=args.row_window,
+                              reorder=args.reorder)
 
     head = forms[0]
     hr = per_form[head]

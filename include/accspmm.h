@@ -61,7 +61,8 @@ typedef struct accspmm_csr_view {
  * (:55), IMBALANCE_THRESHOLD 8 (:62) and the driver's CSR2AdpBME(csr,3,1,2)
  * cost model (src/mma_tf32.cu:222). */
 typedef struct accspmm_build_opts {
-  uint32_t row_window;          /* 8 or 16 (16 => 128-bit tiles)            */
+  uint32_t row_window;          /* 8 or 16 (16 => 128-bit tiles); 0: 16 when
+                                   rows average >= 128 nonzeros, else 8     */
   uint32_t col_window;          /* must be 8                                */
   uint32_t t_load, t_comp, t_write, target_num; /* AdpBME cost model        */
   uint32_t imbalance_threshold; /* balanceStrategy threshold                */
@@ -71,7 +72,8 @@ typedef struct accspmm_build_opts {
    * so 8-row windows of like rows share more columns: fewer tiles and fewer
    * B rows gathered.  C is still written in input row order.  No reference
    * counterpart (the reference reads an externally reordered file, named
-   * ddi.my_reorder.mtx in its tree). */
+   * ddi.my_reorder.mtx in its tree).  2: reorder when rows average >= 128
+   * nonzeros (measured faster there, slower on short-row graphs). */
   uint32_t reorder_rows;
 } accspmm_build_opts;
 
@@ -187,7 +189,9 @@ typedef struct accspmm_tile {
                               that the kernel gathers; 2 the kernel gathers
                               the FP32 B and splits it in registers (no copy,
                               the same C bits); 0 the library's choice.  The
-                              batched and served paths use 1.              */
+                              batched and served paths always use 1 (a second
+                              instantiation in the persistent kernel slowed
+                              every problem, measured).                   */
 } accspmm_tile;
 
 /* C[rows x K] = A * B[cols x K].  B (ldb >= K) and C (ldc >= K) are
@@ -352,6 +356,12 @@ accspmm_status accspmm_server_create(const accspmm_server_opts* opts,
 accspmm_status accspmm_server_register(accspmm_server* s,
                                        const accspmm_csr_view* csr,
                                        uint32_t* graph_id);
+/* Same, built with the given options (row window, reorder_rows; NULL: the
+ * defaults; AdpBME groups are never built for served graphs). */
+accspmm_status accspmm_server_register_opts(accspmm_server* s,
+                                            const accspmm_csr_view* csr,
+                                            const accspmm_build_opts* opts,
+                                            uint32_t* graph_id);
 /* Registers a resident graph saved by accspmm_metcf_save (warm start). */
 accspmm_status accspmm_server_register_file(accspmm_server* s, const char* path,
                                             uint32_t* graph_id);

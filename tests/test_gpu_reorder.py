@@ -124,7 +124,29 @@ def test_reorder_rejects_bad_row_ptr():
                                        ctypes.byref(acc.default_build_opts(8, reorder_rows=1)),
                                        ctypes.byref(h), None)
     assert st == acc.Status.ERR_INVALID_ARG
-    st = acc.lib().accspmm_metcf_build(ctypes.byref(view),
-                                       ctypes.byref(acc.default_build_opts(8, reorder_rows=2)),
-                                       ctypes.byref(h), None)
-    assert st == acc.Status.ERR_UNSUPPORTED
+    for rw, ro in ((8, 3), (12, 0)):
+        st = acc.lib().accspmm_metcf_build(ctypes.byref(view),
+                                           ctypes.byref(acc.default_build_opts(rw, reorder_rows=ro)),
+                                           ctypes.byref(h), None)
+        assert st == acc.Status.ERR_UNSUPPORTED
+
+
+def test_automatic_choices():
+    # row_window 0 and reorder_rows 2: 16-row windows and reordered rows for
+    # rows averaging >= 128 nonzeros, the reference's layout otherwise; the
+    # result is the same as asking for that layout explicitly
+    torch = _torch()
+    rng = np.random.default_rng(12)
+    long_rp, long_ci, long_va = random_csr(rng, 300, 2000, 0.08)      # ~160 per row
+    short = acc.gen_chung_lu(5000, 20.0, 2.1, 9, 0)
+    for csr, want in ((acc.CSR(300, 2000, long_rp, long_ci, long_va), (16, 1)), (short, (8, 0))):
+        assert (csr.nnz >= 128 * csr.rows) == (want == (16, 1))
+        g = acc.DeviceGraph(csr, acc.default_build_opts(0, reorder_rows=2))
+        assert (g.info.row_window, g.info.reordered) == want
+        ge = acc.DeviceGraph(csr, acc.default_build_opts(want[0], reorder_rows=want[1]))
+        ea, eb = g.export(), ge.export()
+        for k in eb:
+            assert np.array_equal(ea[k], eb[k]), k
+        B = torch.from_numpy(acc.gen_dense(4, csr.cols, 64)).cuda()
+        assert torch.equal(g.spmm(B, None, acc.FORM_SPLIT_BF16),
+                           ge.spmm(B, None, acc.FORM_SPLIT_BF16))

@@ -21,16 +21,19 @@ def _pool(n, seed):
     return out
 
 
-@pytest.mark.parametrize("large_work", [0, 3_000_000])
-def test_mix_equals_alone(large_work):
+@pytest.mark.parametrize("large_work,layout", [(0, "input"), (3_000_000, "input"),
+                                               (0, "rw16_reordered")])
+def test_mix_equals_alone(large_work, layout):
     # large_work = 3e6 makes the requests of the bigger graphs (nnz * K above
-    # it) run in launches of their own beside batches of the small ones
+    # it) run in launches of their own beside batches of the small ones;
+    # rw16_reordered registers graphs with 16-row windows and reordered rows
     import torch
     pool = _pool(8, 1)
     srv = acc.Server(max_batch=8, max_inflight=3, form=acc.FORM_SPLIT_BF16,
                      large_work=large_work)
-    gids = [srv.register(c) for c in pool]
-    graphs = [acc.DeviceGraph(c) for c in pool]
+    opts = acc.default_build_opts(16, reorder_rows=1) if layout != "input" else None
+    gids = [srv.register(c, opts) for c in pool]
+    graphs = [acc.DeviceGraph(c, opts) for c in pool]
     rng = np.random.default_rng(2)
     reqs = []
     for i in range(40):
@@ -88,9 +91,10 @@ def test_budget_queue_and_unknown_graph():
     probe.register(c)
     gbytes = probe.stats().resident_bytes
     probe.close()
-    # a budget that holds the graph but not a second copy
+    # a budget that holds the graph but not a second copy (BF16 form: its
+    # requests always need a converted copy of B in the workspace)
     srv = acc.Server(hbm_budget_bytes=int(gbytes * 1.5), max_queue=2,
-                     max_inflight=1)
+                     max_inflight=1, form=acc.FORM_BF16)
     gid = srv.register(c)
     with pytest.raises(acc.AccSpMMError) as e:
         srv.register(c)
