@@ -27,7 +27,7 @@ import numpy as np
 
 __all__ = [
     "lib", "Status", "AccSpMMError", "FORM_FP32", "FORM_SPLIT_BF16", "FORM_BF16",
-    "BuildOpts", "Tile", "CSR", "read_mtx", "canonicalize_device", "METCFBit", "AdpBME",
+    "BuildOpts", "Tile", "CSR", "read_mtx", "canonicalize_device", "autotune_layout", "METCFBit", "AdpBME",
     "balanceStrategy", "DeviceGraph", "Batch", "Server", "gen_chung_lu", "gen_stencil27",
     "gen_dense", "MetcfInfo",
 ]
@@ -93,7 +93,7 @@ class Tile(C.Structure):
 
     def as_tuple(self):
         return (self.feat_per_lane, self.waves_per_block, self.split_tiles, self.xcd_slice,
-                self.pack_tiles, self.col_panels, self.sweep)
+                self.pack_tiles, self.col_panels, self.sweep, self.b_split)
 
 
 class HostCsr(C.Structure):
@@ -138,6 +138,10 @@ _SIGS = {
     "accspmm_metcf_get_info": (C.c_int, [C.c_void_p, C.POINTER(MetcfInfo)]),
     "accspmm_metcf_export": (C.c_int, [C.c_void_p] + [C.c_void_p] * 7),
     "accspmm_metcf_row_order": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "accspmm_autotune_layout": (C.c_int, [C.POINTER(CsrView), C.c_void_p, C.c_uint32,
+                                          C.c_void_p, C.c_uint32, C.c_uint32, C.c_int,
+                                          C.POINTER(BuildOpts), C.POINTER(C.c_float),
+                                          C.c_void_p]),
     "accspmm_metcf_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "accspmm_metcf_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_void_p]),
     "accspmm_spmm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
@@ -336,6 +340,20 @@ def default_build_opts(row_window: int = 8, **kw) -> BuildOpts:
     for k, v in kw.items():
         setattr(o, k, v)
     return o
+
+
+def autotune_layout(csr: CSR, B, C_out, form: int = FORM_FP32, stream=None):
+    """Fastest of the four layouts (row window 8/16 x input/reordered rows)
+    for (csr, B, form): returns (BuildOpts, ms) (accspmm_autotune_layout)."""
+    rp, ci, va = csr.to_device()
+    view = CsrView(csr.rows, csr.cols, csr.nnz, rp.data_ptr(), ci.data_ptr(), va.data_ptr())
+    o = BuildOpts()
+    ms = C.c_float()
+    _check(lib().accspmm_autotune_layout(C.byref(view), B.data_ptr(), B.stride(0),
+                                         C_out.data_ptr(), C_out.stride(0), B.shape[1], form,
+                                         C.byref(o), C.byref(ms), _stream(stream)),
+           "autotune_layout")
+    return o, ms.value
 
 
 def canonicalize_device(rows: int, cols: int, row_ptr, col_idx, val, stream=None) -> int:

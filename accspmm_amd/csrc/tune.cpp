@@ -31,6 +31,10 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
                            float* C, uint32_t ldc, uint32_t K, int form,
                            const accspmm_tile& tile, hipStream_t s,
                            int phases);
+accspmm_status spmm_fit_tile(const accspmm_metcf* h, const float* B,
+                             uint32_t ldb, float* C, uint32_t ldc, uint32_t K,
+                             int form, const accspmm_tile* tile,
+                             accspmm_tile* out);
 }  // namespace accspmm
 
 using namespace accspmm;
@@ -215,6 +219,68 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
     if (cache_path && *cache_path) save_file(cache_path);
   }
   if (best) *best = bt;
+  if (best_ms) *best_ms = bms;
+  return ACCSPMM_OK;
+}
+
+// Layout search: the row window (8, 16) and the row order (input, by
+// decreasing length) are build options, so each candidate is a build.  Each
+// layout runs its library tile (tuned pick or default) and the median of 5
+// timed launches after 2 warm-up launches decides.
+extern "C" accspmm_status accspmm_autotune_layout(const accspmm_csr_view* csr,
+                                                  const float* B, uint32_t ldb,
+                                                  float* C, uint32_t ldc, uint32_t K,
+                                                  accspmm_form form,
+                                                  accspmm_build_opts* best,
+                                                  float* best_ms, void* stream) {
+  if (!csr || !B || !C || !best || K == 0) return ACCSPMM_ERR_INVALID_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  accspmm_build_opts base;
+  accspmm_build_opts_default(&base);
+  hipEvent_t e0, e1;
+  ACC_HIP(hipEventCreate(&e0));
+  ACC_HIP(hipEventCreate(&e1));
+  float bms = 1e30f;
+  uint32_t brw = 8, bro = 0;
+  accspmm_status err = ACCSPMM_OK;
+  for (uint32_t rw : {8u, 16u}) {
+    for (uint32_t ro : {0u, 1u}) {
+      accspmm_build_opts o = base;
+      o.row_window = rw;
+      o.reorder_rows = ro;
+      o.build_adpbme = 0;  // the SpMM path does not read the AdpBME groups
+      accspmm_metcf* h = nullptr;
+      err = accspmm_metcf_build(csr, &o, &h, stream);
+      if (err != ACCSPMM_OK) break;
+      accspmm_tile t{};
+      err = spmm_fit_tile(h, B, ldb, C, ldc, K, (int)form, nullptr, &t);
+      std::vector<float> ms;
+      for (int i = 0; i < 7 && err == ACCSPMM_OK; ++i) {
+        (void)hipEventRecord(e0, s);
+        err = spmm_launch(h, B, ldb, C, ldc, K, (int)form, t, s, 3);
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float x = 0.f;
+        (void)hipEventElapsedTime(&x, e0, e1);
+        if (i >= 2) ms.push_back(x);
+      }
+      (void)accspmm_metcf_destroy(h);
+      if (err != ACCSPMM_OK) break;
+      std::sort(ms.begin(), ms.end());
+      if (ms[2] < bms) {
+        bms = ms[2];
+        brw = rw;
+        bro = ro;
+      }
+    }
+    if (err != ACCSPMM_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err != ACCSPMM_OK) return err;
+  *best = base;
+  best->row_window = brw;
+  best->reorder_rows = bro;
   if (best_ms) *best_ms = bms;
   return ACCSPMM_OK;
 }

@@ -150,3 +150,27 @@ def test_automatic_choices():
         B = torch.from_numpy(acc.gen_dense(4, csr.cols, 64)).cuda()
         assert torch.equal(g.spmm(B, None, acc.FORM_SPLIT_BF16),
                            ge.spmm(B, None, acc.FORM_SPLIT_BF16))
+
+
+@pytest.mark.parametrize("form", [acc.FORM_SPLIT_BF16, acc.FORM_FP32])
+def test_layout_autotuner(form):
+    # the returned layout is one of the four, and a graph built with it
+    # computes C within the form's tolerance; bad input reports its status
+    torch = _torch()
+    for name, csr in _graphs().items():
+        B = torch.from_numpy(acc.gen_dense(8, csr.cols, 64)).cuda()
+        Cs = torch.empty((csr.rows, 64), device="cuda")
+        opts, ms = acc.autotune_layout(csr, B, Cs, form)
+        assert opts.row_window in (8, 16) and opts.reorder_rows in (0, 1) and ms > 0
+        g = acc.DeviceGraph(csr, opts)
+        Cg = g.spmm(B, None, form).cpu().numpy()
+        Bh = B.cpu().numpy()
+        mx, _ = O.errors(Cg, O.spmm_f64(csr.row_ptr, csr.col_idx, csr.data, Bh),
+                         O.spmm_abs_f64(csr.row_ptr, csr.col_idx, csr.data, Bh))
+        assert mx <= TOL[form], (name, mx)
+    bad = acc.CSR(3, 3, np.array([0, 2, 3, 3], np.uint32), np.array([2, 1, 0], np.uint32),
+                  np.ones(3, np.float32))                      # row 0 not sorted
+    B = torch.zeros((3, 16), device="cuda")
+    with pytest.raises(acc.AccSpMMError) as e:
+        acc.autotune_layout(bad, B, torch.empty((3, 16), device="cuda"), form)
+    assert e.value.status == acc.Status.ERR_NOT_CANONICAL
