@@ -86,6 +86,30 @@ def time_form(g, B, Cd, form, tile, steps, warmup):
     }
 
 
+def format_diagnostics(g, K: int, form: int) -> dict:
+    """Two ungraded diagnostics beside the roofline: the METCF format's bytes
+    4(W+1) + 4(T+1) + 8T*RW/8 + 32T + s_A*nnz, and the window-gather bytes
+    sum_w uniq_w * K * s_B (uniq_w = the window's condensed columns, i.e.
+    the non-padding sparseA2B entries), the B traffic L2/MALL must serve."""
+    import ctypes
+
+    import numpy as np
+
+    import accspmm_amd as acc
+    inf = g.info
+    s = 2 if form == acc.FORM_BF16 else 4
+    a2b = np.empty(inf.num_tiles * 8, np.uint32)
+    null = ctypes.c_void_p()
+    acc._check(acc.lib().accspmm_metcf_export(g.handle, null, null, null,
+                                              a2b.ctypes.data, null, null, null),
+               "metcf_export")
+    uniq = int(np.count_nonzero(a2b != 0xFFFFFFFF))
+    T, W = inf.num_tiles, inf.num_windows
+    return {"format_bytes": 4 * (W + 1) + 4 * (T + 1) + 8 * T * (inf.row_window // 8) +
+                            32 * T + s * inf.nnz,
+            "gather_bytes": uniq * K * s, "gathered_tile_columns": uniq}
+
+
 def copy_bandwidth() -> float:
     """Device-to-device copy rate of a 1 GiB buffer (read + write bytes per
     second, GB/s), the measured counterpart of the 8 TB/s HBM peak."""
@@ -503,6 +527,9 @@ def main():
             torch.cuda.synchronize()
             outputs[fname] = Cd.cpu().numpy()
 
+    diag = format_diagnostics(g, K, name_to_form[forms[0]])
+    log(f"bench: format {diag['format_bytes'] / 1e9:.3f} GB, window gathers "
+        f"{diag['gather_bytes'] / 1e9:.2f} GB per launch")
     cpu = None
     if not args.no_check:
         log("bench: FP64 oracle check and CPU baseline ...")
@@ -587,6 +614,7 @@ def main():
                      "stream_copy_gbs": round(copy_gbs, 1),
                      "kernel_ms": round(hr["kernel_ms_mean"], 4),
                      "alg_bytes_per_launch": ab},
+        "diagnostics": diag,
         "cpu_baseline": cpu,
         "batching": batching,
         "serving": serving,
