@@ -336,7 +336,10 @@ accspmm_status BatchLauncher::launch(hipStream_t s) const {
 using namespace accspmm;
 
 struct accspmm_batch {
-  BatchLauncher L;
+  // one persistent kernel serves one row-window size: the problems on
+  // 8-row graphs go to L[0], those on 16-row graphs to L[1]
+  BatchLauncher L[2];
+  int form = 0;
   std::vector<accspmm_problem> probs;
   std::vector<ProblemPlan> plans;
   hipStream_t cap_stream = nullptr;
@@ -346,7 +349,28 @@ struct accspmm_batch {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
-    if (L.upload_event) (void)hipEventDestroy(L.upload_event);
+    for (auto& l : L)
+      if (l.upload_event) (void)hipEventDestroy(l.upload_event);
+  }
+  // uploads each row-window group's descriptors (problems in input order)
+  accspmm_status prepare_groups(hipStream_t s) {
+    for (int gi = 0; gi < 2; ++gi) {
+      std::vector<accspmm_problem> P;
+      std::vector<ProblemPlan> Q;
+      const uint32_t rw = gi ? 16u : 8u;
+      for (size_t i = 0; i < probs.size(); ++i)
+        if (probs[i].graph->rw == rw) {
+          P.push_back(probs[i]);
+          Q.push_back(plans[i]);
+        }
+      L[gi].form = form;
+      ACC_TRY(L[gi].prepare(P.data(), (uint32_t)P.size(), Q.data(), s));
+    }
+    return ACCSPMM_OK;
+  }
+  accspmm_status launch(hipStream_t s) const {
+    for (const auto& l : L) ACC_TRY(l.launch(s));
+    return ACCSPMM_OK;
   }
 };
 
@@ -359,13 +383,13 @@ extern "C" accspmm_status accspmm_batch_create(const accspmm_problem* problems,
   hipStream_t s = (hipStream_t)stream;
   auto* b = new (std::nothrow) accspmm_batch();
   if (!b) return ACCSPMM_ERR_OUT_OF_MEMORY;
-  b->L.form = (int)form;
+  b->form = (int)form;
   b->probs.assign(problems, problems + count);
   b->plans.resize(count);
   accspmm_status st = ACCSPMM_OK;
   for (uint32_t i = 0; i < count && st == ACCSPMM_OK; ++i)
     st = problem_plan(problems[i], (int)form, s, &b->plans[i]);
-  if (st == ACCSPMM_OK) st = b->L.prepare(problems, count, b->plans.data(), s);
+  if (st == ACCSPMM_OK) st = b->prepare_groups(s);
   if (st == ACCSPMM_OK && hipStreamSynchronize(s) != hipSuccess) st = ACCSPMM_ERR_HIP;
   if (st != ACCSPMM_OK) {
     (void)hipStreamSynchronize(s);
@@ -386,7 +410,7 @@ extern "C" accspmm_status accspmm_batch_rebind(accspmm_batch* b,
   for (uint32_t i = 0; i < count; ++i) {
     if (problems[i].graph != b->probs[i].graph || problems[i].K != b->probs[i].K)
       return ACCSPMM_ERR_SHAPE;
-    ACC_TRY(problem_plan(problems[i], b->L.form, s, &plans[i]));
+    ACC_TRY(problem_plan(problems[i], b->form, s, &plans[i]));
     // the new operands must keep the planned tile (captured graphs replay it)
     if (plans[i].nb != b->plans[i].nb) return ACCSPMM_ERR_UNSUPPORTED;
   }
@@ -394,14 +418,14 @@ extern "C" accspmm_status accspmm_batch_rebind(accspmm_batch* b,
   b->probs.assign(problems, problems + count);
   b->plans = plans;
   // same sizes => no buffer is reallocated, so a captured graph stays valid
-  ACC_TRY(b->L.prepare(problems, count, b->plans.data(), s));
+  ACC_TRY(b->prepare_groups(s));
   ACC_HIP(hipStreamSynchronize(s));
   return ACCSPMM_OK;
 }
 
 extern "C" accspmm_status accspmm_batch_run(accspmm_batch* b, void* stream) {
   if (!b) return ACCSPMM_ERR_INVALID_ARG;
-  return b->L.launch((hipStream_t)stream);
+  return b->launch((hipStream_t)stream);
 }
 
 extern "C" accspmm_status accspmm_batch_run_graph(accspmm_batch* b, void* stream) {
@@ -410,7 +434,7 @@ extern "C" accspmm_status accspmm_batch_run_graph(accspmm_batch* b, void* stream
     if (!b->cap_stream)
       ACC_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
     ACC_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-    accspmm_status st = b->L.launch(b->cap_stream);
+    accspmm_status st = b->launch(b->cap_stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
     if (st != ACCSPMM_OK) {
@@ -435,6 +459,6 @@ extern "C" accspmm_status accspmm_batch_destroy(accspmm_batch* b) {
 extern "C" accspmm_status accspmm_batch_num_tasks(const accspmm_batch* b,
                                                   uint64_t* tasks) {
   if (!b || !tasks) return ACCSPMM_ERR_INVALID_ARG;
-  *tasks = b->L.total_tasks;
+  *tasks = b->L[0].total_tasks + b->L[1].total_tasks;
   return ACCSPMM_OK;
 }

@@ -265,7 +265,9 @@ typedef struct accspmm_batch accspmm_batch;
 
 /* Plans a batch (work-unit queue over all problems, one form); allocates its
  * device task list and workspaces.  `problems` may be re-bound later with
- * accspmm_batch_rebind as long as graphs and K stay the same. */
+ * accspmm_batch_rebind as long as graphs and K stay the same.  Graphs of
+ * both row-window sizes may be mixed: each size gets its own persistent
+ * launch, enqueued one after the other. */
 accspmm_status accspmm_batch_create(const accspmm_problem* problems,
                                     uint32_t count, accspmm_form form,
                                     accspmm_batch** out, void* stream);

@@ -29,12 +29,17 @@ def _pool(n, seed, nmin=500, nmax=6000):
 
 @pytest.mark.parametrize("form", [acc.FORM_FP32, acc.FORM_SPLIT_BF16, acc.FORM_BF16],
                          ids=acc.FORM_NAMES.get)
-def test_batch_equals_alone(form):
+@pytest.mark.parametrize("layouts", ["rw8", "mixed"])
+def test_batch_equals_alone(form, layouts):
+    # mixed: every other graph has 16-row windows and reordered rows, so the
+    # batch runs one persistent kernel per row-window size
     import torch
     pool = _pool(12, form)
     graphs, Bs, Cs, alone, hostB = [], [], [], [], []
-    for csr, K, bseed in pool:
-        g = acc.DeviceGraph(csr)
+    for i, (csr, K, bseed) in enumerate(pool):
+        opts = acc.default_build_opts(16, reorder_rows=1) if layouts == "mixed" and i % 2 \
+            else None
+        g = acc.DeviceGraph(csr, opts)
         Bh = acc.gen_dense(bseed, csr.cols, K)
         B = torch.from_numpy(Bh).cuda()
         graphs.append(g)
@@ -89,14 +94,25 @@ def test_batch_graph_replay_and_rebind():
     batch.close()
 
 
-def test_batch_rejects_mixed_row_windows():
+def test_batch_mixed_row_windows_and_replay():
+    # 8- and 16-row graphs in one batch: one persistent kernel per row-window
+    # size, eager and hipGraph replays equal to the standalone calls
     import torch
     a = acc.gen_chung_lu(1000, 8.0, 2.2, 1, 0)
-    g8 = acc.DeviceGraph(a)
-    g16 = acc.DeviceGraph(a, acc.default_build_opts(16))
-    B = torch.zeros((1000, 16), device="cuda")
-    C1 = torch.zeros((1000, 16), device="cuda")
-    C2 = torch.zeros((1000, 16), device="cuda")
-    with pytest.raises(acc.AccSpMMError) as e:
-        acc.Batch([(g8, B, C1), (g16, B, C2)], acc.FORM_FP32)
-    assert e.value.status == acc.Status.ERR_UNSUPPORTED
+    b = acc.gen_chung_lu(3000, 20.0, 2.1, 2, 0)
+    gs = [acc.DeviceGraph(a), acc.DeviceGraph(a, acc.default_build_opts(16)),
+          acc.DeviceGraph(b, acc.default_build_opts(16, reorder_rows=1)), acc.DeviceGraph(b)]
+    Bs = [torch.from_numpy(acc.gen_dense(i, g.info.cols, K)).cuda()
+          for i, (g, K) in enumerate(zip(gs, (16, 64, 128, 32)))]
+    Cs = [torch.full((g.info.rows, B.shape[1]), float("nan"), device="cuda")
+          for g, B in zip(gs, Bs)]
+    for form in (acc.FORM_FP32, acc.FORM_SPLIT_BF16):
+        batch = acc.Batch(list(zip(gs, Bs, Cs)), form)
+        for graph in (False, True, True):
+            for c in Cs:
+                c.fill_(float("nan"))
+            batch.run(graph=graph)
+            torch.cuda.synchronize()
+            for g, B, Cm in zip(gs, Bs, Cs):
+                assert torch.equal(Cm, g.spmm(B, None, form)), (form, graph)
+        batch.close()

@@ -22,18 +22,21 @@ def _pool(n, seed):
 
 
 @pytest.mark.parametrize("large_work,layout", [(0, "input"), (3_000_000, "input"),
-                                               (0, "rw16_reordered")])
+                                               (0, "rw16_reordered"), (0, "mixed")])
 def test_mix_equals_alone(large_work, layout):
     # large_work = 3e6 makes the requests of the bigger graphs (nnz * K above
     # it) run in launches of their own beside batches of the small ones;
-    # rw16_reordered registers graphs with 16-row windows and reordered rows
+    # rw16_reordered registers graphs with 16-row windows and reordered rows;
+    # mixed alternates the two layouts (a launch serves one row window)
     import torch
     pool = _pool(8, 1)
     srv = acc.Server(max_batch=8, max_inflight=3, form=acc.FORM_SPLIT_BF16,
                      large_work=large_work)
-    opts = acc.default_build_opts(16, reorder_rows=1) if layout != "input" else None
-    gids = [srv.register(c, opts) for c in pool]
-    graphs = [acc.DeviceGraph(c, opts) for c in pool]
+    rw16 = acc.default_build_opts(16, reorder_rows=1)
+    opts = [None if layout == "input" or (layout == "mixed" and i % 2) else rw16
+            for i in range(len(pool))]
+    gids = [srv.register(c, o) for c, o in zip(pool, opts)]
+    graphs = [acc.DeviceGraph(c, o) for c, o in zip(pool, opts)]
     rng = np.random.default_rng(2)
     reqs = []
     for i in range(40):
