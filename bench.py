@@ -612,6 +612,12 @@ def main():
                      "traffic": traffic,
                      "kernel": "spmm_kernel+k_reduce_split",
                      "stream_copy_gbs": round(copy_gbs, 1),
+                     # the profiled HBM/MALL bytes per launch over this run's
+                     # kernel time: the rate the gathers actually move
+                     "traffic_gbs": (round(traffic / (hr["kernel_ms_mean"] * 1e-3) / 1e9, 1)
+                                     if traffic else None),
+                     "traffic_frac": (round(traffic / (hr["kernel_ms_mean"] * 1e-3) / 1e9 /
+                                            HBM_PEAK_GBS, 4) if traffic else None),
                      "kernel_ms": round(hr["kernel_ms_mean"], 4),
                      "alg_bytes_per_launch": ab},
         "diagnostics": diag,
