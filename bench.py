@@ -133,8 +133,10 @@ def copy_bandwidth() -> float:
     return 2.0 * 4 * n / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline(csr, B, K, min_seconds=10.0, max_runs=10):
-    """The oracle's OpenMP FP32 CSR SpMM on the host cores (a baseline only)."""
+def cpu_baseline(csr, B, K, min_seconds=10.0, max_runs=60):
+    """The oracle's OpenMP FP32 CSR SpMM on the host cores (a baseline only):
+    one untimed run, then timed runs until at least min_seconds of CPU work
+    (at least 3, at most max_runs); the median run is reported."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     out, used = O.spmm_f32_omp(csr.row_ptr, csr.col_idx, csr.data, B, threads=threads)
