@@ -208,3 +208,46 @@ def test_threads_and_sizes_agree(tmp_path, monkeypatch):
     for th in ("2", "3", "16", "64"):
         got = _read(p, monkeypatch, th)
         _same(got, ref.row_ptr, ref.col_idx, ref.data)
+
+
+def _outcome(path, monkeypatch, threads):
+    monkeypatch.setenv("ACCSPMM_HOST_THREADS", threads)
+    try:
+        c = acc.read_mtx(path)
+        return ("ok", c.rows, c.cols, c.row_ptr.tobytes(), c.col_idx.tobytes(),
+                c.data.tobytes())
+    except acc.AccSpMMError as e:
+        return ("err", e.status, e.mtx_code)
+
+
+def test_random_mutations_match_sequential(tmp_path, monkeypatch):
+    # random single-character edits anywhere in the entry section: whatever
+    # the edit does (a new token, a split or merged line, a comment, a bad
+    # number), the parallel reader returns what the sequential tokenizer does
+    rng = np.random.default_rng(2024)
+    base = _big_general(52000)
+    assert len(base) > MIN_BYTES
+    head = base.index("\n", base.index("\n") + 1) + 1
+    alphabet = [" ", "\n", "\t", "%", "-", "+", "e", "E", ".", "0", "7", "x", "\r", "\n\n"]
+    agree = fallback = 0
+    for case in range(24):
+        text = list(base)
+        for _ in range(int(rng.integers(1, 4))):
+            pos = int(rng.integers(head, len(text)))
+            op = int(rng.integers(3))
+            ch = alphabet[int(rng.integers(len(alphabet)))]
+            if op == 0:
+                text.insert(pos, ch)
+            elif op == 1:
+                text[pos] = ch
+            else:
+                del text[pos]
+        p = tmp_path / f"m{case}.mtx"
+        p.write_text("".join(text))
+        seq = _outcome(p, monkeypatch, "1")
+        par = _outcome(p, monkeypatch, PAR)
+        assert par == seq, (case, seq[:3], par[:3])
+        used = acc.lib().accspmm_read_mtx_threads_used()
+        fallback += used == 1
+        agree += 1
+    assert agree == 24 and fallback < 24   # some edits keep the parallel path
