@@ -17,6 +17,11 @@
 //                                 |A||B|) and the Frobenius error, exit 6
 //                                 above the form's tolerance (the reference
 //                                 keeps B = 1 and never checks C)
+//   --layout reference|auto       the graph the launches use: the reference's
+//                                 METCF (default) or the library's automatic
+//                                 layout (16-row windows and rows reordered by
+//                                 length when rows average >= 128 nonzeros);
+//                                 C is in input row order either way
 // The CSR is canonicalised on load (rows sorted by column, duplicates
 // summed), which the METCF tile layout requires.
 #include <hip/hip_runtime.h>
@@ -49,7 +54,8 @@ static std::string match_filename(const std::string& s) {
 int main(int argc, char** argv) {
   if (argc < 3) {
     std::printf("Usage: ./mma <matrix.mtx> <feature_dim> [--form fp32|split_bf16|bf16]"
-                " [--random-b SEED] [--warmup N] [--exe N] [--check]\n");
+                " [--random-b SEED] [--warmup N] [--exe N] [--check]"
+                " [--layout reference|auto]\n");
     return ERROR_ARGS;
   }
   const char* filename = argv[1];
@@ -61,7 +67,7 @@ int main(int argc, char** argv) {
   accspmm_form form = ACCSPMM_FORM_FP32;
   long long bseed = -1;
   int warmup = 10, exe = 10;
-  bool check = false;
+  bool check = false, auto_layout = false;
   for (int i = 3; i < argc; ++i) {
     std::string a = argv[i];
     if (a == "--form" && i + 1 < argc) {
@@ -78,6 +84,11 @@ int main(int argc, char** argv) {
       exe = std::max(1, std::atoi(argv[++i]));
     } else if (a == "--check") {
       check = true;
+    } else if (a == "--layout" && i + 1 < argc) {
+      std::string l = argv[++i];
+      if (l == "reference") auto_layout = false;
+      else if (l == "auto") auto_layout = true;
+      else { std::printf("unknown layout %s\n", l.c_str()); return ERROR_ARGS; }
     } else {
       std::printf("unknown option %s\n", a.c_str());
       return ERROR_ARGS;
@@ -112,14 +123,33 @@ int main(int argc, char** argv) {
   HIP_OK(hipMemcpy(dB, hB.data(), nB * sizeof(float), hipMemcpyHostToDevice));
   HIP_OK(hipMemset(dC, 0, nC * sizeof(float)));
 
+  // the launches run on the reference's METCF, or on a second build in the
+  // library's automatic layout
+  accspmm_metcf* graph = metcf_bit.handle;
+  accspmm_metcf* own = nullptr;
+  if (auto_layout) {
+    accspmm_build_opts o;
+    accspmm_build_opts_default(&o);
+    o.row_window = 0;
+    o.reorder_rows = 2;
+    o.build_adpbme = 0;
+    const accspmm_csr_view v = csr.view();
+    const accspmm_status bst = accspmm_metcf_build_host(&v, &o, &own);
+    if (bst != ACCSPMM_OK) {
+      std::fprintf(stderr, "build: %s\n", accspmm_status_string(bst));
+      return 5;
+    }
+    graph = own;
+  }
   accspmm_tile tile;
-  accspmm_tile_select(metcf_bit.handle, feature_dim, form, &tile);
-  if (!load_balance) tile.split_tiles = 0;  // heavy windows are not split
+  accspmm_tile_select(graph, feature_dim, form, &tile);
+  // the reference's unbalanced path does not split heavy windows
+  if (!load_balance && !auto_layout) tile.split_tiles = 0;
   std::cout << (load_balance ? "doing AdpBalance pipelining ..." : "doing pipelining mma ...")
             << std::endl;
 
   auto run = [&]() {
-    return accspmm_spmm(metcf_bit.handle, dB, feature_dim, dC, feature_dim, feature_dim,
+    return accspmm_spmm(graph, dB, feature_dim, dC, feature_dim, feature_dim,
                         form, &tile, nullptr);
   };
   for (int i = 0; i < warmup; ++i) {
@@ -186,5 +216,6 @@ int main(int argc, char** argv) {
   }
   (void)hipFree(dB);
   (void)hipFree(dC);
+  if (own) (void)accspmm_metcf_destroy(own);
   return rc;
 }

@@ -91,3 +91,26 @@ def test_mma_driver_appends_result_row(tmp_path, form):
     assert len(rows_csv) == 2
     name, k, ms, gf = rows_csv[0].split(",")
     assert name == "g.mtx" and k == "32" and float(ms) > 0 and float(gf) > 0
+
+
+@pytest.mark.parametrize("form", ["fp32", "split_bf16"])
+def test_mma_driver_auto_layout(tmp_path, form):
+    # --layout auto launches on the library's automatic layout (here 16-row
+    # windows and reordered rows: rows average >= 128 nonzeros); the driver's
+    # FP64 check of C in input row order passes
+    rng = np.random.default_rng(33)
+    rows, cols = 1200, 1500
+    from helpers import random_csr
+    rp, ci, va = random_csr(rng, rows, cols, 0.12, empty_rows=0.05, heavy_rows=2)
+    csr = acc.CSR(rows, cols, rp, ci, va)
+    assert csr.nnz >= 128 * rows
+    mtx = tmp_path / "long.mtx"
+    csr.write_mtx(mtx)
+    mma = str(REPO / "accspmm_amd" / "bin" / "mma")
+    r = subprocess.run([mma, str(mtx), "64", "--form", form, "--layout", "auto",
+                        "--warmup", "2", "--exe", "3", "--check"],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("check:")]
+    assert line and line[0].endswith("ok"), r.stdout
+    assert (tmp_path / "result.csv").read_text().startswith("long.mtx,64,")

@@ -122,6 +122,9 @@ def test_mma_driver_exit_codes(tmp_path):
     r = subprocess.run([mma, str(tmp_path / "none.mtx"), "32"], capture_output=True,
                        text=True, cwd=tmp_path)
     assert r.returncode == 3
+    r = subprocess.run([mma, str(tmp_path / "none.mtx"), "32", "--layout", "fast"],
+                       capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 4 and "unknown layout" in r.stdout
     bad = _write(tmp_path, "bad.mtx", K["mtx"]["bad_banners"][0])
     r = subprocess.run([mma, str(bad), "32"], capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode == 3 and "banner" in r.stdout
