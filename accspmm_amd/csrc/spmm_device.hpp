@@ -624,6 +624,22 @@ __device__ inline void reduce_split_window(const uint32_t* __restrict__ sinfo,
                                            const uint32_t* __restrict__ rmap) {
   const uint32_t w = sinfo[3ull * sw], first = sinfo[3ull * sw + 1],
                  count = sinfo[3ull * sw + 2];
+  // four features per thread when C rows allow 16-byte stores (the
+  // partials are: kcp is a multiple of 16); the same per-element order
+  if ((K & 3u) == 0 && (ldc & 3u) == 0 && (reinterpret_cast<uintptr_t>(C) & 15u) == 0) {
+    const uint32_t K4 = K >> 2;
+    for (uint32_t i = threadIdx.x; i < RW * K4; i += blockDim.x) {
+      const uint32_t r = i / K4, k = (i - r * K4) * 4u;
+      const uint64_t row = (uint64_t)w * RW + r;
+      if (row >= rows) continue;
+      const float* src = partials + ((uint64_t)first * RW + r) * kcp + k;
+      f4 v = f4{0.f, 0.f, 0.f, 0.f};
+      for (uint32_t p = 0; p < count; ++p)
+        v += *reinterpret_cast<const f4*>(src + (uint64_t)p * RW * kcp);
+      *reinterpret_cast<f4*>(C + out_row(rmap, (uint32_t)row) * ldc + k) = v;
+    }
+    return;
+  }
   for (uint32_t i = threadIdx.x; i < RW * K; i += blockDim.x) {
     const uint32_t r = i / K, k = i - r * K;
     const uint64_t row = (uint64_t)w * RW + r;
