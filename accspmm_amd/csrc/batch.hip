@@ -27,6 +27,9 @@
 #include "spmm_device.hpp"
 
 namespace accspmm {
+accspmm_status launch_single(const dev::SpmmArgs& g, int form, uint32_t rw, uint32_t nb,
+                             uint32_t wpb, bool packed, uint32_t num_units, uint32_t n_kc,
+                             hipStream_t s);
 accspmm_status spmm_fit_tile(const accspmm_metcf* h, const float* B,
                              uint32_t ldb, float* C, uint32_t ldc, uint32_t K,
                              int form, const accspmm_tile* tile,
@@ -146,6 +149,9 @@ accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
     if (form != 0) ACC_TRY(ensure_vsplit(h, s));
   }
   out->tasks = (uint64_t)out->sched->num_units * out->n_kc;
+  out->b_inreg = form == 1 && t.b_split == 2;
+  out->wpb = t.waves_per_block ? t.waves_per_block : 4;
+  out->xcd = t.xcd_slice;
   out->bconv_bytes =
       form == 0 ? 0
                 : align256(((uint64_t)h->cols + 1) * out->kc_padded * (form == 1 ? 4 : 2));
@@ -176,6 +182,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
                                       const ProblemPlan* plans, hipStream_t s) {
   nprob = 0;
   total_tasks = 0;
+  single = false;
   if (n == 0) return ACCSPMM_OK;
   // one persistent kernel serves one row-window size
   const uint32_t rw0 = P[0].graph->rw;
@@ -250,7 +257,20 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     g.pad_tile = h->T;
     g.partials = reinterpret_cast<float*>(A + bc);
     bc += q.partial_bytes;
-    if (form == 0) {
+    single = n == 1 && q.b_inreg;
+    if (single) {
+      // the single-graph kernel with the batch's own workspace
+      g.xcd_slice = q.xcd == 2 ? 2u : (q.xcd == 1 && q.n_kc >= 2 && (8 % q.n_kc) == 0) ? 1u : 0u;
+      single_g = g;
+      single_g.B = p.B;
+      single_g.ldb = p.ldb;
+      single_nb = q.nb;
+      single_wpb = q.wpb;
+      single_units = q.sched->num_units;
+      single_nkc = q.n_kc;
+      single_packed = q.sched->packed;
+    }
+    if (form == 0 || single) {
       g.B = p.B;
       g.ldb = p.ldb;
     } else {
@@ -294,6 +314,21 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
 
 accspmm_status BatchLauncher::launch(hipStream_t s) const {
   if (nprob == 0) return ACCSPMM_OK;
+  if (single) {
+    ACC_TRY(launch_single(single_g, kSPLITF, rw, single_nb, single_wpb, single_packed,
+                          single_units, single_nkc, s));
+    if (total_split) {
+      const dim3 blk(256);
+      if (rw == 8)
+        hipLaunchKernelGGL(k_batch_reduce<8>, dim3(total_split), blk, 0, s, descs,
+                           split_prefix, nprob);
+      else
+        hipLaunchKernelGGL(k_batch_reduce<16>, dim3(total_split), blk, 0, s, descs,
+                           split_prefix, nprob);
+      ACC_HIP(hipGetLastError());
+    }
+    return ACCSPMM_OK;
+  }
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, head.as<uint32_t>(),
                      (uint64_t)1, 0u);
   ACC_HIP(hipGetLastError());

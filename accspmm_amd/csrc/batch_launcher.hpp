@@ -2,11 +2,9 @@
 // shared by the batch API (batch.hip) and the serving loop (server.hip).
 #pragma once
 #include "internal.hpp"
+#include "spmm_device.hpp"
 
 namespace accspmm {
-namespace dev {
-struct SpmmArgs;
-}
 
 // What one problem needs inside a batch: its tile (the one its standalone
 // call would use), its work-unit schedule and its workspace sizes.
@@ -15,6 +13,10 @@ struct ProblemPlan {
   const Schedule* sched = nullptr;
   uint64_t tasks = 0;          // units x feature chunks
   uint64_t bconv_bytes = 0;    // converted B (BF16 forms)
+  // the single-graph call's choices: B split in registers (split form),
+  // waves per block and XCD mapping
+  bool b_inreg = false;
+  uint32_t wpb = 4, xcd = 0;
   uint64_t partial_bytes = 0;  // split-window partial sums
   uint64_t workspace_bytes() const;
 };
@@ -42,6 +44,12 @@ struct BatchLauncher {
   uint32_t* split_prefix = nullptr;  // split windows before each problem
   uint64_t total_tasks = 0;
   uint32_t nprob = 0, num_conv = 0, conv_blocks = 0, grid = 0;
+  // a batch of one split-form problem whose B is split in registers runs
+  // the single-graph kernel (its persistent-kernel form spills registers)
+  bool single = false;
+  dev::SpmmArgs single_g{};
+  uint32_t single_nb = 1, single_wpb = 4, single_units = 0, single_nkc = 1;
+  bool single_packed = false;
   uint32_t total_split = 0;          // split windows of all problems
   BatchLauncher() = default;
   BatchLauncher(const BatchLauncher&) = delete;

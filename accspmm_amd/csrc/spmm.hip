@@ -458,6 +458,28 @@ accspmm_status ensure_vsplit(const accspmm_metcf* h, hipStream_t s) {
   return ACCSPMM_OK;
 }
 
+// The single-graph SpMM kernel over g's work units: blocks of wpb waves,
+// mapped to XCDs as g.xcd_slice says (multiples of 8 blocks for modes 1, 2).
+accspmm_status launch_single(const SpmmArgs& g, int form, uint32_t rw, uint32_t nb,
+                             uint32_t wpb, bool packed, uint32_t num_units, uint32_t n_kc,
+                             hipStream_t s) {
+  uint64_t blocks;
+  if (g.xcd_slice == 1) {
+    const uint64_t ub = (num_units + wpb - 1) / wpb;
+    blocks = (ub * n_kc + 7) / 8 * 8;
+  } else if (g.xcd_slice == 2) {
+    blocks = ((uint64_t)num_units * n_kc + wpb - 1) / wpb;
+    blocks = (blocks + 7) / 8 * 8;
+  } else {
+    blocks = ((uint64_t)num_units * n_kc + wpb - 1) / wpb;
+  }
+  if (blocks == 0) return ACCSPMM_OK;
+  if (blocks > 0x7FFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
+  ACC_TRY(launch_form(form, rw, nb, g, (uint32_t)blocks, wpb, packed, s));
+  ACC_HIP(hipGetLastError());
+  return ACCSPMM_OK;
+}
+
 // sweep.hip: the resident-window sweep over the same pieces
 accspmm_status sweep_launch(const accspmm_metcf* h, const dev::SpmmArgs& g,
                             const accspmm_tile& tile, uint32_t nb, int form,
@@ -544,20 +566,7 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   if (!(phases & 2)) return ACCSPMM_OK;
   if (tile.sweep) return sweep_launch(h, g, tile, nb, kform, sch, s);
 
-  uint64_t blocks;
-  if (g.xcd_slice == 1) {
-    const uint64_t ub = (sch->num_units + wpb - 1) / wpb;
-    blocks = (ub * n_kc + 7) / 8 * 8;
-  } else if (g.xcd_slice == 2) {
-    blocks = ((uint64_t)sch->num_units * n_kc + wpb - 1) / wpb;
-    blocks = (blocks + 7) / 8 * 8;
-  } else {
-    blocks = ((uint64_t)sch->num_units * n_kc + wpb - 1) / wpb;
-  }
-  if (blocks == 0) return ACCSPMM_OK;
-  if (blocks > 0x7FFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
-  ACC_TRY(launch_form(kform, h->rw, nb, g, (uint32_t)blocks, wpb, sch->packed, s));
-  ACC_HIP(hipGetLastError());
+  ACC_TRY(launch_single(g, kform, h->rw, nb, wpb, sch->packed, sch->num_units, n_kc, s));
   if (sch->num_split_windows) {
     if (h->rw == 8)
       hipLaunchKernelGGL(k_reduce_split<8>, dim3(sch->num_split_windows), dim3(256), 0, s,

@@ -116,3 +116,27 @@ def test_batch_mixed_row_windows_and_replay():
             for g, B, Cm in zip(gs, Bs, Cs):
                 assert torch.equal(Cm, g.spmm(B, None, form)), (form, graph)
         batch.close()
+
+
+def test_single_problem_batches():
+    # a batch of one split-form problem whose B is split in registers runs
+    # the single-graph kernel with the batch's workspace; eager and replayed
+    # results equal the standalone call, with split windows and without
+    import torch
+    from helpers import random_csr
+    rng = np.random.default_rng(44)
+    short = acc.gen_chung_lu(20000, 6.0, 2.1, 3, 0)             # few gathers per B row
+    rp, ci, va = random_csr(rng, 700, 900, 0.05, heavy_rows=4)   # heavy rows: split windows
+    for csr in (short, acc.CSR(700, 900, rp, ci, va)):
+        g = acc.DeviceGraph(csr)
+        for K in (16, 100):
+            B = torch.from_numpy(acc.gen_dense(K, csr.cols, K)).cuda()
+            Cm = torch.full((csr.rows, K), float("nan"), device="cuda")
+            alone = g.spmm(B, None, acc.FORM_SPLIT_BF16).clone()
+            batch = acc.Batch([(g, B, Cm)], acc.FORM_SPLIT_BF16)
+            for graph in (False, True, True):
+                Cm.fill_(float("nan"))
+                batch.run(graph=graph)
+                torch.cuda.synchronize()
+                assert torch.equal(Cm, alone), (csr.rows, K, graph)
+            batch.close()
