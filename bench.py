@@ -167,6 +167,11 @@ def check_errors(csr, B, results):
         mx, fr = O.errors(Cm, C64, ab)
         plain = float(np.max(np.abs(Cm[big] - C64[big]) / np.abs(C64[big]))) if big.any() else 0.0
         errs[name] = (mx, fr, plain)
+    # the reference's precision class for context: A and B rounded to TF32
+    # (round to nearest, ties away) as its kernels do, accumulation exact
+    Ct = O.spmm_tf32_f64(csr.row_ptr, csr.col_idx, csr.data, B).astype(np.float32)
+    mx, fr = O.errors(Ct, C64, ab)
+    errs["_reference_tf32"] = (mx, fr, None)
     return errs
 
 
@@ -533,9 +538,15 @@ def main():
     log(f"bench: format {diag['format_bytes'] / 1e9:.3f} GB, window gathers "
         f"{diag['gather_bytes'] / 1e9:.2f} GB per launch")
     cpu = None
+    ref_tf32 = None
     if not args.no_check:
         log("bench: FP64 oracle check and CPU baseline ...")
         errs = check_errors(csr, Bh, outputs)
+        ref_mx, ref_fr, _ = errs.pop("_reference_tf32")
+        ref_tf32 = {"max_rel": ref_mx, "fro_rel": ref_fr,
+                    "what": "A and B rounded to TF32 (round to nearest, ties away) as the "
+                            "reference's kernels do, FP64 accumulation"}
+        log(f"bench: reference TF32 input rounding: max_rel={ref_mx:.3e} fro_rel={ref_fr:.3e}")
         for fname, (mx, fr, plain) in errs.items():
             per_form[fname]["max_rel"] = mx
             per_form[fname]["fro_rel"] = fr
@@ -623,6 +634,7 @@ def main():
                      "kernel_ms": round(hr["kernel_ms_mean"], 4),
                      "alg_bytes_per_launch": ab},
         "diagnostics": diag,
+        "reference_tf32_precision": ref_tf32,
         "cpu_baseline": cpu,
         "batching": batching,
         "serving": serving,

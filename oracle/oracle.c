@@ -362,6 +362,32 @@ void orc_spmm_f64(uint32_t rows, const uint32_t* row_ptr,
   }
 }
 
+float orc_tf32_rna(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  if ((u & 0x7F800000u) == 0x7F800000u) return x;  /* infinity, NaN */
+  /* 13 mantissa bits are dropped: add half of the lowest kept bit to the
+   * magnitude (sign-magnitude, so ties go away from zero), then truncate */
+  u = (u + 0x1000u) & 0xFFFFE000u;
+  memcpy(&x, &u, 4);
+  return x;
+}
+
+void orc_spmm_tf32_f64(uint32_t rows, const uint32_t* row_ptr,
+                       const uint32_t* col_idx, const float* val, const float* B,
+                       uint32_t ldb, uint32_t K, double* C) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < (int64_t)rows; ++i) {
+    double* out = C + (size_t)i * K;
+    for (uint32_t k = 0; k < K; ++k) out[k] = 0.0;
+    for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+      const double a = orc_tf32_rna(val[e]);
+      const float* b = B + (size_t)col_idx[e] * ldb;
+      for (uint32_t k = 0; k < K; ++k) out[k] += a * (double)orc_tf32_rna(b[k]);
+    }
+  }
+}
+
 void orc_spmm_abs_f64(uint32_t rows, const uint32_t* row_ptr,
                       const uint32_t* col_idx, const float* val,
                       const float* B, uint32_t ldb, uint32_t K, double* C) {

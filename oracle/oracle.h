@@ -80,6 +80,18 @@ uint32_t orc_metcf_decode(const orc_metcf* m, uint32_t* r, uint32_t* c,
 void orc_spmm_f64(uint32_t rows, const uint32_t* row_ptr,
                   const uint32_t* col_idx, const float* val, const float* B,
                   uint32_t ldb, uint32_t K, double* C);
+/* include/ptx_tf32.hpp:19-29: the reference rounds A and B to TF32 before
+ * each mma (cvt.rna.tf32.f32 in the PTX ISA: 10 explicit mantissa bits,
+ * round to nearest, ties away from zero), keeping an FP32 container.
+ * Infinity and NaN pass through; a carry out of the mantissa raises the
+ * exponent (up to infinity). */
+float orc_tf32_rna(float x);
+/* C64 = tf32(A) * tf32(B) with FP64 accumulation: the input rounding of the
+ * reference's TF32 kernels with their accumulation taken as exact, i.e. the
+ * reference's precision class (for context beside the GPU forms' errors). */
+void orc_spmm_tf32_f64(uint32_t rows, const uint32_t* row_ptr,
+                       const uint32_t* col_idx, const float* val, const float* B,
+                       uint32_t ldb, uint32_t K, double* C);
 /* |A|*|B| (FP64), for the componentwise error metric. */
 void orc_spmm_abs_f64(uint32_t rows, const uint32_t* row_ptr,
                       const uint32_t* col_idx, const float* val,

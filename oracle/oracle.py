@@ -59,6 +59,9 @@ def _lib():
         L.orc_spmm_f64.argtypes = [C.c_uint32] + [C.c_void_p] * 4 + [C.c_uint32,
                                                                      C.c_uint32, C.c_void_p]
         L.orc_spmm_abs_f64.argtypes = L.orc_spmm_f64.argtypes
+        L.orc_spmm_tf32_f64.argtypes = L.orc_spmm_f64.argtypes
+        L.orc_tf32_rna.restype = C.c_float
+        L.orc_tf32_rna.argtypes = [C.c_float]
         L.orc_errors.argtypes = [C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32,
                                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]
@@ -189,6 +192,23 @@ def spmm_f64(row_ptr, col_idx, val, B: np.ndarray) -> np.ndarray:
     ci = np.ascontiguousarray(col_idx, np.uint32)
     va = np.ascontiguousarray(val, np.float32)
     _lib().orc_spmm_f64(rows, _p(rp), _p(ci), _p(va), _p(B), K, K, _p(out))
+    return out
+
+
+def tf32_rna(x: float) -> float:
+    return _lib().orc_tf32_rna(x)
+
+
+def spmm_tf32_f64(row_ptr, col_idx, val, B: np.ndarray) -> np.ndarray:
+    """tf32(A) * tf32(B), FP64 accumulation (the reference's precision class)."""
+    rows = np.asarray(row_ptr).size - 1
+    B = np.ascontiguousarray(B, np.float32)
+    K = B.shape[1]
+    out = np.empty((rows, K), np.float64)
+    rp = np.ascontiguousarray(row_ptr, np.uint32)
+    ci = np.ascontiguousarray(col_idx, np.uint32)
+    va = np.ascontiguousarray(val, np.float32)
+    _lib().orc_spmm_tf32_f64(rows, _p(rp), _p(ci), _p(va), _p(B), K, K, _p(out))
     return out
 
 

@@ -185,3 +185,43 @@ def test_errors_metric_and_cpu_baseline():
     assert O.errors(z, np.zeros_like(C64), zero_ab)[0] == 0.0
     z[0, 0] = 1e-30
     assert O.errors(z, np.zeros_like(C64), zero_ab)[0] == float("inf")
+
+
+def test_tf32_rna_known_values():
+    # TF32 keeps 10 explicit mantissa bits; ties (exactly half an ulp of
+    # 2^-10 at 1.0) round away from zero, as the PTX ISA's .rna
+    import math
+    ulp = 2.0 ** -10
+    cases = [
+        (1.0, 1.0),
+        (1.0 + 2.0 ** -11, 1.0 + ulp),          # tie -> away from zero
+        (-(1.0 + 2.0 ** -11), -(1.0 + ulp)),
+        (1.0 + 2.0 ** -12, 1.0),                 # below half: down
+        (1.0 + 3 * 2.0 ** -12, 1.0 + ulp),       # above half: up
+        (2.0 - 2.0 ** -12, 2.0),                 # carry into the exponent
+        (0.0, 0.0),
+        (1.5 * 2.0 ** 100, 1.5 * 2.0 ** 100),   # exactly representable: kept
+    ]
+    for x, want in cases:
+        assert O.tf32_rna(float(np.float32(x))) == want, (x, O.tf32_rna(x), want)
+    assert math.isinf(O.tf32_rna(float("inf"))) and math.isnan(O.tf32_rna(float("nan")))
+    assert math.isinf(O.tf32_rna(float(np.finfo(np.float32).max)))  # rounds past the top
+    # every result has its 13 dropped bits clear
+    xs = np.random.default_rng(0).standard_normal(5000).astype(np.float32)
+    ys = np.array([O.tf32_rna(float(x)) for x in xs], np.float32)
+    assert np.all(ys.view(np.uint32) & 0x1FFF == 0)
+    assert np.all(np.abs(ys - xs) <= 2.0 ** -11 * np.abs(xs) * (1 + 1e-6))
+
+
+def test_tf32_precision_class():
+    # the reference's input rounding gives errors of order 2^-11..2^-10
+    # relative to |A||B|: far above the split-BF16 form's 1e-5 bound
+    rng = np.random.default_rng(5)
+    from helpers import random_csr
+    rp, ci, va = random_csr(rng, 300, 400, 0.05)
+    B = rng.uniform(-1, 1, (400, 32)).astype(np.float32)
+    C64 = O.spmm_f64(rp, ci, va, B)
+    ab = O.spmm_abs_f64(rp, ci, va, B)
+    Ct = O.spmm_tf32_f64(rp, ci, va, B).astype(np.float32)
+    mx, fr = O.errors(Ct, C64, ab)
+    assert 1e-5 < mx <= 2.0 ** -10 * (1 + 1e-3), mx
