@@ -629,7 +629,7 @@ class Server:
 
     def register(self, csr: CSR, opts: BuildOpts | None = None) -> int:
         """Builds and registers a resident graph (build options: row window,
-        reorder_rows; None = defaults)."""
+        reorder_rows; None = the library's automatic layout)."""
         import torch
         rp, ci, va = csr.to_device()
         view = CsrView(csr.rows, csr.cols, csr.nnz, rp.data_ptr(), ci.data_ptr(),

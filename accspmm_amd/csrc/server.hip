@@ -311,8 +311,14 @@ extern "C" accspmm_status accspmm_server_register_opts(accspmm_server* s,
                                                        uint32_t* graph_id) {
   if (!s || !csr || !graph_id) return ACCSPMM_ERR_INVALID_ARG;
   accspmm_build_opts bo;
-  if (opts) bo = *opts;
-  else accspmm_build_opts_default(&bo);
+  if (opts) {
+    bo = *opts;
+  } else {
+    // served graphs are never exported: the library's automatic layout
+    accspmm_build_opts_default(&bo);
+    bo.row_window = 0;
+    bo.reorder_rows = 2;
+  }
   bo.build_adpbme = 0;  // the served kernels use work units, not AdpBME groups
   accspmm_metcf* h = nullptr;
   ACC_TRY(accspmm_metcf_build(csr, &bo, &h, s->setup));

@@ -368,8 +368,9 @@ accspmm_status accspmm_server_create(const accspmm_server_opts* opts,
 accspmm_status accspmm_server_register(accspmm_server* s,
                                        const accspmm_csr_view* csr,
                                        uint32_t* graph_id);
-/* Same, built with the given options (row window, reorder_rows; NULL: the
- * defaults; AdpBME groups are never built for served graphs). */
+/* Same, built with the given options (row window, reorder_rows; NULL, and
+ * accspmm_server_register: the automatic layout, row_window 0 and
+ * reorder_rows 2; AdpBME groups are never built for served graphs). */
 accspmm_status accspmm_server_register_opts(accspmm_server* s,
                                             const accspmm_csr_view* csr,
                                             const accspmm_build_opts* opts,
