@@ -82,7 +82,7 @@ class MetcfInfo(C.Structure):
                 ("num_tiles", C.c_uint32), ("num_groups", C.c_uint32),
                 ("ibd", C.c_uint32), ("balanced", C.c_uint32),
                 ("max_window_tiles", C.c_uint32), ("device_bytes", C.c_uint64),
-                ("reordered", C.c_uint32), ("reserved", C.c_uint32)]
+                ("reordered", C.c_uint32), ("max_group_values", C.c_uint32)]
 
 
 class Tile(C.Structure):

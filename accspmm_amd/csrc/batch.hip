@@ -90,9 +90,9 @@ __global__ __launch_bounds__(256) void k_batch_spmm(
     uint32_t nprob, uint32_t* __restrict__ head) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t total = prefix[nprob];
-  __shared__ __align__(16) float lds_vals[4 * kWaveLds];
-  float* lv = lds_vals + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * kWaveLds;
-  if (lane == 0) lv[kGroupValues] = 0.f;
+  __shared__ __align__(16) float lds_vals[4 * wave_lds<RW>()];
+  float* lv = lds_vals + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * wave_lds<RW>();
+  if (lane == 0) lv[group_values<RW>()] = 0.f;
   for (;;) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(head, 1u);
@@ -255,6 +255,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     g.nb = q.nb;
       g.zero_row = h->cols;
     g.pad_tile = h->T;
+    g.wide = h->rw == 16 && h->max_group_values > dev::kSpanValues ? 1u : 0u;
     g.partials = reinterpret_cast<float*>(A + bc);
     bc += q.partial_bytes;
     single = n == 1 && q.b_inreg;

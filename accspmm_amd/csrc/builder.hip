@@ -220,6 +220,27 @@ __global__ void k_ibd_sum(const uint32_t* __restrict__ rwo, uint32_t W,
   }
 }
 
+// Values of the fullest 4-tile group: groups start at each window's first
+// tile and every 4 tiles after it (the SpMM kernels' groups; work-unit
+// pieces start on them too).  One wave per window, lanes over its groups.
+__global__ void k_group_values(const uint32_t* __restrict__ rwo,
+                               const uint32_t* __restrict__ tco, uint32_t W,
+                               uint32_t* __restrict__ max_values) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+  uint32_t mx = 0;
+  for (uint64_t w = blockIdx.x * (uint64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); w < W;
+       w += waves) {
+    const uint32_t tb = rwo[w], te = rwo[w + 1];
+    for (uint64_t t = tb + 4ull * lane; t < te; t += 256) {
+      const uint32_t e = (uint32_t)min<uint64_t>(t + 4, te);
+      mx = max(mx, tco[e] - tco[t]);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_down(mx, off, 64));
+  if (lane == 0) atomicMax(max_values, mx);
+}
+
 // ---- AdpBME grouping (include/format.hpp:346,366-382,418-423) ----------
 // Tile t costs cf = t_load+t_comp+t_write when it is the first tile of its
 // window, else co = t_load+t_comp.  A group starting at tile i always takes
@@ -312,6 +333,24 @@ __global__ void k_group_write(const uint32_t* __restrict__ next, uint32_t T,
 }
 
 }  // namespace
+
+// h->max_group_values from the device rowWindowOffset and tcOffset
+accspmm_status measure_group_values(accspmm_metcf* h, hipStream_t s) {
+  h->max_group_values = 0;
+  if (h->W == 0 || h->T == 0) return ACCSPMM_OK;
+  DevBuf mx;
+  ACC_TRY(mx.alloc(4));
+  ACC_HIP(hipMemsetAsync(mx.p, 0, 4, s));
+  const uint32_t blocks = std::min<uint32_t>((h->W + 3) / 4, 4096);
+  hipLaunchKernelGGL(k_group_values, dim3(blocks), dim3(256), 0, s, h->rwo.as<uint32_t>(),
+                     h->tco.as<uint32_t>(), h->W, mx.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  uint32_t v = 0;
+  ACC_HIP(hipMemcpyAsync(&v, mx.p, 4, hipMemcpyDeviceToHost, s));
+  ACC_HIP(hipStreamSynchronize(s));
+  h->max_group_values = v;
+  return ACCSPMM_OK;
+}
 }  // namespace accspmm
 
 using namespace accspmm;
@@ -506,6 +545,7 @@ accspmm_status build_impl(const accspmm_csr_view* csr,
     h->balanced = h->ibd > o.imbalance_threshold;
     h->max_window_tiles = (uint32_t)(hacc[1] & 0xFFFFFFFFull);
   }
+  ACC_TRY(measure_group_values(h, s));
 
   // AdpBME groups
   if (o.build_adpbme) {
@@ -655,7 +695,7 @@ extern "C" accspmm_status accspmm_metcf_get_info(const accspmm_metcf* h,
   info->max_window_tiles = h->max_window_tiles;
   info->device_bytes = h->device_bytes();
   info->reordered = h->row_perm.p ? 1u : 0u;
-  info->reserved = 0;
+  info->max_group_values = h->max_group_values;
   return ACCSPMM_OK;
 }
 

@@ -118,6 +118,7 @@ struct accspmm_metcf {
   uint32_t rw = 8, words = 1;
   uint32_t W = 0, T = 0, G = 0;
   uint32_t ibd = 0, balanced = 0, max_window_tiles = 0;
+  uint32_t max_group_values = 0;  // values of the fullest 4-tile group
   accspmm_build_opts opts{};
   accspmm::DevBuf rwo, tco, bits, a2b, data, grp, rix;
   // rows reordered at build (opts.reorder_rows): format row i is input row
@@ -162,4 +163,6 @@ accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
                             hipStream_t s, const Schedule** out);
 // builds h->vsplit if absent (caller holds h->mu)
 accspmm_status ensure_vsplit(const accspmm_metcf* h, hipStream_t s);
+// sets h->max_group_values (builder.hip); after a build or a load
+accspmm_status measure_group_values(accspmm_metcf* h, hipStream_t s);
 }  // namespace accspmm

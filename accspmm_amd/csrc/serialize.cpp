@@ -245,6 +245,7 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
   }
   if (reorder) ACC_TRY(to_device(h->row_perm, order, order.size()));
   ACC_HIP(hipDeviceSynchronize());
+  ACC_TRY(measure_group_values(h.get(), nullptr));
   *out = h.release();
   return ACCSPMM_OK;
 }

@@ -73,9 +73,9 @@ __global__ __launch_bounds__(512) void spmm_kernel(SpmmArgs g) {
     kc = tau % g.n_kc;
   }
   if (unit >= g.num_units) return;
-  __shared__ __align__(16) float lds_vals[8 * kWaveLds];
-  float* lv = lds_vals + wave * kWaveLds;
-  if ((threadIdx.x & 63u) == 0) lv[kGroupValues] = 0.f;
+  __shared__ __align__(16) float lds_vals[8 * wave_lds<RW>()];
+  float* lv = lds_vals + wave * wave_lds<RW>();
+  if ((threadIdx.x & 63u) == 0) lv[group_values<RW>()] = 0.f;
   const WorkUnit u = g.units[unit];
   run_unit<FORM, NB, RW, PACKED>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv);
 }
@@ -534,6 +534,7 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
                 : (tile.xcd_slice == 1 && (8 % n_kc) == 0) ? 1u : 0u;
   g.zero_row = h->cols;
   g.pad_tile = h->T;
+  g.wide = h->rw == 16 && h->max_group_values > dev::kSpanValues ? 1u : 0u;
 
   if (sch->num_pieces && !tile.sweep && (phases & 2)) {
     const uint64_t pbytes = (uint64_t)sch->num_pieces * h->rw * g.kc_padded * 4;

@@ -52,6 +52,9 @@ struct SpmmArgs {
   uint32_t kc_padded;  // KC * n_kc (partials row stride)
   uint32_t xcd_slice;
   uint32_t rw, nb;     // used by the batched kernel's dispatch
+  // 1: 16-row windows with groups of more than 256 values (the graph's
+  // max_group_values), which load their values in two spans
+  uint32_t wide;
 };
 
 __device__ inline uint32_t popc64(unsigned long long x) { return __popcll(x); }
@@ -131,14 +134,19 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
 }
 
 // A 4-tile group's A values are consecutive in the METCF values array (its
-// tiles are consecutive), at most 4 x 64 of them: one 16-byte load per lane
-// brings a 256-float span starting at the group's first value, which goes
-// through a per-wave LDS buffer (kGroupValues floats, then one zero read by
-// empty bitmap slots).  One vector-memory instruction instead of one per
-// tile column: the texture-address unit, which handles every lane address
-// of every load at four lanes a clock, bounds this kernel.
-constexpr uint32_t kGroupValues = 256;
-constexpr uint32_t kWaveLds = kGroupValues + 4;  // floats per wave (16-byte multiple)
+// tiles are consecutive): at most 4 x 64 of them with 8-row windows, 4 x 128
+// with 16-row windows.  One 16-byte load per lane brings a 256-float span
+// starting at the group's first value, which goes through a per-wave LDS
+// buffer (group_values floats, then one zero read by empty bitmap slots).
+// One vector-memory instruction instead of one per tile column: the
+// texture-address unit, which handles every lane address of every load at
+// four lanes a clock, bounds this kernel.  A 16-row group of more than 256
+// values (tiles over half full) loads the rest when it is consumed.
+constexpr uint32_t kSpanValues = 256;
+template <int RW>
+constexpr uint32_t group_values() { return RW == 16 ? 512u : 256u; }
+template <int RW>
+constexpr uint32_t wave_lds() { return group_values<RW>() + 4u; }  // 16-byte multiple
 
 // One pipeline stage: everything a lane needs for one 4-tile group.  Padded
 // columns of the BF16 forms read the converted B's zero row; bit j of bmask
@@ -151,6 +159,7 @@ struct Stage {
   uint32_t base;  // index of the row's first value within the span
   uint32_t bmask;
   uint32_t win;   // tag of the group (wave-uniform)
+  uint32_t vbase; // RW=16: index of the group's first value (wave-uniform)
 };
 
 // Tile metadata as loaded: lane n of a quarter holds columns 4(n&1)..+3 of
@@ -264,8 +273,8 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
   // nnz, clamped to the span) re-read the span's first 16 bytes, so only
   // the cache lines holding values are requested.  Every lane still issues
   // the load: the wait counts of the pipeline stay static.
-  uint32_t nv = kGroupValues;
-  if (RW == 8) nv = min(__builtin_amdgcn_readlane(mt.y, 49) - vbase, kGroupValues);
+  uint32_t nv = kSpanValues;
+  if (RW == 8) nv = min(__builtin_amdgcn_readlane(mt.y, 49) - vbase, kSpanValues);
   const uint32_t vi = 4u * lane < nv ? 4u * lane : 0u;
   const u4a v = *reinterpret_cast<const u4a*>(vsrc + vbase + vi);
   st.vals = u4{v.x, v.y, v.z, v.w};
@@ -273,20 +282,37 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
   st.byte = a_rows ? (uint32_t)(bw >> sh) & 0xFFu : 0u;
   st.base = off - vbase + (sh ? popc64(bw & ((1ull << sh) - 1ull)) : 0u);
   st.win = mt.win;
+  if (RW == 16) st.vbase = vbase;
 }
 
 template <int FORM, int NB, int RW>
 __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
-                                     uint32_t m, float* lv, uint32_t lane) {
+                                     uint32_t m, float* lv, uint32_t lane,
+                                     const uint32_t* __restrict__ vsrc, bool wide) {
   // the group's value span through the wave's LDS buffer: a set bitmap bit
-  // j reads the row's next value, a clear one the zero after the span
+  // j reads the row's next value, a clear one the zero after the values
   *reinterpret_cast<u4*>(lv + 4u * lane) = st.vals;
+  if (RW == 16 && wide) {
+    // the graph has 16-row groups of more than 256 values: count this
+    // group's (every lane holds one tile row's bitmap byte) and load the
+    // rest of a full one here.  The load waits for every load in flight;
+    // graphs whose groups all fit the span (wide = false) never take it.
+    uint32_t c = __popc(st.byte);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) c += __shfl_xor(c, o, 64);
+    const uint32_t nv = __builtin_amdgcn_readfirstlane(c);
+    if (nv > kSpanValues) {
+      const uint32_t i = kSpanValues + 4u * lane;
+      const u4a v = *reinterpret_cast<const u4a*>(vsrc + st.vbase + (i < nv ? i : 0u));
+      *reinterpret_cast<u4*>(lv + kSpanValues + 4u * lane) = u4{v.x, v.y, v.z, v.w};
+    }
+  }
   uint32_t a[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     const bool on = (st.byte >> j) & 1u;
     const uint32_t rank = __popc(st.byte & ((1u << j) - 1u));
-    a[j] = __builtin_bit_cast(uint32_t, lv[on ? st.base + rank : kGroupValues]);
+    a[j] = __builtin_bit_cast(uint32_t, lv[on ? st.base + rank : group_values<RW>()]);
   }
   if (FORM == kFP32) {
 #pragma unroll
@@ -476,9 +502,10 @@ __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
                                  uint32_t lane) {
   Meta mA, mB;
   Stage<FORM, NB> dA, dB;
+  const uint32_t* vsrc = FORM == kFP32 ? reinterpret_cast<const uint32_t*>(g.data) : g.vsplit;
   auto consume = [&](Stage<FORM, NB>& d) {
     if (PACKED && out.direct) out.move_to(d.win);
-    compute_stage<FORM, NB, RW>(d, out.acc, m, lv, lane);
+    compute_stage<FORM, NB, RW>(d, out.acc, m, lv, lane, vsrc, g.wide != 0u);
   };
   // pos: position of the most recently loaded metadata (group gn)
   uint32_t gn = 0;

@@ -124,12 +124,13 @@ __device__ void sweep_loop(const SweepArgs& a, f4* acc, f4* slots, GroupPos pos,
   };
   Meta mA, mB;
   Stage<FORM, NB> dA, dB;
+  const uint32_t* vsrc = FORM == kFP32 ? reinterpret_cast<const uint32_t*>(g.data) : g.vsplit;
   auto consume = [&](Stage<FORM, NB>& d) {
     if (d.win != cur) {
       end_piece(cur);
       cur = d.win;
     }
-    compute_stage<FORM, NB, RW>(d, acc, m, lv, lane);
+    compute_stage<FORM, NB, RW>(d, acc, m, lv, lane, vsrc, g.wide != 0u);
   };
   load_meta<RW>(mA, g, pos, true, q, m);
   if (ngroups > 1) sweep_next(pos, a.pieces);
@@ -179,7 +180,7 @@ __device__ void sweep_loop(const SweepArgs& a, f4* acc, f4* slots, GroupPos pos,
 template <int FORM, int NB, int RW>
 __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
   extern __shared__ f4 lds_slots[];
-  __shared__ __align__(16) float lds_vals[8 * kWaveLds];
+  __shared__ __align__(16) float lds_vals[8 * wave_lds<RW>()];
   const SpmmArgs& g = a.g;
   constexpr uint32_t KC = 16u * NB;
   constexpr uint32_t NL = slot_lanes<RW>();
@@ -191,8 +192,8 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
   const uint32_t m = n;
   const bool a_rows = RW == 16 || m < 8 || is_split(FORM);
   f4* slots = lds_slots + (size_t)wv * a.M * NB * NL;
-  float* lv = lds_vals + wv * kWaveLds;
-  if (lane == 0) lv[kGroupValues] = 0.f;
+  float* lv = lds_vals + wv * wave_lds<RW>();
+  if (lane == 0) lv[group_values<RW>()] = 0.f;
   for (uint32_t r = 0; r < a.R; ++r) {
     const uint32_t idx = r * a.NW + v;
     const uint32_t pb = __builtin_amdgcn_readfirstlane(a.poff[idx]);
@@ -230,8 +231,8 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
   }
 }
 
-// running-sum slots; with the per-wave value buffers the block stays
-// within the CU's 160 KiB of LDS
+// running-sum slots; with the per-wave value buffers (16.5 KiB for 16-row
+// windows) the block stays within the CU's 160 KiB of LDS
 constexpr uint32_t kSweepLdsBytes = 136u << 10;
 
 template <int FORM, int NB, int RW>

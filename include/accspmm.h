@@ -94,7 +94,10 @@ typedef struct accspmm_metcf_info {
   uint32_t max_window_tiles;
   uint64_t device_bytes;  /* device memory held by the handle         */
   uint32_t reordered;     /* built with reorder_rows = 1              */
-  uint32_t reserved;
+  /* values of the fullest 4-tile group (groups start at each window's
+   * first tile); with 16-row windows, groups of more than 256 values
+   * (tiles over half full) take a slower two-span load in the kernels */
+  uint32_t max_group_values;
 } accspmm_metcf_info;
 
 /* GPU METCF builder.  Replaces METCFBit::convertFromCSR

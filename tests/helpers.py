@@ -101,3 +101,38 @@ def scrambled_csr(rng: np.random.Generator, rows: int, cols: int, per_row_max: i
         ci[idx] = ci[src]
     va = rng.uniform(-1, 1, nnz).astype(np.float32)
     return rp, ci, va
+
+
+def dense_block_csr(rng: np.random.Generator, rows: int, cols: int, block_cols: int,
+                    fill: float):
+    """Canonical CSR whose 16-row blocks each draw block_cols random columns
+    and keep each (row, column) of the block with probability fill: tiles of
+    16-row windows near `fill` full (groups of up to 512 values), every
+    third block empty of values in its second half of rows."""
+    rp, ci, va = [0], [], []
+    for b0 in range(0, rows, 16):
+        cs = np.sort(rng.choice(cols, size=min(block_cols, cols), replace=False))
+        for r in range(b0, min(rows, b0 + 16)):
+            if (b0 // 16) % 3 == 2 and r - b0 >= 8:
+                rp.append(len(ci))
+                continue
+            keep = cs[rng.random(len(cs)) < fill]
+            ci += keep.tolist()
+            va += rng.uniform(-1, 1, size=len(keep)).astype(np.float32).tolist()
+            rp.append(len(ci))
+    return (np.array(rp, np.uint32), np.array(ci, np.uint32), np.array(va, np.float32))
+
+
+def max_group_values(row_window: int, rows: int, row_ptr, col_idx, val) -> int:
+    """Values of the fullest 4-tile group of the oracle's METCF (groups start
+    at each window's first tile and every 4 tiles after it)."""
+    from oracle import oracle as O
+    m = O.metcf_build(row_window, rows, row_ptr, col_idx, val)
+    rwo = np.asarray(m["rowWindowOffset"], np.int64)
+    tco = np.asarray(m["tcOffset"], np.int64)
+    best = 0
+    for w in range(len(rwo) - 1):
+        s = np.arange(rwo[w], rwo[w + 1], 4)
+        if len(s):
+            best = max(best, int((tco[np.minimum(s + 4, rwo[w + 1])] - tco[s]).max()))
+    return best

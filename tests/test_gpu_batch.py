@@ -140,3 +140,36 @@ def test_single_problem_batches():
                 torch.cuda.synchronize()
                 assert torch.equal(Cm, alone), (csr.rows, K, graph)
             batch.close()
+
+
+@pytest.mark.parametrize("form", [acc.FORM_FP32, acc.FORM_SPLIT_BF16, acc.FORM_BF16],
+                         ids=acc.FORM_NAMES.get)
+def test_batch_dense_groups(form):
+    # 16-row graphs whose 4-tile groups hold more than 256 values, batched
+    # with sparse ones: each C bit-equal to its standalone call, eager and
+    # replayed, in a batch of several problems and in a batch of one
+    import torch
+    from helpers import dense_block_csr
+    rng = np.random.default_rng(98)
+    dense = acc.CSR(200, 700, *dense_block_csr(rng, 200, 700, 100, 0.85))
+    sparse = acc.gen_chung_lu(3000, 12.0, 2.2, 9, 0)
+    gs = [acc.DeviceGraph(dense, acc.default_build_opts(16)), acc.DeviceGraph(sparse),
+          acc.DeviceGraph(dense, acc.default_build_opts(16, reorder_rows=1)),
+          acc.DeviceGraph(sparse, acc.default_build_opts(16))]
+    assert gs[0].info.max_group_values > 256
+    Ks = (64, 16, 128, 32)
+    Bs = [torch.from_numpy(acc.gen_dense(i + 3, g.info.cols, K)).cuda()
+          for i, (g, K) in enumerate(zip(gs, Ks))]
+    Cs = [torch.full((g.info.rows, B.shape[1]), float("nan"), device="cuda")
+          for g, B in zip(gs, Bs)]
+    alone = [g.spmm(B, None, form).clone() for g, B in zip(gs, Bs)]
+    for sel in (slice(0, 4), slice(0, 1)):
+        batch = acc.Batch(list(zip(gs[sel], Bs[sel], Cs[sel])), form)
+        for graph in (False, True, True):
+            for c in Cs[sel]:
+                c.fill_(float("nan"))
+            batch.run(graph=graph)
+            torch.cuda.synchronize()
+            for Cm, A in zip(Cs[sel], alone[sel]):
+                assert torch.equal(Cm, A), (sel, graph)
+        batch.close()

@@ -114,3 +114,28 @@ def test_mma_driver_auto_layout(tmp_path, form):
     line = [ln for ln in r.stdout.splitlines() if ln.startswith("check:")]
     assert line and line[0].endswith("ok"), r.stdout
     assert (tmp_path / "result.csv").read_text().startswith("long.mtx,64,")
+
+
+def test_mma_driver_path_selection(tmp_path):
+    # the driver picks the balanced schedule exactly when the oracle's
+    # balanceStrategy restatement says IBD > 8 (include/utils.hpp:75-85) and
+    # prints the reference's line for the chosen path
+    from helpers import random_csr
+    rng = np.random.default_rng(90)
+    even = acc.gen_stencil27(12, 3)                         # every window alike
+    rp, ci, va = random_csr(rng, 900, 4000, 0.004, heavy_rows=6)
+    skew = acc.CSR(900, 4000, rp, ci, va)                   # a few dense rows
+    mma = str(REPO / "accspmm_amd" / "bin" / "mma")
+    seen = set()
+    for name, csr in (("even", even), ("skew", skew)):
+        m = O.metcf_build(8, csr.rows, csr.row_ptr, csr.col_idx, csr.data)
+        _, balanced = O.ibd(m["rowWindowOffset"])
+        seen.add(balanced)
+        mtx = tmp_path / f"{name}.mtx"
+        csr.write_mtx(mtx)
+        r = subprocess.run([mma, str(mtx), "32", "--warmup", "1", "--exe", "2"],
+                           capture_output=True, text=True, cwd=tmp_path, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        want = "doing AdpBalance pipelining ..." if balanced else "doing pipelining mma ..."
+        assert want in r.stdout, (name, balanced, r.stdout)
+    assert seen == {True, False}, "the two matrices must take both paths"

@@ -71,6 +71,8 @@ def test_random_matches_oracle(seed, rw):
     _assert_same(got, exp, ALL)
     ibd, bal = O.ibd(exp["rowWindowOffset"])
     assert (g.info.ibd, bool(g.info.balanced)) == (ibd, bal)
+    from helpers import max_group_values
+    assert g.info.max_group_values == max_group_values(rw, rows, rp, ci, va)
 
 
 @pytest.mark.parametrize("t", [(1, 1, 1, 4), (2, 2, 0, 8), (5, 0, 3, 10)])
@@ -152,7 +154,7 @@ def test_save_load_round_trip(tmp_path, rw, adpbme):
         assert np.array_equal(a[k], b[k]), k
     ia, ib = g.info, h.info
     for f in ("rows", "cols", "nnz", "row_window", "num_windows", "num_tiles", "num_groups",
-              "ibd", "balanced", "max_window_tiles"):
+              "ibd", "balanced", "max_window_tiles", "max_group_values"):
         assert getattr(ia, f) == getattr(ib, f), f
     assert g.tune_key(64) == h.tune_key(64)
     B = torch.from_numpy(acc.gen_dense(3, cols, 64)).cuda()

@@ -259,3 +259,71 @@ def test_split_in_registers_unaligned_b_falls_back():
     got = g.spmm(Bv, None, acc.FORM_SPLIT_BF16, acc.Tile(4, 4, 0, 0, 64, 0, 0, 2))
     torch.cuda.synchronize()
     assert torch.equal(ref, got)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+def test_edge_shapes(form):
+    # an all-empty matrix writes exact zeros over a NaN-filled C; a 1x1
+    # matrix; one row of 3,000 nonzeros (one window, split into pieces);
+    # explicit zero values; K far above the reference's 512
+    torch = _torch()
+    empty = acc.CSR(19, 7, np.zeros(20, np.uint32), np.zeros(0, np.uint32),
+                    np.zeros(0, np.float32))
+    g = acc.DeviceGraph(empty)
+    for K in (1, 16, 33):
+        B = torch.from_numpy(acc.gen_dense(K, 7, K)).cuda()
+        C = torch.full((19, K), float("nan"), device="cuda")
+        g.spmm(B, C, form)
+        torch.cuda.synchronize()
+        assert torch.count_nonzero(C).item() == 0 and not torch.isnan(C).any()
+    one = acc.CSR(1, 1, np.array([0, 1], np.uint32), np.array([0], np.uint32),
+                  np.array([2.5], np.float32))
+    for K in (1, 8):
+        mx, _, Cg = run_case(one, K, form, seed=K)
+        assert mx <= TOL[form], (K, mx)
+    rng = np.random.default_rng(95)
+    ci = np.sort(rng.choice(5000, 3000, replace=False)).astype(np.uint32)
+    va = rng.uniform(-1, 1, 3000).astype(np.float32)
+    va[::7] = 0.0                                            # stored zeros
+    row = acc.CSR(1, 5000, np.array([0, 3000], np.uint32), ci, va)
+    for tile in (None, acc.Tile(4, 4, 64, 0, 64)):
+        mx, _, _ = run_case(row, 64, form, tile=tile, seed=4)
+        assert mx <= TOL[form], (tile, mx)
+    rp, ci, va = random_csr(rng, 300, 400, 0.03, empty_rows=0.1, heavy_rows=2)
+    wide = acc.CSR(300, 400, rp, ci, va)
+    for K in (1024, 2000):
+        mx, _, _ = run_case(wide, K, form, seed=K)
+        assert mx <= TOL[form], (K, mx)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+def test_dense_groups(form):
+    # 16-row tiles about 90% full: 4-tile groups of up to 512 values, more
+    # than the 256 one span load brings (and full 8-row groups of 256).
+    # Every schedule shape against the oracle; the sweep bit-equal to the
+    # work units; the fullest group reported as the oracle counts it
+    torch = _torch()
+    from helpers import dense_block_csr, max_group_values
+    rng = np.random.default_rng(97)
+    rp, ci, va = dense_block_csr(rng, 300, 900, 120, 0.9)
+    csr = acc.CSR(300, 900, rp, ci, va)
+    full = acc.CSR(64, 64, *dense_block_csr(rng, 64, 64, 64, 1.0))
+    bs = (0,) if form == acc.FORM_FP32 else (1, 2)
+    for rw in (16, 8):
+        g = acc.DeviceGraph(csr, acc.default_build_opts(rw))
+        assert g.info.max_group_values == max_group_values(rw, 300, rp, ci, va)
+        assert (g.info.max_group_values > 256) == (rw == 16)
+        for K in (16, 100):
+            for b in bs:
+                for tile in (acc.Tile(2, 4, 0, 0, 64, 0, 0, b), acc.Tile(4, 4, 8, 0, 0, 0, 0, b),
+                             acc.Tile(1, 8, 0, 1, 64, 3, 0, b), acc.Tile(8, 2, 4, 2, 64, 0, 0, b)):
+                    mx, fr, _ = run_case(csr, K, form, tile=tile, rw=rw, seed=K, graph=g)
+                    assert mx <= TOL[form], (rw, K, tile.as_tuple(), mx)
+        B = torch.from_numpy(acc.gen_dense(7, 900, 64)).cuda()
+        outs = [g.spmm(B, None, form, acc.Tile(2, 4, 8, 0, 64, 2, sw)).clone() for sw in (0, 1)]
+        torch.cuda.synchronize()
+        assert torch.equal(outs[0], outs[1]), rw
+        gf = acc.DeviceGraph(full, acc.default_build_opts(rw))
+        assert gf.info.max_group_values == (512 if rw == 16 else 256)
+        mx, fr, _ = run_case(full, 32, form, rw=rw, seed=5, graph=gf)
+        assert mx <= TOL_HIGH_DEGREE[form], (rw, mx)
