@@ -620,6 +620,7 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
   if (o.build_adpbme && o.t_load + o.t_comp == 0) return ACCSPMM_ERR_UNSUPPORTED;
   if (o.reorder_rows > 2) return ACCSPMM_ERR_UNSUPPORTED;
   if (csr->rows == 0 || csr->cols == 0) return ACCSPMM_ERR_INVALID_ARG;
+  if (csr->nnz > kMaxNnz) return ACCSPMM_ERR_UNSUPPORTED;
   // automatic choices (row_window 0, reorder_rows 2): 16-row windows and
   // rows by length only for long rows (mean >= kLongRows nonzeros), where
   // they were measured faster (232,965-node power-law graph, mean 493:

@@ -35,6 +35,9 @@ constexpr uint32_t kPad = 0xFFFFFFFFu;
 // zero floats kept past the METCF values array (a 4-tile group's values are
 // loaded as one 256-float span)
 constexpr uint32_t kValuePad = 264;
+// value offsets are 32-bit in the kernels (a group's span may run up to 512
+// values past its first): nonzeros up to 2^32 - 1024
+constexpr uint32_t kMaxNnz = 0xFFFFFFFFu - 1023u;
 
 // Device buffer owned by RAII.
 struct DevBuf {

@@ -97,6 +97,8 @@ bool monotone(const std::vector<uint32_t>& v, uint32_t last) {
 extern "C" accspmm_status accspmm_metcf_save(const accspmm_metcf* h, const char* path) {
   if (!h || !path) return ACCSPMM_ERR_INVALID_ARG;
   const uint32_t T = h->T, W = h->W, words = h->words;
+  // the loader's bound on sparseA2B (8T entries counted in 32 bits)
+  if ((uint64_t)T * kColWindow > 0xFFFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
   const bool adp = h->grp.p != nullptr;
   std::vector<uint32_t> rwo, tco, a2b, grp, rix, order;
   std::vector<unsigned long long> bits;
@@ -160,7 +162,7 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
   const uint32_t rows = hdr[0], cols = hdr[1], nnz = hdr[2], rw = hdr[3], W = hdr[4],
                  T = hdr[5], G = hdr[6], adp = hdr[10], key_len = hdr[18],
                  reorder = hdr[19];
-  if (!r.ok || (rw != 8 && rw != 16) || rows == 0 || cols == 0 ||
+  if (!r.ok || (rw != 8 && rw != 16) || rows == 0 || cols == 0 || nnz > kMaxNnz ||
       W != (rows + rw - 1) / rw || adp > 1 || reorder > 1 || key_len > 4096 ||
       (uint64_t)T * kColWindow > 0xFFFFFFFFull)
     return ACCSPMM_ERR_FILE;

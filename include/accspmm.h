@@ -104,8 +104,10 @@ typedef struct accspmm_metcf_info {
  * (include/format.hpp:215-274), AdpBME::CSR2AdpBME (:340-429) and
  * balanceStrategy (include/utils.hpp:75-85).  `csr` holds DEVICE pointers to
  * a canonical CSR (each row strictly increasing in column); a
- * non-canonical row yields ACCSPMM_ERR_NOT_CANONICAL.  Blocks until the
- * handle is complete.  The handle owns its device arrays. */
+ * non-canonical row yields ACCSPMM_ERR_NOT_CANONICAL; more than
+ * 2^32 - 1024 nonzeros (value offsets are 32-bit, as the reference's vint)
+ * yields ACCSPMM_ERR_UNSUPPORTED.  Blocks until the handle is complete.
+ * The handle owns its device arrays. */
 accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
                                    const accspmm_build_opts* opts,
                                    accspmm_metcf** out, void* stream);

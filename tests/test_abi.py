@@ -48,3 +48,17 @@ def test_null_arguments_rejected_without_gpu():
     assert L.accspmm_metcf_destroy(None) == acc.Status.ERR_INVALID_ARG
     assert L.accspmm_batch_run(None, None) == acc.Status.ERR_INVALID_ARG
     assert L.accspmm_server_poll(None, 0, None) == acc.Status.ERR_INVALID_ARG
+
+
+def test_oversized_nnz_rejected_before_device_work():
+    # value offsets are 32-bit: more than 2^32 - 1024 nonzeros is refused
+    # before any device call (the pointers are never read)
+    L = acc.lib()
+    C = ctypes
+    dummy = (C.c_uint32 * 4)()
+    view = acc.CsrView(4, 4, 0xFFFFFFFF - 1023 + 1, C.addressof(dummy), C.addressof(dummy),
+                       C.addressof(dummy))
+    h = C.c_void_p()
+    assert L.accspmm_metcf_build(C.byref(view), None, C.byref(h), None) == \
+        acc.Status.ERR_UNSUPPORTED
+    assert not h.value
