@@ -41,9 +41,11 @@ struct ConvDesc {
   const float* src;
   void* dst;
   uint32_t ldb, K, rows, ldx;
+  uint32_t vec;  // 16-byte loads allowed (ldb % 4 == 0, src 16-byte aligned)
 };
 
-constexpr uint32_t kConvRows = 16;
+// four-feature chunks of converted B per block (8 per thread)
+constexpr uint32_t kConvChunks = 2048;
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
@@ -62,26 +64,21 @@ __device__ inline uint32_t last_le(const uint32_t* __restrict__ a, uint32_t n,
   return lo;
 }
 
+// Every problem's B conversion in one launch: the blocks of problem p
+// convert its chunks [(b - prefix[p]) * kConvChunks, ...) of the
+// (rows + 1) * ldx / 4 (convert_chunk, the single-graph conversion's code,
+// so the converted bits are the same).
 template <int FORM>
 __global__ __launch_bounds__(256) void k_batch_convert(
     const ConvDesc* __restrict__ d, uint32_t nconv,
     const uint32_t* __restrict__ prefix) {
   const uint32_t p = last_le(prefix, nconv, blockIdx.x);
   const ConvDesc c = d[p];
-  // rows [0, c.rows) of B and the zero row c.rows that padded columns read
-  const uint32_t r0 = (blockIdx.x - prefix[p]) * kConvRows;
-  const uint32_t r1 = min(r0 + kConvRows, c.rows + 1);
-  for (uint32_t r = r0; r < r1; ++r)
-    for (uint32_t k = threadIdx.x; k < c.ldx; k += blockDim.x) {
-      float x = k < c.K && r < c.rows ? c.src[(uint64_t)r * c.ldb + k] : 0.0f;
-      uint32_t hp = pack_bf16(x, 0.f) & 0xFFFFu;
-      if (FORM == 1) {
-        uint32_t lp = pack_bf16(x - bf16_lo_to_f32(hp), 0.f) & 0xFFFFu;
-        static_cast<uint32_t*>(c.dst)[(uint64_t)r * c.ldx + k] = hp | (lp << 16);
-      } else {
-        static_cast<uint16_t*>(c.dst)[(uint64_t)r * c.ldx + k] = (uint16_t)hp;
-      }
-    }
+  const uint64_t total = (uint64_t)(c.rows + 1) * (c.ldx / 4);
+  const uint64_t c0 = (uint64_t)(blockIdx.x - prefix[p]) * kConvChunks;
+  const uint64_t c1 = min<uint64_t>(c0 + kConvChunks, total);
+  for (uint64_t ch = c0 + threadIdx.x; ch < c1; ch += blockDim.x)
+    convert_chunk<FORM>(c.src, c.rows, c.K, c.ldb, c.dst, c.ldx, c.vec, ch);
 }
 
 template <int FORM, int RW>
@@ -236,7 +233,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     dev::SpmmArgs& g = D[j];
     std::memset(&g, 0, sizeof(g));
     g.rwo = h->rwo.as<uint32_t>();
-  g.tco = h->tco.as<uint32_t>();
+    g.tco = h->tco.as<uint32_t>();
     g.bits = h->bits.as<unsigned long long>();
     g.a2b = h->a2b.as<uint32_t>();
     g.data = h->data.as<float>();
@@ -279,9 +276,14 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
       bc += q.bconv_bytes;
       g.B = dst;
       g.ldb = q.kc_padded;
-      CV[nconv] = ConvDesc{p.B, dst, p.ldb, p.K, h->cols, q.kc_padded};
+      const uint32_t vec = (p.ldb % 4 == 0 && ((uintptr_t)p.B % 16) == 0) ? 1u : 0u;
+      CV[nconv] = ConvDesc{static_cast<const float*>(p.B), dst, p.ldb, p.K, h->cols,
+                           q.kc_padded, vec};
       CP[nconv] = cblocks;
-      cblocks += (h->cols + 1 + kConvRows - 1) / kConvRows;
+      const uint64_t chunks = ((uint64_t)h->cols + 1) * (q.kc_padded / 4);
+      const uint64_t nblk = (chunks + kConvChunks - 1) / kConvChunks;
+      if ((uint64_t)cblocks + nblk > 0x7FFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
+      cblocks += (uint32_t)nblk;
       ++nconv;
     }
     PF[j] = run;

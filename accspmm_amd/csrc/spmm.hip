@@ -81,47 +81,16 @@ __global__ __launch_bounds__(512) void spmm_kernel(SpmmArgs g) {
 }
 
 // B conversion for the BF16 forms: zero-padded to ldx >= K columns, plus
-// one all-zero row N that padded tile columns read.  One thread converts
-// four consecutive features (ldx is a multiple of 16): a 16-byte load when
-// the row segment lies inside K and B is 16-byte aligned with ldb % 4 == 0,
-// and one 16-byte (split: hi | lo << 16 per feature) or 8-byte (BF16)
-// store.  Grid-stride over (N + 1) * ldx / 4 chunks.
+// one all-zero row N that padded tile columns read: grid-stride over the
+// (N + 1) * ldx / 4 four-feature chunks (convert_chunk).
 template <int FORM>
 __global__ __launch_bounds__(256) void convert_b(const float* __restrict__ B, uint32_t N,
                                                  uint32_t K, uint32_t ldb, void* __restrict__ out,
                                                  uint32_t ldx, uint32_t vec) {
-  const uint32_t cpr = ldx / 4;  // chunks per row
-  const uint64_t total = (uint64_t)(N + 1) * cpr;
+  const uint64_t total = (uint64_t)(N + 1) * (ldx / 4);
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total;
-       c += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t r = (uint32_t)(c / cpr), k = (uint32_t)(c % cpr) * 4;
-    float x[4] = {0.f, 0.f, 0.f, 0.f};
-    if (r < N) {
-      const float* src = B + (uint64_t)r * ldb + k;
-      if (vec && k + 4 <= K) {
-        const f4 v = *reinterpret_cast<const f4*>(src);
-        x[0] = v[0]; x[1] = v[1]; x[2] = v[2]; x[3] = v[3];
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) x[i] = k + i < K ? src[i] : 0.f;
-      }
-    }
-    if (FORM == kSPLIT) {
-      u4 o;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t hp = pack_bf16(x[i], 0.f) & 0xFFFFu;
-        const uint32_t lp = pack_bf16(x[i] - bf16_lo_to_f32(hp), 0.f) & 0xFFFFu;
-        o[i] = hp | (lp << 16);
-      }
-      reinterpret_cast<u4*>(out)[c] = o;
-    } else {
-      uint2 o;
-      o.x = pack_bf16(x[0], x[1]);
-      o.y = pack_bf16(x[2], x[3]);
-      reinterpret_cast<uint2*>(out)[c] = o;
-    }
-  }
+       c += (uint64_t)gridDim.x * blockDim.x)
+    convert_chunk<FORM>(B, N, K, ldb, out, ldx, vec, c);
 }
 
 // The METCF values as bf16 hi | lo << 16 (round to nearest even, lo = the

@@ -72,6 +72,46 @@ __device__ inline float bf16_hi_to_f32(uint32_t p) {
   return __builtin_bit_cast(float, p & 0xFFFF0000u);
 }
 
+// B conversion for the BF16 forms, one chunk of four consecutive features:
+// chunk c of the (N + 1) x ldx / 4 chunks of the converted B (ldx a multiple
+// of 16, zero past K and in the all-zero row N that padded tile columns
+// read).  A 16-byte load when the segment lies inside K and B is 16-byte
+// aligned with ldb % 4 == 0 (vec); one 16-byte (split: hi | lo << 16 per
+// feature) or 8-byte (BF16) store.
+template <int FORM>
+__device__ inline void convert_chunk(const float* __restrict__ B, uint32_t N, uint32_t K,
+                                     uint32_t ldb, void* __restrict__ out, uint32_t ldx,
+                                     uint32_t vec, uint64_t c) {
+  const uint32_t cpr = ldx / 4;  // chunks per row
+  const uint32_t r = (uint32_t)(c / cpr), k = (uint32_t)(c % cpr) * 4;
+  float x[4] = {0.f, 0.f, 0.f, 0.f};
+  if (r < N) {
+    const float* src = B + (uint64_t)r * ldb + k;
+    if (vec && k + 4 <= K) {
+      const f4 v = *reinterpret_cast<const f4*>(src);
+      x[0] = v[0]; x[1] = v[1]; x[2] = v[2]; x[3] = v[3];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[i] = k + i < K ? src[i] : 0.f;
+    }
+  }
+  if (FORM == kSPLIT) {
+    u4 o;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t hp = pack_bf16(x[i], 0.f) & 0xFFFFu;
+      const uint32_t lp = pack_bf16(x[i] - bf16_lo_to_f32(hp), 0.f) & 0xFFFFu;
+      o[i] = hp | (lp << 16);
+    }
+    reinterpret_cast<u4*>(out)[c] = o;
+  } else {
+    uint2 o;
+    o.x = pack_bf16(x[0], x[1]);
+    o.y = pack_bf16(x[2], x[3]);
+    reinterpret_cast<uint2*>(out)[c] = o;
+  }
+}
+
 // Elements each lane gathers per B row: NB features, as 32-bit words.
 template <int FORM, int NB>
 struct BRow {
