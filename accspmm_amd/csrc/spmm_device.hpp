@@ -144,8 +144,9 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
       r.w[0] = *p;
     }
   } else if constexpr (FORM == kSPLIT || FULL) {
-    // split form: the converted B (zero row for padded columns); FP32 form:
-    // the caller's B (padded columns read row 0, masked when consumed)
+    // split form: the converted B (zero row for padded columns); forms that
+    // read the caller's FP32 B (FP32, split in registers): padded columns
+    // read row 0, masked when consumed
     const uint32_t row = FORM == kSPLIT ? c : (col != kPad ? col : 0u);
     const uint32_t* p = static_cast<const uint32_t*>(B) + (uint64_t)row * ldb + feat;
     if constexpr (NB >= 4) {
@@ -547,13 +548,9 @@ __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
     if (PACKED && out.direct) out.move_to(d.win);
     compute_stage<FORM, NB, RW>(d, out.acc, m, lv, lane, vsrc, g.wide != 0u);
   };
-  // pos: position of the most recently loaded metadata (group gn)
-  uint32_t gn = 0;
+  // pos: position of the most recently loaded metadata
   load_meta<RW>(mA, g, pos, true, q, m);
-  if (ngroups > 1) {
-    next_group<PACKED>(pos, g.rwo);
-    gn = 1;
-  }
+  if (ngroups > 1) next_group<PACKED>(pos, g.rwo);
   load_meta<RW>(mB, g, pos, ngroups > 1, q, m);
   load_data<FORM, NB, FULL, RW>(dA, mA, g, m, a_rows, feat, B, ldb, lane);
   uint32_t gi = 0;
@@ -572,7 +569,6 @@ __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
     consume(dB);
     __builtin_amdgcn_sched_barrier(0);
   }
-  (void)gn;
   // dA holds group gi, mB the metadata of group gi+1
   const uint32_t left = ngroups - gi;  // 1, 2 or 3
   if (left == 3) {
