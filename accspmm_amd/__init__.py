@@ -231,6 +231,34 @@ def _ptr(a) -> int:
     return a.data_ptr()  # torch tensor
 
 
+def _dense(t, rows: int, what: str, name: str, K: int | None = None) -> int:
+    """Checks a dense operand: a 2-D float32 CUDA tensor of `rows` rows (and
+    K columns when given) with unit column stride.  The library receives
+    bare pointers and strides, so a smaller buffer would be read or written
+    out of bounds on the GPU; this raises instead.  Returns its K."""
+    import torch
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or not t.is_cuda \
+            or t.dim() != 2:
+        raise TypeError(f"{what}: {name} must be a 2-D float32 CUDA tensor")
+    if t.shape[0] != rows or t.shape[1] < 1 or (K is not None and t.shape[1] != K):
+        want = f"({rows}, {K if K is not None else 'K >= 1'})"
+        raise ValueError(f"{what}: {name} has shape {tuple(t.shape)}, expected {want}")
+    if t.shape[1] > 1 and t.stride(1) != 1:
+        raise ValueError(f"{what}: {name} needs unit column stride (row-major rows)")
+    if t.shape[0] > 1 and t.stride(0) < t.shape[1]:
+        raise ValueError(f"{what}: {name} rows overlap (row stride {t.stride(0)})")
+    return int(t.shape[1])
+
+
+def _operands(B, C_out, rows: int, cols: int, what: str) -> int:
+    """B (cols, K) and C (rows, K) on one device; returns K."""
+    K = _dense(B, cols, what, "B")
+    _dense(C_out, rows, what, "C", K)
+    if B.device != C_out.device:
+        raise ValueError(f"{what}: B and C are on different devices")
+    return K
+
+
 def _stream(stream) -> int:
     if stream is None:
         import torch
@@ -470,11 +498,11 @@ class DeviceGraph:
              stream=None):
         """C = A @ B on the GPU.  B: (cols, K) float32 CUDA tensor."""
         import torch
-        assert B.dtype == torch.float32 and B.is_cuda and B.dim() == 2
-        assert B.stride(1) == 1
-        K = B.shape[1]
+        inf = self.info
+        K = _dense(B, inf.cols, "spmm", "B")
         if C_out is None:
-            C_out = torch.empty((self.info.rows, K), dtype=torch.float32, device=B.device)
+            C_out = torch.empty((inf.rows, K), dtype=torch.float32, device=B.device)
+        _operands(B, C_out, inf.rows, inf.cols, "spmm")
         st = lib().accspmm_spmm(self.handle, B.data_ptr(), B.stride(0),
                                 C_out.data_ptr(), C_out.stride(0), K, form,
                                 C.byref(tile) if tile is not None else None,
@@ -484,6 +512,7 @@ class DeviceGraph:
 
     def prepare_b(self, B, form: int, tile: Tile | None = None, stream=None):
         """First half of :meth:`spmm`: converts B for the BF16 forms."""
+        _dense(B, self.info.cols, "spmm_prepare_b", "B")
         st = lib().accspmm_spmm_prepare_b(self.handle, B.data_ptr(), B.stride(0),
                                           B.shape[1], form,
                                           C.byref(tile) if tile is not None else None,
@@ -492,6 +521,7 @@ class DeviceGraph:
 
     def spmm_prepared(self, B, C_out, form: int, tile: Tile | None = None, stream=None):
         """Second half of :meth:`spmm`: the SpMM kernel on the prepared B."""
+        _operands(B, C_out, self.info.rows, self.info.cols, "spmm_prepared")
         st = lib().accspmm_spmm_prepared(self.handle, B.data_ptr(), B.stride(0),
                                          C_out.data_ptr(), C_out.stride(0), B.shape[1],
                                          form, C.byref(tile) if tile is not None else None,
@@ -501,6 +531,7 @@ class DeviceGraph:
 
     def autotune(self, B, C_out, form: int = FORM_FP32, cache_path: str | None = None,
                  stream=None):
+        _operands(B, C_out, self.info.rows, self.info.cols, "autotune")
         t = Tile()
         ms = C.c_float()
         st = lib().accspmm_autotune(self.handle, B.data_ptr(), B.stride(0),
@@ -575,6 +606,8 @@ class Batch:
         self._keep = problems
         arr = (Problem * len(problems))()
         for i, (g, B, Cout) in enumerate(problems):
+            inf = g.info
+            _operands(B, Cout, inf.rows, inf.cols, f"batch problem {i}")
             arr[i] = Problem(g.handle.value, B.data_ptr(), B.stride(0), Cout.data_ptr(),
                              Cout.stride(0), B.shape[1])
         h = C.c_void_p()
@@ -626,6 +659,7 @@ class Server:
         _check(lib().accspmm_server_create(C.byref(o), C.byref(h)), "server_create")
         self.handle = h
         self._keep = {}
+        self._shapes = {}  # graph id -> (rows, cols), for operand checks
 
     def register(self, csr: CSR, opts: BuildOpts | None = None) -> int:
         """Builds and registers a resident graph (build options: row window,
@@ -639,6 +673,7 @@ class Server:
         _check(lib().accspmm_server_register_opts(self.handle, C.byref(view),
                                                   C.byref(opts) if opts is not None else None,
                                                   C.byref(gid)), "server_register")
+        self._shapes[gid.value] = (csr.rows, csr.cols)
         return gid.value
 
     def register_file(self, path) -> int:
@@ -646,12 +681,20 @@ class Server:
         gid = C.c_uint32()
         _check(lib().accspmm_server_register_file(self.handle, os.fsencode(path),
                                                   C.byref(gid)), "server_register_file")
+        # rows and cols: the first two header words after magic and version
+        with open(path, "rb") as f:
+            rows, cols = np.frombuffer(f.read(20)[12:20], "<u4").tolist()
+        self._shapes[gid.value] = (rows, cols)
         return gid.value
 
     def unregister(self, gid: int) -> None:
         _check(lib().accspmm_server_unregister(self.handle, gid), "server_unregister")
+        self._shapes.pop(gid, None)
 
     def submit(self, gid: int, B, C_out) -> int:
+        if gid in self._shapes:
+            rows, cols = self._shapes[gid]
+            _operands(B, C_out, rows, cols, "server_submit")
         rid = C.c_uint64()
         st = lib().accspmm_server_submit(self.handle, gid, B.data_ptr(), B.stride(0),
                                          C_out.data_ptr(), C_out.stride(0), B.shape[1],

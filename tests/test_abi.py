@@ -1,8 +1,11 @@
-"""The C-ABI library loads and exports every symbol declared in include/*.h
-(no compute calls: this runs without a GPU)."""
+"""The C-ABI library loads and exports every symbol declared in include/*.h;
+argument checks that fire before any device call (no compute calls: this
+runs without a GPU)."""
 import ctypes
 import re
 from pathlib import Path
+
+import numpy as np
 
 import accspmm_amd as acc
 
@@ -62,3 +65,13 @@ def test_oversized_nnz_rejected_before_device_work():
     assert L.accspmm_metcf_build(C.byref(view), None, C.byref(h), None) == \
         acc.Status.ERR_UNSUPPORTED
     assert not h.value
+
+
+def test_operand_checks_refuse_host_tensors():
+    # no GPU needed: host tensors and wrong shapes are refused in Python
+    import pytest
+    import torch
+    with pytest.raises(TypeError):
+        acc._operands(torch.zeros((4, 2)), torch.zeros((3, 2)), 3, 4, "spmm")
+    with pytest.raises(TypeError):
+        acc._dense(np.zeros((4, 2), np.float32), 4, "spmm", "B")

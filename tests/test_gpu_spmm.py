@@ -327,3 +327,35 @@ def test_dense_groups(form):
         assert gf.info.max_group_values == (512 if rw == 16 else 256)
         mx, fr, _ = run_case(full, 32, form, rw=rw, seed=5, graph=gf)
         assert mx <= TOL_HIGH_DEGREE[form], (rw, mx)
+
+
+def test_operand_checks():
+    # the wrappers refuse operands that would make the kernels read or write
+    # outside the caller's buffers, before any library call
+    torch = _torch()
+    rng = np.random.default_rng(96)
+    rp, ci, va = random_csr(rng, 50, 70, 0.1)
+    g = acc.DeviceGraph(acc.CSR(50, 70, rp, ci, va))
+    B = torch.zeros((70, 16), device="cuda")
+    Cm = torch.zeros((50, 16), device="cuda")
+    bad = [
+        (torch.zeros((69, 16), device="cuda"), Cm, ValueError),      # B short of rows
+        (B, torch.zeros((49, 16), device="cuda"), ValueError),       # C short of rows
+        (B, torch.zeros((50, 8), device="cuda"), ValueError),        # K mismatch
+        (torch.zeros((16, 70), device="cuda").t(), Cm, ValueError),  # column stride 16
+        (B.cpu(), Cm, TypeError),                                    # host memory
+        (B.double(), Cm, TypeError),                                 # float64
+        (B, Cm.cpu(), TypeError),
+    ]
+    for Bx, Cx, err in bad:
+        with pytest.raises(err):
+            g.spmm(Bx, Cx, acc.FORM_FP32)
+        with pytest.raises(err):
+            acc.Batch([(g, Bx, Cx)], acc.FORM_FP32)
+    srv = acc.Server(max_inflight=1)
+    gid = srv.register(acc.CSR(50, 70, rp, ci, va))
+    with pytest.raises(ValueError):
+        srv.submit(gid, torch.zeros((69, 16), device="cuda"), Cm)
+    srv.close()
+    # the checked call itself still runs
+    assert torch.equal(g.spmm(B, Cm, acc.FORM_FP32), torch.zeros((50, 16), device="cuda"))
