@@ -465,8 +465,9 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(row-window sharding is described in DESIGN.md); run with --gpus 1",
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(it would shard by row windows, each GPU holding its METCF slice and "
+              "the B rows it gathers); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
