@@ -52,7 +52,15 @@ std::string make_key(const accspmm_metcf* h, uint32_t K, int form) {
   return "f" + std::to_string(form) + "-k" + std::to_string(K) + "-" + h->hist_key;
 }
 
-void load_file(const char* path) {
+// An entry the kernels can launch (a cache file is outside input: others
+// are skipped, and the heuristic default serves their key)
+bool valid_tile(const accspmm_tile& t) {
+  const uint32_t nb = t.feat_per_lane;
+  return (nb == 1 || nb == 2 || nb == 4 || nb == 8) && t.waves_per_block <= 8 &&
+         t.xcd_slice <= 2 && t.sweep <= 1 && t.b_split <= 2 && t.col_panels <= 1024;
+}
+
+void load_file(const char* path, std::map<std::string, accspmm_tile>& into) {
   std::ifstream in(path);
   if (!in) return;
   std::string line;
@@ -67,23 +75,29 @@ void load_file(const char* path) {
     unsigned v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int got = std::sscanf(line.c_str() + l + 1, "%u , %u , %u , %u , %u , %u , %u , %u",
                                 &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7]);
-    if (got >= 5)  // older entries lack col_panels, sweep and b_split (all 0)
-      g_cache[key] = accspmm_tile{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+    const accspmm_tile t{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+    // older entries lack col_panels, sweep and b_split (all 0)
+    if (got >= 5 && valid_tile(t)) into[key] = t;
   }
 }
 
+// Writes the file's entries merged with this process's (which win), so
+// processes tuning different graphs into one file keep each other's picks.
 void save_file(const char* path) {
+  std::map<std::string, accspmm_tile> merged;
+  load_file(path, merged);
+  for (auto& kv : g_cache) merged[kv.first] = kv.second;
   std::ofstream out(path);
   if (!out) return;
   out << "{\n";
   size_t i = 0;
-  for (auto& kv : g_cache) {
+  for (auto& kv : merged) {
     out << "  \"" << kv.first << "\": [" << kv.second.feat_per_lane << ", "
         << kv.second.waves_per_block << ", " << kv.second.split_tiles << ", "
         << kv.second.xcd_slice << ", " << kv.second.pack_tiles << ", "
         << kv.second.col_panels << ", " << kv.second.sweep << ", " << kv.second.b_split
         << "]"
-        << (++i < g_cache.size() ? "," : "")
+        << (++i < merged.size() ? "," : "")
         << "\n";
   }
   out << "}\n";
@@ -93,7 +107,7 @@ void ensure_env() {
   if (g_env_loaded) return;
   g_env_loaded = true;
   const char* p = std::getenv("ACCSPMM_TUNE_CACHE");
-  if (p && *p) load_file(p);
+  if (p && *p) load_file(p, g_cache);
 }
 
 }  // namespace

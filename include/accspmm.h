@@ -232,8 +232,11 @@ accspmm_status accspmm_tile_select(const accspmm_metcf* h, uint32_t K,
 
 /* Autotuner: times each candidate tile configuration on (h, B, C, K, form)
  * on `stream`, stores the fastest under the handle's degree-histogram key
- * and K (in memory, and in the JSON file `cache_path` if non-NULL), and
- * returns it.  B and C are scratch operands of the right shape. */
+ * and K (in memory, and in the JSON file `cache_path` if non-NULL: the
+ * file's other entries are kept), and returns it.  B and C are scratch
+ * operands of the right shape.  A process reads the file named by the
+ * environment variable ACCSPMM_TUNE_CACHE at its first tile choice;
+ * entries that are not launchable tiles are ignored. */
 accspmm_status accspmm_autotune(const accspmm_metcf* h, const float* B,
                                 uint32_t ldb, float* C, uint32_t ldc,
                                 uint32_t K, accspmm_form form,

@@ -150,3 +150,33 @@ def test_register_file_warm_start(tmp_path):
     with pytest.raises(acc.AccSpMMError):
         srv.register_file(tmp_path / "missing.metcf")
     srv.close()
+
+
+def test_plain_register_is_the_automatic_layout():
+    # accspmm_server_register (no build options) serves the graph in the
+    # library's automatic layout: here rows average >= 128 nonzeros, so
+    # 16-row windows and reordered rows; C still equals that layout's
+    # standalone C, in input row order
+    import torch
+    from helpers import random_csr
+    rng = np.random.default_rng(92)
+    rp, ci, va = random_csr(rng, 600, 1500, 0.12, empty_rows=0.05, heavy_rows=2)
+    c = acc.CSR(600, 1500, rp, ci, va)
+    assert c.nnz >= 128 * c.rows
+    srv = acc.Server(form=acc.FORM_SPLIT_BF16)
+    rp_d, ci_d, va_d = c.to_device()
+    view = acc.CsrView(c.rows, c.cols, c.nnz, rp_d.data_ptr(), ci_d.data_ptr(),
+                       va_d.data_ptr())
+    gid = acc.C.c_uint32()
+    torch.cuda.synchronize()
+    assert acc.lib().accspmm_server_register(srv.handle, acc.C.byref(view),
+                                             acc.C.byref(gid)) == acc.Status.OK
+    B = torch.from_numpy(acc.gen_dense(13, c.cols, 32)).cuda()
+    Cm = torch.full((c.rows, 32), float("nan"), device="cuda")
+    rid = srv.submit(gid.value, B, Cm)
+    srv.drain()
+    assert srv.poll(rid) is not None
+    auto = acc.DeviceGraph(c, acc.default_build_opts(0, reorder_rows=2))
+    assert auto.info.row_window == 16 and auto.info.reordered == 1
+    assert torch.equal(Cm, auto.spmm(B, None, acc.FORM_SPLIT_BF16))
+    srv.close()

@@ -8,12 +8,17 @@ Tolerances (componentwise max-rel against |A||B|, FP64 oracle):
   leaves at most 2^-16 |x| of each operand, so one product can be off by
   2*2^-16 relative, which rows with one or two nonzeros expose.
 """
+import os
+from pathlib import Path
+
 import numpy as np
 import pytest
 
 import accspmm_amd as acc
 from helpers import random_csr
 from oracle import oracle as O
+
+REPO_TESTS = Path(__file__).resolve().parent
 
 pytestmark = pytest.mark.gpu
 
@@ -359,3 +364,74 @@ def test_operand_checks():
     srv.close()
     # the checked call itself still runs
     assert torch.equal(g.spmm(B, Cm, acc.FORM_FP32), torch.zeros((50, 16), device="cuda"))
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+def test_prepare_then_prepared_equals_spmm(form):
+    # the two halves (B conversion, then the kernel on the converted B) give
+    # spmm's C bits, with the library's tile and with explicit ones
+    torch = _torch()
+    rng = np.random.default_rng(93)
+    rp, ci, va = random_csr(rng, 800, 1100, 0.02, empty_rows=0.1, heavy_rows=3)
+    g = acc.DeviceGraph(acc.CSR(800, 1100, rp, ci, va))
+    bs = (0,) if form == acc.FORM_FP32 else (1, 2)
+    for K in (24, 128):
+        B = torch.from_numpy(acc.gen_dense(K + 1, 1100, K)).cuda()
+        tiles = [None] + [acc.Tile(2, 4, 16, 0, 64, 2, 0, b) for b in bs]
+        for tile in tiles:
+            ref = g.spmm(B, None, form, tile).clone()
+            Cm = torch.full((800, K), float("nan"), device="cuda")
+            g.prepare_b(B, form, tile)
+            g.spmm_prepared(B, Cm, form, tile)
+            torch.cuda.synchronize()
+            assert torch.equal(Cm, ref), (K, tile and tile.as_tuple())
+
+
+def test_autotune_cache_file(tmp_path):
+    # the tuner's pick runs within tolerance, becomes the library's tile for
+    # that key, and is written to the cache file beside the file's other
+    # entries; a fresh process reading the file through ACCSPMM_TUNE_CACHE
+    # takes the valid entry and ignores a malformed one
+    import json
+    import subprocess
+    import sys
+    torch = _torch()
+    rng = np.random.default_rng(91)
+    rp, ci, va = random_csr(rng, 900, 1200, 0.02, empty_rows=0.05, heavy_rows=2)
+    csr = acc.CSR(900, 1200, rp, ci, va)
+    g = acc.DeviceGraph(csr)
+    form, K = acc.FORM_SPLIT_BF16, 64
+    path = tmp_path / "tune.json"
+    path.write_text('{\n  "f9-k1-foreign": [2, 4, 0, 0, 64, 0, 0, 0]\n}\n')
+    Bh = acc.gen_dense(5, 1200, K)
+    B = torch.from_numpy(Bh).cuda()
+    Cm = torch.empty((900, K), device="cuda")
+    tile, ms = g.autotune(B, Cm, form, str(path))
+    assert ms > 0
+    assert g.tile_select(K, form).as_tuple() == tile.as_tuple()
+    mx, _, _ = run_case(csr, K, form, tile=tile, seed=5, graph=g, B=Bh)
+    assert mx <= TOL[form]
+    saved = json.loads(path.read_text())
+    key = g.tune_key(K, form)
+    assert saved[key] == list(tile.as_tuple()) and "f9-k1-foreign" in saved
+    # a malformed entry (NB = 3) for another K is ignored by a fresh process
+    saved[g.tune_key(32, form)] = [3, 4, 0, 0, 64, 0, 0, 0]
+    # the library's layout: one entry per line
+    path.write_text("{\n" + ",\n".join(f"  {json.dumps(k)}: {json.dumps(v)}"
+                                         for k, v in saved.items()) + "\n}\n")
+    code = (
+        "import sys; sys.path.insert(0, %r); import accspmm_amd as acc, numpy as np\n"
+        "from helpers import random_csr\n"
+        "rng = np.random.default_rng(91)\n"
+        "rp, ci, va = random_csr(rng, 900, 1200, 0.02, empty_rows=0.05, heavy_rows=2)\n"
+        "g = acc.DeviceGraph(acc.CSR(900, 1200, rp, ci, va))\n"
+        "print(list(g.tile_select(64, 1).as_tuple()), list(g.tile_select(32, 1).as_tuple()))\n"
+        % str(REPO_TESTS))
+    env = dict(os.environ, ACCSPMM_TUNE_CACHE=str(path))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       timeout=120, env=env, cwd=str(REPO_TESTS.parent))
+    assert r.returncode == 0, r.stderr
+    got64, got32 = json.loads("[" + r.stdout.strip().splitlines()[-1].replace("] [", "], [")
+                              + "]")
+    assert got64 == list(tile.as_tuple())
+    assert got32[0] in (1, 2, 4, 8) and got32 != [3, 4, 0, 0, 64, 0, 0, 0]
