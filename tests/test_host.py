@@ -194,3 +194,28 @@ def random_csr_small(rng):
     ci = np.array([1, 4, 0, 3, 5, 2], np.uint32)
     va = rng.uniform(-1, 1, 6).astype(np.float32)
     return rp, ci, va
+
+
+@pytest.mark.parametrize("sym", ["Hermitian", "skew-symmetric"])
+def test_read_mtx_hermitian_and_skew_as_the_reference(tmp_path, sym):
+    # the reference's reading (include/mmio.hpp:123,133-158,201): complex
+    # values keep their real part; Hermitian entries are mirrored like
+    # symmetric ones (without conjugation, the imaginary part being gone);
+    # skew-symmetric entries are neither mirrored nor negated
+    text = (f"%%MatrixMarket matrix coordinate complex {sym}\n3 3 3\n"
+            "1 1 2.5 9.0\n3 1 -1.5 4.0\n3 2 0.25 -7.0\n")
+    csr = acc.read_mtx(_write(tmp_path, "h.mtx", text))
+    if sym == "Hermitian":
+        dense = np.array([[2.5, 0, -1.5], [0, 0, 0.25], [-1.5, 0.25, 0]], np.float32)
+    else:
+        dense = np.array([[2.5, 0, 0], [0, 0, 0], [-1.5, 0.25, 0]], np.float32)
+    got = np.zeros((3, 3), np.float32)
+    for r in range(3):
+        for e in range(csr.row_ptr[r], csr.row_ptr[r + 1]):
+            got[r, csr.col_idx[e]] += csr.data[e]
+    assert np.array_equal(got, dense)
+    code, m, nn, rr, cc, vv = O.read_mtx(str(tmp_path / "h.mtx"))
+    assert code == 0
+    rp, ci, va = O.coo_to_csr(m, rr, cc, vv)
+    assert np.array_equal(csr.row_ptr, rp) and np.array_equal(csr.col_idx, ci)
+    assert np.array_equal(csr.data, va)
