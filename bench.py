@@ -37,6 +37,10 @@ WORKLOADS = {
     # 16,384-node Chung-Lu, exponent 2.1, mean degree 16, U(0,1] symmetric values
     "chung_lu_16k": dict(kind="chung_lu", n=16384, deg=16.0, gamma=2.1, seed=1,
                          values=0, bseed=2),
+    # one of the serving pool's two 2,000,000-node graphs: mean degree 32,
+    # exponent 2.1, U(0,1] values (graph seed 3100)
+    "chung_lu_2m": dict(kind="chung_lu", n=2000000, deg=32.0, gamma=2.1, seed=3100,
+                        values=0, bseed=3101),
 }
 
 
