@@ -606,6 +606,40 @@ accspmm_status build_impl(const accspmm_csr_view* csr,
 
 }  // namespace
 
+// Tiles of the METCF of a canonical CSR (already validated by a build) at
+// row window rw: the builder's steps 2-5 without the format arrays.
+static accspmm_status count_tiles(const accspmm_csr_view* c, uint32_t rw, hipStream_t s,
+                                  uint64_t* T) {
+  const uint32_t rows = c->rows, nnz = c->nnz;
+  const uint32_t W = (rows + rw - 1) / rw;
+  *T = 0;
+  if (nnz == 0) return ACCSPMM_OK;
+  const uint32_t* rp = c->row_ptr;
+  DevBuf pos, merged, incl, wt, off;
+  ACC_TRY(pos.alloc((uint64_t)nnz * 4));
+  ACC_TRY(merged.alloc((uint64_t)nnz * 4));
+  ACC_TRY(incl.alloc((uint64_t)nnz * 4));
+  ACC_TRY(wt.alloc(((uint64_t)W + 1) * 4));
+  ACC_TRY(off.alloc(((uint64_t)W + 1) * 4));
+  hipLaunchKernelGGL(k_merge_rank, dim3(blocks_for(nnz)), dim3(kThreads), 0, s, rp,
+                     c->col_idx, rows, nnz, rw, pos.as<uint32_t>(), merged.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_first_flags, dim3(blocks_for(nnz)), dim3(kThreads), 0, s, rp, rows,
+                     nnz, rw, pos.as<uint32_t>(), merged.as<uint32_t>(), incl.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  ACC_TRY(inclusive_scan_u32(incl.as<uint32_t>(), incl.as<uint32_t>(), nnz, s));
+  ACC_HIP(hipMemsetAsync(wt.p, 0, ((uint64_t)W + 1) * 4, s));
+  hipLaunchKernelGGL(k_window_tiles, dim3(blocks_for(W)), dim3(kThreads), 0, s, rp, rows, W,
+                     rw, incl.as<uint32_t>(), wt.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  ACC_TRY(exclusive_scan_u32(wt.as<uint32_t>(), off.as<uint32_t>(), (uint64_t)W + 1, s));
+  uint32_t t = 0;
+  ACC_HIP(hipMemcpyAsync(&t, off.as<uint32_t>() + W, 4, hipMemcpyDeviceToHost, s));
+  ACC_HIP(hipStreamSynchronize(s));
+  *T = t;
+  return ACCSPMM_OK;
+}
+
 extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
                                               const accspmm_build_opts* opts,
                                               accspmm_metcf** out,
@@ -627,6 +661,7 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
   // 3.74 -> 3.09 ms per split-BF16 step at K=128); on graphs with short
   // rows (2M nodes, mean 33; 500-50,000 nodes, means 4-64) both were slower
   const bool long_rows = (uint64_t)csr->nnz >= (uint64_t)kLongRows * csr->rows;
+  const bool auto_window = o.row_window == 0;
   if (o.row_window == 0) o.row_window = long_rows ? 16 : 8;
   if (o.reorder_rows == 2) o.reorder_rows = long_rows ? 1 : 0;
   if (!csr->row_ptr || (csr->nnz && (!csr->col_idx || !csr->val)))
@@ -646,6 +681,35 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
     src = &pv;
   }
   if (st == ACCSPMM_OK) st = build_impl(src, o, h, (hipStream_t)stream);
+  // automatic choice on short rows of a large graph (>= 64 * 8192 8-row
+  // tiles): 16-row windows when they hold at most 92% of the 8-row tiles,
+  // i.e. rows of a window share columns as in banded matrices (27-point
+  // stencils: 0.88-0.90; large random and power-law graphs: 0.95-0.99).
+  // 27-point stencil 100^3, split-BF16 at K=128: 0.591 ms against 0.674 ms;
+  // FP32 0.712 against 0.822 ms.  Small graphs keep 8-row windows (many
+  // small dense ones share columns too, unmeasured there).
+  if (st == ACCSPMM_OK && auto_window && o.row_window == 8 && !o.reorder_rows &&
+      (uint64_t)h->T >= 64ull * 8192) {
+    uint64_t t16 = 0;
+    st = count_tiles(csr, 16, (hipStream_t)stream, &t16);
+    if (st == ACCSPMM_OK && t16 * 100 <= (uint64_t)h->T * 92) {
+      auto* h16 = new (std::nothrow) accspmm_metcf();
+      if (!h16) {
+        st = ACCSPMM_ERR_OUT_OF_MEMORY;
+      } else {
+        accspmm_build_opts o16 = o;
+        o16.row_window = 16;
+        st = build_impl(csr, o16, h16, (hipStream_t)stream);
+        if (st == ACCSPMM_OK) {
+          delete h;
+          h = h16;
+        } else {
+          (void)hipStreamSynchronize((hipStream_t)stream);
+          delete h16;
+        }
+      }
+    }
+  }
   if (st != ACCSPMM_OK) {
     (void)hipStreamSynchronize((hipStream_t)stream);
     delete h;

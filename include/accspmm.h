@@ -62,7 +62,9 @@ typedef struct accspmm_csr_view {
  * cost model (src/mma_tf32.cu:222). */
 typedef struct accspmm_build_opts {
   uint32_t row_window;          /* 8 or 16 (16 => 128-bit tiles); 0: 16 when
-                                   rows average >= 128 nonzeros, else 8     */
+                                   rows average >= 128 nonzeros, or when a
+                                   graph of >= 64*8192 8-row tiles has <= 92%
+                                   as many 16-row ones (banded), else 8     */
   uint32_t col_window;          /* must be 8                                */
   uint32_t t_load, t_comp, t_write, target_num; /* AdpBME cost model        */
   uint32_t imbalance_threshold; /* balanceStrategy threshold                */

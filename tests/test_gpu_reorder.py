@@ -133,14 +133,25 @@ def test_reorder_rejects_bad_row_ptr():
 
 def test_automatic_choices():
     # row_window 0 and reorder_rows 2: 16-row windows and reordered rows for
-    # rows averaging >= 128 nonzeros, the reference's layout otherwise; the
-    # result is the same as asking for that layout explicitly
+    # rows averaging >= 128 nonzeros; for shorter rows, 16-row windows in
+    # input order when a graph of >= 64 * 8192 8-row tiles has at most 92%
+    # as many 16-row ones (banded: the 27-point stencil), the reference's
+    # layout otherwise; the result
+    # is the same as asking for that layout explicitly
     torch = _torch()
     rng = np.random.default_rng(12)
     long_rp, long_ci, long_va = random_csr(rng, 300, 2000, 0.08)      # ~160 per row
     short = acc.gen_chung_lu(5000, 20.0, 2.1, 9, 0)
-    for csr, want in ((acc.CSR(300, 2000, long_rp, long_ci, long_va), (16, 1)), (short, (8, 0))):
+    banded = acc.gen_stencil27(82, 13)      # 810,286 8-row tiles: large enough
+    cases = ((acc.CSR(300, 2000, long_rp, long_ci, long_va), (16, 1)), (short, (8, 0)),
+             (banded, (16, 0)))
+    for csr, want in cases:
         assert (csr.nnz >= 128 * csr.rows) == (want == (16, 1))
+        if want != (16, 1):
+            # the tile ratio decides, as the oracle's builder counts it
+            t8, t16 = (len(O.metcf_build(rw, csr.rows, csr.row_ptr, csr.col_idx,
+                                         csr.data)["tcOffset"]) - 1 for rw in (8, 16))
+            assert (t16 * 100 <= t8 * 92 and t8 >= 64 * 8192) == (want == (16, 0)), (t8, t16)
         g = acc.DeviceGraph(csr, acc.default_build_opts(0, reorder_rows=2))
         assert (g.info.row_window, g.info.reordered) == want
         ge = acc.DeviceGraph(csr, acc.default_build_opts(want[0], reorder_rows=want[1]))
