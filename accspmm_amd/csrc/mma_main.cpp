@@ -20,8 +20,9 @@
 //   --layout reference|auto       the graph the launches use: the reference's
 //                                 METCF (default) or the library's automatic
 //                                 layout (16-row windows and rows reordered by
-//                                 length when rows average >= 128 nonzeros);
-//                                 C is in input row order either way
+//                                 length when rows average >= 128 nonzeros;
+//                                 16-row windows alone for large banded
+//                                 graphs); C is in input row order either way
 // The CSR is canonicalised on load (rows sorted by column, duplicates
 // summed), which the METCF tile layout requires.
 #include <hip/hip_runtime.h>
