@@ -6,13 +6,14 @@
 // bit-identical alone and inside a batch.  The task space is implicit: task
 // t belongs to problem p with prefix[p] <= t < prefix[p+1] and is unit
 // (t - prefix[p]) / n_kc, feature chunk (t - prefix[p]) % n_kc of it.
-// A run is four kernels on one stream: reset of the queue head, one
-// conversion pass over every problem's B (BF16 forms), the persistent kernel
-// whose waves pop tasks from the atomic head until it passes the last task
-// (every wave reaches that test, so the grid always drains), and one
-// reduction block per split window of any problem, which sums its piece
-// partials in slot order (as the single-graph call does).  Problems are
-// ordered largest first.
+// A run is a few kernels on one stream: reset of the queue heads, one
+// conversion pass over every problem's B (BF16 form), one persistent kernel
+// per NB class of problems whose waves pop tasks from the class's atomic
+// head until it passes the class's last task (every wave reaches that test,
+// so the grid always drains), and one reduction block per split window of
+// any problem, which sums its piece partials in slot order (as the
+// single-graph call does).  Problems are ordered by NB, widest first, then
+// largest first.
 //
 // BatchLauncher is shared with the serving loop: planning is host
 // arithmetic over cached per-graph state, uploads go through pinned staging
@@ -64,11 +65,10 @@ __device__ inline uint32_t last_le(const uint32_t* __restrict__ a, uint32_t n,
   return lo;
 }
 
-// Every problem's B conversion in one launch: the blocks of problem p
-// convert its chunks [(b - prefix[p]) * kConvChunks, ...) of the
+// Every problem's B conversion in one launch (BF16 form): the blocks of
+// problem p convert its chunks [(b - prefix[p]) * kConvChunks, ...) of the
 // (rows + 1) * ldx / 4 (convert_chunk, the single-graph conversion's code,
 // so the converted bits are the same).
-template <int FORM>
 __global__ __launch_bounds__(256) void k_batch_convert(
     const ConvDesc* __restrict__ d, uint32_t nconv,
     const uint32_t* __restrict__ prefix) {
@@ -78,15 +78,20 @@ __global__ __launch_bounds__(256) void k_batch_convert(
   const uint64_t c0 = (uint64_t)(blockIdx.x - prefix[p]) * kConvChunks;
   const uint64_t c1 = min<uint64_t>(c0 + kConvChunks, total);
   for (uint64_t ch = c0 + threadIdx.x; ch < c1; ch += blockDim.x)
-    convert_chunk<FORM>(c.src, c.rows, c.K, c.ldb, c.dst, c.ldx, c.vec, ch);
+    convert_chunk(c.src, c.rows, c.K, c.ldb, c.dst, c.ldx, c.vec, ch);
 }
 
-template <int FORM, int RW>
+// The persistent kernel of one (form, row window, NB) class: problems
+// [p0, p1) of the batch (consecutive: the batch orders its problems by NB),
+// whose tasks are prefix[p0] .. prefix[p1] - 1.  One NB per kernel keeps the
+// register budget of the standalone kernel (three NB variants in one body
+// spilled).
+template <int FORM, int RW, int NB>
 __global__ __launch_bounds__(256) void k_batch_spmm(
     const SpmmArgs* __restrict__ descs, const uint32_t* __restrict__ prefix,
-    uint32_t nprob, uint32_t* __restrict__ head) {
+    uint32_t p0, uint32_t p1, uint32_t* __restrict__ head) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t total = prefix[nprob];
+  const uint32_t first = prefix[p0], total = prefix[p1] - first;
   __shared__ __align__(16) float lds_vals[4 * wave_lds<RW>()];
   float* lv = lds_vals + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * wave_lds<RW>();
   if (lane == 0) lv[group_values<RW>()] = 0.f;
@@ -95,20 +100,26 @@ __global__ __launch_bounds__(256) void k_batch_spmm(
     if (lane == 0) t = atomicAdd(head, 1u);
     t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
     if (t >= total) return;
-    const uint32_t p = last_le(prefix, nprob, t);
+    t += first;
+    const uint32_t p = p0 + last_le(prefix + p0, p1 - p0, t);
     const SpmmArgs g = descs[p];
     const uint32_t local = t - prefix[p];
     const WorkUnit u = g.units[local / g.n_kc];
     const uint32_t kc = local % g.n_kc;
-    // one row-window size per kernel keeps the register budget of the
-    // widest variant (NB=4) without spills
-    switch (g.nb) {
-      case 1: run_unit<FORM, 1, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv); break;
-      case 2: run_unit<FORM, 2, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv); break;
-      case 4: run_unit<FORM, 4, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv); break;
-      default: break;
-    }
+    run_unit<FORM, NB, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv);
   }
+}
+
+template <int FORM, int RW>
+void launch_batch_class(uint32_t nb, dim3 grid, hipStream_t s, const SpmmArgs* descs,
+                        const uint32_t* prefix, uint32_t p0, uint32_t p1, uint32_t* head) {
+  const dim3 blk(256);
+  if (nb == 4)
+    hipLaunchKernelGGL((k_batch_spmm<FORM, RW, 4>), grid, blk, 0, s, descs, prefix, p0, p1, head);
+  else if (nb == 2)
+    hipLaunchKernelGGL((k_batch_spmm<FORM, RW, 2>), grid, blk, 0, s, descs, prefix, p0, p1, head);
+  else
+    hipLaunchKernelGGL((k_batch_spmm<FORM, RW, 1>), grid, blk, 0, s, descs, prefix, p0, p1, head);
 }
 
 template <int RW>
@@ -149,15 +160,12 @@ accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
   {
     std::lock_guard<std::mutex> lock(h->mu);
     ACC_TRY(get_schedule(h, t, s, &out->sched));
-    if (form != 0) ACC_TRY(ensure_vsplit(h, s));
   }
   out->tasks = (uint64_t)out->sched->num_units * out->n_kc;
-  out->b_inreg = form == 1 && t.b_split == 2;
   out->wpb = t.waves_per_block ? t.waves_per_block : 4;
   out->xcd = t.xcd_slice;
   out->bconv_bytes =
-      form == 0 ? 0
-                : align256(((uint64_t)h->cols + 1) * out->kc_padded * (form == 1 ? 4 : 2));
+      form != kBF16 ? 0 : align256(((uint64_t)h->cols + 1) * out->kc_padded * 2);
   out->partial_bytes =
       align256((uint64_t)out->sched->num_pieces * h->rw * out->kc_padded * 4);
   return ACCSPMM_OK;
@@ -192,10 +200,12 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
   for (uint32_t i = 1; i < n; ++i)
     if (P[i].graph->rw != rw0) return ACCSPMM_ERR_UNSUPPORTED;
   rw = rw0;
-  // largest problems first, so heavy work starts early
+  // one persistent kernel per NB class (widest first); largest problems
+  // first within a class, so heavy work starts early
   std::vector<uint32_t> order(n);
   std::iota(order.begin(), order.end(), 0u);
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    if (plans[a].nb != plans[b].nb) return plans[a].nb > plans[b].nb;
     return P[a].graph->T * (uint64_t)P[a].K > P[b].graph->T * (uint64_t)P[b].K;
   });
   // workspace layout: [partials + converted B per problem]
@@ -222,6 +232,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
   ACC_TRY(dev_meta.reserve(stage));
   ACC_TRY(arena.reserve(ws));
   ACC_TRY(head.reserve(256));
+  nclass = 0;
   char* hb = static_cast<char*>(pinned);
   auto* D = reinterpret_cast<dev::SpmmArgs*>(hb);
   auto* PF = reinterpret_cast<uint32_t*>(hb + off_p);
@@ -243,7 +254,6 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     g.bits = h->bits.as<unsigned long long>();
     g.a2b = h->a2b.as<uint32_t>();
     g.data = h->data.as<float>();
-    g.vsplit = h->vsplit.as<uint32_t>();
     g.units = q.sched->units.as<WorkUnit>();
     g.split_info = q.sched->split_info.as<uint32_t>();
     g.C = p.C;
@@ -261,7 +271,10 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     g.wide = h->rw == 16 && h->max_group_values > dev::kSpanValues ? 1u : 0u;
     g.partials = reinterpret_cast<float*>(A + bc);
     bc += q.partial_bytes;
-    single = n == 1 && q.b_inreg;
+    // a batch of one split-BF16 problem (the server's large requests) runs
+    // the single-graph kernel, which keeps the standalone call's register
+    // budget
+    single = n == 1 && form == kSPLIT;
     if (single) {
       // the single-graph kernel with the batch's own workspace
       g.xcd_slice = q.xcd == 2 ? 2u : (q.xcd == 1 && q.n_kc >= 2 && (8 % q.n_kc) == 0) ? 1u : 0u;
@@ -274,7 +287,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
       single_nkc = q.n_kc;
       single_packed = q.sched->packed;
     }
-    if (form == 0 || single) {
+    if (form != kBF16) {
       g.B = p.B;
       g.ldb = p.ldb;
     } else {
@@ -292,6 +305,13 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
       cblocks += (uint32_t)nblk;
       ++nconv;
     }
+    if (nclass == 0 || cls[nclass - 1].nb != q.nb) {
+      cls[nclass] = BatchClass{q.nb, j, j + 1, 0};
+      ++nclass;
+    } else {
+      cls[nclass - 1].p1 = j + 1;
+    }
+    cls[nclass - 1].tasks += q.tasks;
     PF[j] = run;
     run += (uint32_t)q.tasks;
     SP[j] = nsplit;
@@ -316,15 +336,17 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
   int dv = 0, cus = 0;
   ACC_HIP(hipGetDevice(&dv));
   ACC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dv));
-  const uint64_t want = (total_tasks + 3) / 4;
-  grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)cus * 4));
+  for (uint32_t c = 0; c < nclass; ++c) {
+    const uint64_t want = (cls[c].tasks + 3) / 4;
+    cls[c].grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)cus * 4));
+  }
   return ACCSPMM_OK;
 }
 
 accspmm_status BatchLauncher::launch(hipStream_t s) const {
   if (nprob == 0) return ACCSPMM_OK;
   if (single) {
-    ACC_TRY(launch_single(single_g, kSPLITF, rw, single_nb, single_wpb, single_packed,
+    ACC_TRY(launch_single(single_g, kSPLIT, rw, single_nb, single_wpb, single_packed,
                           single_units, single_nkc, s));
     if (total_split) {
       const dim3 blk(256);
@@ -339,30 +361,31 @@ accspmm_status BatchLauncher::launch(hipStream_t s) const {
     return ACCSPMM_OK;
   }
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, head.as<uint32_t>(),
-                     (uint64_t)1, 0u);
+                     (uint64_t)kMaxBatchClasses, 0u);
   ACC_HIP(hipGetLastError());
-  if (form != 0 && conv_blocks) {
-    if (form == 1)
-      hipLaunchKernelGGL(k_batch_convert<1>, dim3(conv_blocks), dim3(256), 0, s,
-                         static_cast<const ConvDesc*>(conv), num_conv, conv_prefix);
-    else
-      hipLaunchKernelGGL(k_batch_convert<2>, dim3(conv_blocks), dim3(256), 0, s,
-                         static_cast<const ConvDesc*>(conv), num_conv, conv_prefix);
+  if (form == kBF16 && conv_blocks) {
+    hipLaunchKernelGGL(k_batch_convert, dim3(conv_blocks), dim3(256), 0, s,
+                       static_cast<const ConvDesc*>(conv), num_conv, conv_prefix);
     ACC_HIP(hipGetLastError());
   }
   if (total_tasks == 0) return ACCSPMM_OK;
   auto* H = head.as<uint32_t>();
-  const dim3 g(grid), blk(256);
-  if (rw == 8) {
-    if (form == 0) hipLaunchKernelGGL((k_batch_spmm<0, 8>), g, blk, 0, s, descs, prefix, nprob, H);
-    else if (form == 1) hipLaunchKernelGGL((k_batch_spmm<1, 8>), g, blk, 0, s, descs, prefix, nprob, H);
-    else hipLaunchKernelGGL((k_batch_spmm<2, 8>), g, blk, 0, s, descs, prefix, nprob, H);
-  } else {
-    if (form == 0) hipLaunchKernelGGL((k_batch_spmm<0, 16>), g, blk, 0, s, descs, prefix, nprob, H);
-    else if (form == 1) hipLaunchKernelGGL((k_batch_spmm<1, 16>), g, blk, 0, s, descs, prefix, nprob, H);
-    else hipLaunchKernelGGL((k_batch_spmm<2, 16>), g, blk, 0, s, descs, prefix, nprob, H);
+  const dim3 blk(256);
+  for (uint32_t c = 0; c < nclass; ++c) {
+    const BatchClass& k = cls[c];
+    if (k.tasks == 0) continue;
+    const dim3 g(k.grid);
+    if (rw == 8) {
+      if (form == kFP32) launch_batch_class<kFP32, 8>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
+      else if (form == kSPLIT) launch_batch_class<kSPLIT, 8>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
+      else launch_batch_class<kBF16, 8>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
+    } else {
+      if (form == kFP32) launch_batch_class<kFP32, 16>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
+      else if (form == kSPLIT) launch_batch_class<kSPLIT, 16>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
+      else launch_batch_class<kBF16, 16>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
+    }
+    ACC_HIP(hipGetLastError());
   }
-  ACC_HIP(hipGetLastError());
   if (total_split) {
     if (rw == 8)
       hipLaunchKernelGGL(k_batch_reduce<8>, dim3(total_split), blk, 0, s, descs,

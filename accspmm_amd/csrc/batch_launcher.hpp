@@ -12,10 +12,8 @@ struct ProblemPlan {
   uint32_t nb = 1, n_kc = 1, kc_padded = 16;
   const Schedule* sched = nullptr;
   uint64_t tasks = 0;          // units x feature chunks
-  uint64_t bconv_bytes = 0;    // converted B (BF16 forms)
-  // the single-graph call's choices: B split in registers (split form),
-  // waves per block and XCD mapping
-  bool b_inreg = false;
+  uint64_t bconv_bytes = 0;    // converted B (BF16 form)
+  // the single-graph call's choices: waves per block and XCD mapping
   uint32_t wpb = 4, xcd = 0;
   uint64_t partial_bytes = 0;  // split-window partial sums
   uint64_t workspace_bytes() const;
@@ -23,6 +21,16 @@ struct ProblemPlan {
 
 accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
                             ProblemPlan* out);
+
+// Problems of one NB, consecutive in the batch's order: one persistent
+// kernel each.
+struct BatchClass {
+  uint32_t nb, p0, p1;
+  uint64_t tasks;
+  uint32_t grid = 1;
+};
+// NB 4, 2 and 1 (the persistent kernels carry NB <= 4)
+constexpr uint32_t kMaxBatchClasses = 3;
 
 // Device buffers and pinned staging for one stream's batched launches.
 // Buffers only grow; prepare() uploads asynchronously on the stream and
@@ -43,9 +51,10 @@ struct BatchLauncher {
   uint32_t* conv_prefix = nullptr;
   uint32_t* split_prefix = nullptr;  // split windows before each problem
   uint64_t total_tasks = 0;
-  uint32_t nprob = 0, num_conv = 0, conv_blocks = 0, grid = 0;
-  // a batch of one split-form problem whose B is split in registers runs
-  // the single-graph kernel (its persistent-kernel form spills registers)
+  uint32_t nprob = 0, num_conv = 0, conv_blocks = 0;
+  BatchClass cls[kMaxBatchClasses] = {};
+  uint32_t nclass = 0;
+  // a batch of one split-form problem runs the single-graph kernel
   bool single = false;
   dev::SpmmArgs single_g{};
   uint32_t single_nb = 1, single_wpb = 4, single_units = 0, single_nkc = 1;

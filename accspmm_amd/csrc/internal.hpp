@@ -102,6 +102,16 @@ struct Schedule {
   std::vector<WorkUnit> h_units;  // host copy (the sweep plan is derived from it)
 };
 
+// Workspace of one accspmm_spmm call: split-window partials and the BF16
+// form's converted B.
+struct StreamWorkspace {
+  DevBuf partials, bconv;
+};
+struct SpmmWorkspace {
+  float* partials = nullptr;
+  void* bconv = nullptr;
+};
+
 // Plan of the resident-window sweep (sweep.hip): the schedule's pieces
 // regrouped per wave.  Wave v of round r owns up to M windows (their running
 // sums live in LDS) and runs its pieces in (panel, window, slot) order.
@@ -139,14 +149,12 @@ struct accspmm_metcf {
   mutable std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>,
                    std::unique_ptr<accspmm::SweepPlan>>
       sweep_plans;
-  mutable accspmm::DevBuf bconv;    // converted-B workspace (BF16 forms)
-  mutable accspmm::DevBuf partials; // split-window partial sums
-  // A values as bf16 hi | lo << 16 (BF16 forms), kValuePad zeros after;
-  // built on first use, counted in device_bytes from then on
-  mutable accspmm::DevBuf vsplit;
+  // per-stream workspaces of accspmm_spmm (split partials, converted B):
+  // calls on one stream share one, calls on different streams never do
+  mutable std::map<hipStream_t, std::unique_ptr<accspmm::StreamWorkspace>> stream_ws;
   uint64_t device_bytes() const {
     return rwo.bytes + tco.bytes + bits.bytes + a2b.bytes + data.bytes +
-           grp.bytes + rix.bytes + vsplit.bytes + row_perm.bytes;
+           grp.bytes + rix.bytes + row_perm.bytes;
   }
 };
 
@@ -164,8 +172,18 @@ accspmm_status inclusive_scan_u32(const uint32_t* in, uint32_t* out,
 // spmm.hip
 accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
                             hipStream_t s, const Schedule** out);
-// builds h->vsplit if absent (caller holds h->mu)
-accspmm_status ensure_vsplit(const accspmm_metcf* h, hipStream_t s);
+// one SpMM call (spmm.hip).  phases: bit 0 converts B (BF16 form), bit 1
+// runs the kernels; ws: the call's workspace, or null for the handle's
+// workspace of stream s
+struct SpmmWorkspace;
+accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
+                           float* C, uint32_t ldc, uint32_t K, int form,
+                           const accspmm_tile& tile, hipStream_t s, int phases,
+                           const SpmmWorkspace* ws);
+// workspace bytes of one call with this tile and schedule (spmm.hip)
+void spmm_workspace_need(const accspmm_metcf* h, const Schedule* sch, uint32_t K, int form,
+                         const accspmm_tile& tile, uint64_t* partial_bytes,
+                         uint64_t* bconv_bytes);
 // sets h->max_group_values (builder.hip); after a build or a load
 accspmm_status measure_group_values(accspmm_metcf* h, hipStream_t s);
 }  // namespace accspmm

@@ -208,9 +208,7 @@ int main(int argc, char** argv) {
       }
     }
     const double fro = ref2 > 0 ? std::sqrt(err2 / ref2) : std::sqrt(err2);
-    const double tol = form == ACCSPMM_FORM_BF16 ? 1e-2
-                       : form == ACCSPMM_FORM_SPLIT_BF16 ? std::ldexp(1.0, -15) + 1e-6
-                                                         : 1e-5;
+    const double tol = form == ACCSPMM_FORM_BF16 ? 1e-2 : 1e-5;
     std::printf("check: max_rel=%.3e fro_rel=%.3e tol=%.1e %s\n", max_rel, fro, tol,
                 max_rel <= tol ? "ok" : "FAILED");
     if (!(max_rel <= tol)) rc = 6;

@@ -34,9 +34,6 @@
 #include "batch_launcher.hpp"
 
 namespace accspmm {
-accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
-                           float* C, uint32_t ldc, uint32_t K, int form,
-                           const accspmm_tile& tile, hipStream_t s, int phases);
 accspmm_status spmm_fit_tile(const accspmm_metcf* h, const float* B,
                              uint32_t ldb, float* C, uint32_t ldc, uint32_t K,
                              int form, const accspmm_tile* tile,
@@ -187,7 +184,7 @@ bool accspmm_server::admit() {
       st = spmm_fit_tile(p.graph, p.B, p.ldb, p.C, p.ldc, p.K, (int)o.form, nullptr, &t);
       if (st == ACCSPMM_OK)
         st = spmm_launch(p.graph, p.B, p.ldb, p.C, p.ldc, p.K, (int)o.form, t,
-                         sl.stream, 3);
+                         sl.stream, 3, nullptr);
     } else {
       // large requests too go through the slot's own launcher (a batch of
       // one): its workspace is the slot's, so two requests for one graph
@@ -283,8 +280,6 @@ accspmm_status register_handle(accspmm_server* s, accspmm_metcf* h, uint32_t* gr
     std::lock_guard<std::mutex> hl(h->mu);
     const Schedule* sch = nullptr;
     st = get_schedule(h, t, s->setup, &sch);
-    // the BF16 forms' split values are part of the resident graph
-    if (st == ACCSPMM_OK && s->o.form != ACCSPMM_FORM_FP32) st = ensure_vsplit(h, s->setup);
   }
   if (st != ACCSPMM_OK) {
     accspmm_metcf_destroy(h);

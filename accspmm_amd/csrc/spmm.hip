@@ -17,14 +17,16 @@
 // g+1 are issued before the MFMAs of group g (register double buffer).
 //
 // MFMA mapping (D[m][n] += sum_k A[m][k] * B[k][n], 16x16 output):
-//   rows m     = tile rows (RW=8: rows 0-7; 8-15 carry A_lo in the split
-//                form and are zero otherwise)
+//   rows m     = tile rows (RW=8: rows 0-7; 8-15 carry a second A term in
+//                the split form and are zero otherwise)
 //   columns n  = feature NB*n + b for block b in [0, NB)
 //   reduction  = the 32 condensed columns of the 4-tile group
 // FP32 form:   v_mfma_f32_16x16x4_f32, k = tile q, one MFMA per column slot.
 // BF16 forms:  v_mfma_f32_16x16x32_bf16, k = 8q + column slot.
-// Split-BF16:  A = [A_hi; A_lo] against B_hi and B_lo (2 MFMAs per block)
-//              gives all four products; rows r and r+8 are added at the end.
+// Split-BF16:  A and B split exactly into three bf16 terms in registers
+//              (bf16x6): RW=8 [A0; A1] against B0, B1, B2 and [A2; 0]
+//              against B0 (4 MFMAs per block; rows r and r+8 are added at
+//              the end), RW=16 the six products A_i B_j, i + j <= 2.
 //
 // Split windows: a window with more than `split_tiles` tiles is cut into
 // balanced pieces; each piece writes a partial, and k_reduce_split sums a
@@ -80,36 +82,16 @@ __global__ __launch_bounds__(512) void spmm_kernel(SpmmArgs g) {
   run_unit<FORM, NB, RW, PACKED>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv);
 }
 
-// B conversion for the BF16 forms: zero-padded to ldx >= K columns, plus
+// B conversion for the BF16 form: zero-padded to ldx >= K columns, plus
 // one all-zero row N that padded tile columns read: grid-stride over the
 // (N + 1) * ldx / 4 four-feature chunks (convert_chunk).
-template <int FORM>
 __global__ __launch_bounds__(256) void convert_b(const float* __restrict__ B, uint32_t N,
                                                  uint32_t K, uint32_t ldb, void* __restrict__ out,
                                                  uint32_t ldx, uint32_t vec) {
   const uint64_t total = (uint64_t)(N + 1) * (ldx / 4);
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total;
        c += (uint64_t)gridDim.x * blockDim.x)
-    convert_chunk<FORM>(B, N, K, ldb, out, ldx, vec, c);
-}
-
-// The METCF values as bf16 hi | lo << 16 (round to nearest even, lo = the
-// rounded remainder), the same split the kernels apply to B, followed by
-// kValuePad zeros.
-__global__ void k_split_values(const float* __restrict__ data, uint32_t nnz,
-                               uint32_t* __restrict__ out) {
-  const uint64_t n = (uint64_t)nnz + kValuePad;
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t v = 0;
-    if (e < nnz) {
-      const float x = data[e];
-      const uint32_t hp = pack_bf16(x, 0.f) & 0xFFFFu;
-      const uint32_t lp = pack_bf16(x - bf16_lo_to_f32(hp), 0.f) & 0xFFFFu;
-      v = hp | (lp << 16);
-    }
-    out[e] = v;
-  }
+    convert_chunk(B, N, K, ldb, out, ldx, vec, c);
 }
 
 // For window w and panel boundary p in [1, P): the number of the window's
@@ -330,12 +312,10 @@ accspmm_status launch_form(int form, uint32_t rw, uint32_t nb,
   if (rw == 8) {
     if (form == kFP32) return launch_nb<kFP32, 8>(nb, g, blocks, wpb, packed, s);
     if (form == kSPLIT) return launch_nb<kSPLIT, 8>(nb, g, blocks, wpb, packed, s);
-    if (form == kSPLITF) return launch_nb<kSPLITF, 8>(nb, g, blocks, wpb, packed, s);
     return launch_nb<kBF16, 8>(nb, g, blocks, wpb, packed, s);
   }
   if (form == kFP32) return launch_nb<kFP32, 16>(nb, g, blocks, wpb, packed, s);
   if (form == kSPLIT) return launch_nb<kSPLIT, 16>(nb, g, blocks, wpb, packed, s);
-  if (form == kSPLITF) return launch_nb<kSPLITF, 16>(nb, g, blocks, wpb, packed, s);
   return launch_nb<kBF16, 16>(nb, g, blocks, wpb, packed, s);
 }
 
@@ -409,31 +389,6 @@ void default_tile(const accspmm_metcf* h, uint32_t K, int form,
   }
 }
 
-// Split-BF16 B path when the tile leaves it open.  The conversion pass costs
-// one read and one write of B; splitting in registers costs a few VALU
-// operations per gathered element instead, i.e. per tile column.  With few
-// gathers per B row (banded matrices: ~14 on the 27-point stencil) the
-// in-register split wins (0.84 -> 0.67 ms per step at 100^3, K=128); with
-// many (~380 on the 232,965-node power-law graph) the conversion pass does
-// (3.08 vs 3.28 ms).  Threshold: 64 gathered tile columns per B row.
-uint32_t default_b_split(const accspmm_metcf* h, uint32_t K) {
-  (void)K;  // both costs scale with K
-  return (uint64_t)h->T * kColWindow < 64ull * h->cols ? 2u : 1u;
-}
-
-accspmm_status ensure_vsplit(const accspmm_metcf* h, hipStream_t s) {
-  if (h->vsplit.p) return ACCSPMM_OK;
-  const uint64_t n = (uint64_t)h->nnz + kValuePad;
-  ACC_TRY(h->vsplit.alloc(n * 4));
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_split_values, dim3(blocks), dim3(256), 0, s, h->data.as<float>(),
-                     h->nnz, h->vsplit.as<uint32_t>());
-  ACC_HIP(hipGetLastError());
-  // other streams (batched and served launches) read it next
-  ACC_HIP(hipStreamSynchronize(s));
-  return ACCSPMM_OK;
-}
-
 // The single-graph SpMM kernel over g's work units: blocks of wpb waves,
 // mapped to XCDs as g.xcd_slice says (multiples of 8 blocks for modes 1, 2).
 accspmm_status launch_single(const SpmmArgs& g, int form, uint32_t rw, uint32_t nb,
@@ -461,22 +416,49 @@ accspmm_status sweep_launch(const accspmm_metcf* h, const dev::SpmmArgs& g,
                             const accspmm_tile& tile, uint32_t nb, int form,
                             const Schedule* sch, hipStream_t s);
 
-// phases: bit 0 = convert B (BF16 forms), bit 1 = SpMM kernel
+// Workspace bytes of one call: the split-window partials (pieces x RW x
+// KC-padded K floats) and, for the BF16 form, the converted B ((cols + 1)
+// rows of KC-padded K bf16, the last all zero), each rounded up to 256.
+void spmm_workspace_need(const accspmm_metcf* h, const Schedule* sch, uint32_t K, int form,
+                         const accspmm_tile& tile, uint64_t* partial_bytes,
+                         uint64_t* bconv_bytes) {
+  const uint32_t KC = 16 * tile.feat_per_lane;
+  const uint64_t kcp = (uint64_t)((K + KC - 1) / KC) * KC;
+  const uint64_t pb = tile.sweep ? 0 : (uint64_t)sch->num_pieces * h->rw * kcp * 4;
+  const uint64_t cb = form == kBF16 ? ((uint64_t)h->cols + 1) * kcp * 2 : 0;
+  *partial_bytes = (pb + 255) / 256 * 256;
+  *bconv_bytes = (cb + 255) / 256 * 256;
+}
+
+// The handle's workspace for calls on stream s (allocated on first use,
+// grown as needed; caller holds h->mu).  Calls on one stream are ordered,
+// so they may share it; calls on different streams never do.
+accspmm_status stream_workspace(const accspmm_metcf* h, hipStream_t s, uint64_t partial_bytes,
+                                uint64_t bconv_bytes, SpmmWorkspace* out) {
+  std::unique_ptr<StreamWorkspace>& w = h->stream_ws[s];
+  if (!w) w = std::make_unique<StreamWorkspace>();
+  if (partial_bytes) ACC_TRY(w->partials.reserve(partial_bytes));
+  if (bconv_bytes) ACC_TRY(w->bconv.reserve(bconv_bytes));
+  out->partials = w->partials.as<float>();
+  out->bconv = w->bconv.p;
+  return ACCSPMM_OK;
+}
+
+// phases: bit 0 = convert B (BF16 form), bit 1 = SpMM kernel.  ws: the
+// call's workspace (at least spmm_workspace_need's bytes), or null for the
+// handle's workspace of stream s.
 accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
                            float* C, uint32_t ldc, uint32_t K, int form,
                            const accspmm_tile& tile, hipStream_t s,
-                           int phases) {
+                           int phases, const SpmmWorkspace* ws) {
   uint32_t nb = tile.feat_per_lane;
   if (nb != 1 && nb != 2 && nb != 4 && nb != 8) return ACCSPMM_ERR_UNSUPPORTED;
   uint32_t wpb = tile.waves_per_block ? tile.waves_per_block : 4;
   if (wpb > 8) return ACCSPMM_ERR_UNSUPPORTED;
   const uint32_t KC = 16 * nb;
   const uint32_t n_kc = (K + KC - 1) / KC;
-  // split-BF16 with b_split 2: the kernel splits the caller's B itself
-  const bool fused = form == kSPLIT && tile.b_split == 2;
-  const int kform = fused ? (int)kSPLITF : form;
   // FP32 B is read with NB-wide vector loads: needs alignment
-  if (raw_b(kform)) {
+  if (raw_b(form)) {
     if ((ldb % nb) != 0 || ((uintptr_t)B % (4u * nb)) != 0) return ACCSPMM_ERR_UNSUPPORTED;
   }
   if ((phases & 2) && ((ldc % nb) != 0 || ((uintptr_t)C % (4u * nb)) != 0))
@@ -485,6 +467,14 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   std::lock_guard<std::mutex> lock(h->mu);
   const Schedule* sch = nullptr;
   ACC_TRY(get_schedule(h, tile, s, &sch));
+  uint64_t pbytes = 0, cbytes = 0;
+  spmm_workspace_need(h, sch, K, form, tile, &pbytes, &cbytes);
+  SpmmWorkspace w{};
+  if (ws) {
+    w = *ws;
+  } else {
+    ACC_TRY(stream_workspace(h, s, (phases & 2) ? pbytes : 0, cbytes, &w));
+  }
 
   SpmmArgs g{};
   g.rwo = h->rwo.as<uint32_t>();
@@ -492,10 +482,6 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   g.bits = h->bits.as<unsigned long long>();
   g.a2b = h->a2b.as<uint32_t>();
   g.data = h->data.as<float>();
-  if (form != kFP32) {
-    ACC_TRY(ensure_vsplit(h, s));
-    g.vsplit = h->vsplit.as<uint32_t>();
-  }
   g.units = sch->units.as<WorkUnit>();
   g.split_info = sch->split_info.as<uint32_t>();
   g.C = C;
@@ -511,39 +497,28 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   g.zero_row = h->cols;
   g.pad_tile = h->T;
   g.wide = h->rw == 16 && h->max_group_values > dev::kSpanValues ? 1u : 0u;
+  g.partials = pbytes ? w.partials : nullptr;
 
-  if (sch->num_pieces && !tile.sweep && (phases & 2)) {
-    const uint64_t pbytes = (uint64_t)sch->num_pieces * h->rw * g.kc_padded * 4;
-    ACC_TRY(h->partials.reserve(pbytes));
-    g.partials = h->partials.as<float>();
-  }
-
-  if (raw_b(kform)) {
+  if (raw_b(form)) {
     g.B = B;
     g.ldb = ldb;
   } else {
     const uint32_t ldx = n_kc * KC;
-    const uint64_t es = form == kSPLIT ? 4 : 2;
-    ACC_TRY(h->bconv.reserve(((uint64_t)h->cols + 1) * ldx * es));
     const uint64_t chunks = ((uint64_t)h->cols + 1) * (ldx / 4);
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((chunks + 255) / 256, 8192);
     const uint32_t vec = (ldb % 4 == 0 && ((uintptr_t)B % 16) == 0) ? 1u : 0u;
-    if (!(phases & 1)) {
-      // converted by an earlier prepare call
-    } else if (form == kSPLIT)
-      hipLaunchKernelGGL(convert_b<kSPLIT>, dim3(blocks), dim3(256), 0, s, B, h->cols, K,
-                         ldb, h->bconv.p, ldx, vec);
-    else
-      hipLaunchKernelGGL(convert_b<kBF16>, dim3(blocks), dim3(256), 0, s, B, h->cols, K,
-                         ldb, h->bconv.p, ldx, vec);
-    ACC_HIP(hipGetLastError());
-    g.B = h->bconv.p;
+    if (phases & 1) {
+      hipLaunchKernelGGL(convert_b, dim3(blocks), dim3(256), 0, s, B, h->cols, K, ldb,
+                         w.bconv, ldx, vec);
+      ACC_HIP(hipGetLastError());
+    }  // else converted by an earlier prepare call on this stream
+    g.B = w.bconv;
     g.ldb = ldx;
   }
   if (!(phases & 2)) return ACCSPMM_OK;
-  if (tile.sweep) return sweep_launch(h, g, tile, nb, kform, sch, s);
+  if (tile.sweep) return sweep_launch(h, g, tile, nb, form, sch, s);
 
-  ACC_TRY(launch_single(g, kform, h->rw, nb, wpb, sch->packed, sch->num_units, n_kc, s));
+  ACC_TRY(launch_single(g, form, h->rw, nb, wpb, sch->packed, sch->num_units, n_kc, s));
   if (sch->num_split_windows) {
     if (h->rw == 8)
       hipLaunchKernelGGL(k_reduce_split<8>, dim3(sch->num_split_windows), dim3(256), 0, s,
@@ -577,16 +552,13 @@ accspmm_status spmm_fit_tile(const accspmm_metcf* h, const float* B,
   } else {
     ACC_TRY(accspmm_tile_select(h, K, (accspmm_form)form, &t));
   }
-  if (form != kSPLIT) t.b_split = 0;
-  else if (t.b_split == 0) t.b_split = default_b_split(h, K);
+  // reserved (the split form always splits the FP32 B in registers)
+  t.b_split = 0;
   while (t.feat_per_lane > 1 &&
-         ((form == kFP32 &&
+         ((raw_b(form) &&
            ((ldb % t.feat_per_lane) || ((uintptr_t)B % (4u * t.feat_per_lane)))) ||
           (C && ((ldc % t.feat_per_lane) || ((uintptr_t)C % (4u * t.feat_per_lane))))))
     t.feat_per_lane /= 2;
-  // the in-register split reads B like the FP32 form; otherwise convert
-  if (t.b_split == 2 && ((ldb % t.feat_per_lane) || ((uintptr_t)B % (4u * t.feat_per_lane))))
-    t.b_split = 1;
   *out = t;
   return ACCSPMM_OK;
 }
@@ -605,7 +577,7 @@ static accspmm_status spmm_entry(const accspmm_metcf* h, const float* B,
   ACC_TRY(spmm_fit_tile(h, B, ldb, (phases & 2) ? C : nullptr, ldc, K, (int)form,
                         tile, &t));
   return spmm_launch(h, B, ldb, C, ldc, K, (int)form, t, (hipStream_t)stream,
-                     phases);
+                     phases, nullptr);
 }
 
 extern "C" accspmm_status accspmm_spmm(const accspmm_metcf* h, const float* B,

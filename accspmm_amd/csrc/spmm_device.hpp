@@ -16,13 +16,13 @@ typedef float f2 __attribute__((__vector_size__(2 * sizeof(float))));
 typedef float f4 __attribute__((__vector_size__(4 * sizeof(float))));
 typedef uint32_t u4 __attribute__((__vector_size__(4 * sizeof(uint32_t))));
 
-// kSPLITF: the split-BF16 form reading the caller's FP32 B and splitting it
-// into bf16 hi and lo in registers, with the rounding the conversion pass
-// uses (the same C bits, no converted copy of B)
-enum : int { kFP32 = 0, kSPLIT = 1, kBF16 = 2, kSPLITF = 3 };
-constexpr bool is_split(int f) { return f == kSPLIT || f == kSPLITF; }
-// forms that gather the caller's FP32 B (padded columns read row 0, masked)
-constexpr bool raw_b(int f) { return f == kFP32 || f == kSPLITF; }
+// Forms.  kFP32 and kSPLIT gather the caller's FP32 B (padded tile columns
+// read row 0, masked when that row is not finite); kBF16 gathers a converted
+// bf16 copy of B with an all-zero row for padded columns.  Both BF16 forms
+// read the FP32 A values and round or split them in registers.
+enum : int { kFP32 = 0, kSPLIT = 1, kBF16 = 2 };
+constexpr bool is_split(int f) { return f == kSPLIT; }
+constexpr bool raw_b(int f) { return f != kBF16; }
 
 struct SpmmArgs {
   const uint32_t* __restrict__ rwo;
@@ -30,19 +30,19 @@ struct SpmmArgs {
   const unsigned long long* __restrict__ bits;
   const uint32_t* __restrict__ a2b;
   const float* __restrict__ data;
-  const uint32_t* __restrict__ vsplit;  // BF16 forms: data as bf16 hi | lo << 16
   const WorkUnit* __restrict__ units;
   const uint32_t* __restrict__ split_info;  // per split window: (window, first slot, count)
   float* __restrict__ partials;
-  const void* __restrict__ B;  // FP32: float; SPLIT: u32 (hi|lo<<16); BF16: u16
+  const void* __restrict__ B;  // FP32 and split-BF16: float; BF16: u16
   float* __restrict__ C;
   // C row of format row r: row_map[r] when the graph was built with its
   // rows reordered, r when row_map is null
   const uint32_t* __restrict__ row_map;
-  // what padded columns read: BF16 forms: row zero_row of the converted B
-  // (its last row, all zero); FP32 form: row 0 of the caller's B, masked
-  // after the load (measured faster on gfx950 than reading a separate zero
-  // buffer)
+  // what padded columns read: BF16 form: row zero_row of the converted B
+  // (its last row, all zero); FP32 and split-BF16: row 0 of the caller's B,
+  // masked after the load when that row holds an infinity or NaN (0 x inf
+  // would otherwise leak into C; reading a separate zero buffer instead was
+  // measured slower on gfx950)
   uint32_t zero_row;
   uint32_t pad_tile;  // the all-padding tile (index T) past the real tiles
   uint32_t ldb, ldc;
@@ -59,26 +59,56 @@ struct SpmmArgs {
 
 __device__ inline uint32_t popc64(unsigned long long x) { return __popcll(x); }
 
-// bf16 round-to-nearest-even pair
+// bf16 round-to-nearest-even pair (a in the low half)
 __device__ inline uint32_t pack_bf16(float a, float b) {
   f2 v = {a, b};
   bf16x2 r = __builtin_convertvector(v, bf16x2);
   return __builtin_bit_cast(uint32_t, r);
 }
-__device__ inline float bf16_lo_to_f32(uint32_t p) {
-  return __builtin_bit_cast(float, p << 16);
+
+// x - p.lo (lo = 1) or y - p.hi (lo = 0) for a bf16 pair p, in one
+// v_dot2_f32_bf16: acc + p.lo * c.lo + p.hi * c.hi with c = (-1, 0) or
+// (0, -1).  The products are exact and so is the difference (the split
+// below only subtracts a rounding of the operand itself), so the result
+// does not depend on how the instruction rounds.  The constant goes through
+// a register: the compiler encodes the pair (-1, 0) as the inline constant
+// -1.0, which gfx950 reads as the fp32 bits 0xBF800000, i.e. the pair
+// (0, -1) (measured: 1,048,454 of 2^20 results off; none through a VGPR).
+__device__ inline uint32_t in_vgpr(uint32_t c) {
+  asm("" : "+v"(c));
+  return c;
 }
-__device__ inline float bf16_hi_to_f32(uint32_t p) {
-  return __builtin_bit_cast(float, p & 0xFFFF0000u);
+template <bool LO>
+__device__ inline float sub_bf16(uint32_t p, float acc) {
+  const uint32_t c = in_vgpr(LO ? 0x0000BF80u : 0xBF800000u);
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, p),
+                                         __builtin_bit_cast(bf16x2, c), acc, false);
 }
 
-// B conversion for the BF16 forms, one chunk of four consecutive features:
+// Three-term bf16 split of two floats x, y (the bf16x6 scheme of Henry, Tang
+// and Heinecke, "Leveraging the bfloat16 Artificial Intelligence Datatype
+// For Higher-Precision Computations", ARITH 2019): t0 = rne(x),
+// t1 = rne(x - t0), t2 = x - t0 - t1, each a bf16 pair (x in the low half).
+// x - t0 carries at most 16 significant bits and x - t0 - t1 at most 8, so
+// t2 is exact and x = t0 + t1 + t2 with no error for every finite x whose
+// last significant bit lies in bf16's range (|x| >= 2^-110 or so).  Seven
+// VALU operations per pair.
+__device__ inline void split3(uint32_t x, uint32_t y, uint32_t& t0, uint32_t& t1,
+                              uint32_t& t2) {
+  const float fx = __builtin_bit_cast(float, x), fy = __builtin_bit_cast(float, y);
+  t0 = pack_bf16(fx, fy);
+  const float rx = sub_bf16<true>(t0, fx), ry = sub_bf16<false>(t0, fy);
+  t1 = pack_bf16(rx, ry);
+  const float sx = sub_bf16<true>(t1, rx), sy = sub_bf16<false>(t1, ry);
+  t2 = __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, sy),
+                             __builtin_bit_cast(uint32_t, sx), 0x07060302u);
+}
+
+// B conversion for the BF16 form, one chunk of four consecutive features:
 // chunk c of the (N + 1) x ldx / 4 chunks of the converted B (ldx a multiple
 // of 16, zero past K and in the all-zero row N that padded tile columns
 // read).  A 16-byte load when the segment lies inside K and B is 16-byte
-// aligned with ldb % 4 == 0 (vec); one 16-byte (split: hi | lo << 16 per
-// feature) or 8-byte (BF16) store.
-template <int FORM>
+// aligned with ldb % 4 == 0 (vec); one 8-byte store.
 __device__ inline void convert_chunk(const float* __restrict__ B, uint32_t N, uint32_t K,
                                      uint32_t ldb, void* __restrict__ out, uint32_t ldx,
                                      uint32_t vec, uint64_t c) {
@@ -95,21 +125,10 @@ __device__ inline void convert_chunk(const float* __restrict__ B, uint32_t N, ui
       for (int i = 0; i < 4; ++i) x[i] = k + i < K ? src[i] : 0.f;
     }
   }
-  if (FORM == kSPLIT) {
-    u4 o;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const uint32_t hp = pack_bf16(x[i], 0.f) & 0xFFFFu;
-      const uint32_t lp = pack_bf16(x[i] - bf16_lo_to_f32(hp), 0.f) & 0xFFFFu;
-      o[i] = hp | (lp << 16);
-    }
-    reinterpret_cast<u4*>(out)[c] = o;
-  } else {
-    uint2 o;
-    o.x = pack_bf16(x[0], x[1]);
-    o.y = pack_bf16(x[2], x[3]);
-    reinterpret_cast<uint2*>(out)[c] = o;
-  }
+  uint2 o;
+  o.x = pack_bf16(x[0], x[1]);
+  o.y = pack_bf16(x[2], x[3]);
+  reinterpret_cast<uint2*>(out)[c] = o;
 }
 
 // Elements each lane gathers per B row: NB features, as 32-bit words.
@@ -143,11 +162,10 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
     } else {
       r.w[0] = *p;
     }
-  } else if constexpr (FORM == kSPLIT || FULL) {
-    // split form: the converted B (zero row for padded columns); forms that
-    // read the caller's FP32 B (FP32, split in registers): padded columns
-    // read row 0, masked when consumed
-    const uint32_t row = FORM == kSPLIT ? c : (col != kPad ? col : 0u);
+  } else if constexpr (FULL) {
+    // the caller's FP32 B: padded columns read row 0 (masked when consumed
+    // if that row is not finite)
+    const uint32_t row = col != kPad ? col : 0u;
     const uint32_t* p = static_cast<const uint32_t*>(B) + (uint64_t)row * ldb + feat;
     if constexpr (NB >= 4) {
 #pragma unroll
@@ -163,7 +181,7 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
       r.w[0] = *p;
     }
   } else {
-    // feature tail (FP32 form only): clamp inside the row, zero past K
+    // feature tail (FP32 B): clamp inside the row, zero past K
     const uint32_t* row = static_cast<const uint32_t*>(B) + (uint64_t)(col != kPad ? col : 0u) * ldb;
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
@@ -305,10 +323,9 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
     if (raw_b(FORM)) bm |= (col[j] != kPad ? 1u : 0u) << j;
   }
   st.bmask = bm;
-  // the group's first value: tcOffset of quarter 0's tile (lane 0); the
-  // BF16 forms read the values already split into bf16 hi | lo << 16
+  // the group's first value: tcOffset of quarter 0's tile (lane 0)
   const uint32_t vbase = __builtin_amdgcn_readfirstlane(off);
-  const uint32_t* vsrc = FORM == kFP32 ? reinterpret_cast<const uint32_t*>(g.data) : g.vsplit;
+  const uint32_t* vsrc = reinterpret_cast<const uint32_t*>(g.data);
   // RW=8: lanes past the group's values (it ends at tcOffset[t3 + 1],
   // which the odd lane 49 of quarter 3 holds; a padding tile there gives
   // nnz, clamped to the span) re-read the span's first 16 bytes, so only
@@ -326,10 +343,15 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
   if (RW == 16) st.vbase = vbase;
 }
 
+// MFMAs of one 4-tile group.  mask (wave-uniform): the forms that gather
+// the caller's FP32 B zero what padded columns loaded from B's row 0, needed
+// only when that row holds an infinity or NaN (the A value there is zero,
+// and 0 x inf is NaN).
 template <int FORM, int NB, int RW>
 __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
                                      uint32_t m, float* lv, uint32_t lane,
-                                     const uint32_t* __restrict__ vsrc, bool wide) {
+                                     const uint32_t* __restrict__ vsrc, bool wide,
+                                     bool mask) {
   // the group's value span through the wave's LDS buffer: a set bitmap bit
   // j reads the row's next value, a clear one the zero after the values
   *reinterpret_cast<u4*>(lv + 4u * lane) = st.vals;
@@ -355,13 +377,15 @@ __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
     const uint32_t rank = __popc(st.byte & ((1u << j) - 1u));
     a[j] = __builtin_bit_cast(uint32_t, lv[on ? st.base + rank : group_values<RW>()]);
   }
-  if (FORM == kFP32) {
+  if (raw_b(FORM) && mask) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const bool bv = (st.bmask >> j) & 1u;
 #pragma unroll
       for (int i = 0; i < NB; ++i) st.b[j].w[i] = bv ? st.b[j].w[i] : 0u;
     }
+  }
+  if (FORM == kFP32) {
 #pragma unroll
     for (int j = 0; j < 8; ++j)
 #pragma unroll
@@ -371,59 +395,16 @@ __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
             acc[b], 0, 0, 0);
     return;
   }
-  // A operand: 8 bf16 along the tile's condensed columns, picked from the
-  // split values (hi in the low half, lo in the high half); RW=8 split:
-  // rows 8-15 carry the low halves
-  u4 aop, alo;
-  {
-    const uint32_t sel_hi = 0x05040100u, sel_lo = 0x07060302u;
-    const uint32_t sel = (is_split(FORM) && RW == 8 && m >= 8) ? sel_lo : sel_hi;
+  if (FORM == kBF16) {
+    // A and B rounded to bf16 (B by the conversion pass): one MFMA a block
+    u4 aop;
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      aop[d] = __builtin_amdgcn_perm(a[2 * d + 1], a[2 * d], sel);
-      alo[d] = (is_split(FORM) && RW == 16) ? __builtin_amdgcn_perm(a[2 * d + 1], a[2 * d], sel_lo)
-                                            : 0u;
-    }
-  }
-  const bf16x8 A = __builtin_bit_cast(bf16x8, aop);
-  const bf16x8 AL = __builtin_bit_cast(bf16x8, alo);
-  if (FORM == kSPLITF) {
+    for (int d = 0; d < 4; ++d)
+      aop[d] = pack_bf16(__builtin_bit_cast(float, a[2 * d]),
+                         __builtin_bit_cast(float, a[2 * d + 1]));
+    const bf16x8 A = __builtin_bit_cast(bf16x8, aop);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const bool bv = (st.bmask >> j) & 1u;
-#pragma unroll
-      for (int i = 0; i < NB; ++i) st.b[j].w[i] = bv ? st.b[j].w[i] : 0u;
-    }
-  }
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    if (is_split(FORM)) {
-      u4 bh, bl;
-#pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        uint32_t x = st.b[2 * d].w[b], y = st.b[2 * d + 1].w[b];
-        if (FORM == kSPLITF) {
-          // hi = bf16(x), lo = bf16(x - hi), both rounded to nearest even,
-          // for the two columns of the pair at once
-          const float fx = __builtin_bit_cast(float, x), fy = __builtin_bit_cast(float, y);
-          const uint32_t hp = pack_bf16(fx, fy);
-          bh[d] = hp;
-          bl[d] = pack_bf16(fx - bf16_lo_to_f32(hp), fy - bf16_hi_to_f32(hp));
-        } else {
-          bh[d] = __builtin_amdgcn_perm(y, x, 0x05040100u);
-          bl[d] = __builtin_amdgcn_perm(y, x, 0x07060302u);
-        }
-      }
-      const bf16x8 BH = __builtin_bit_cast(bf16x8, bh);
-      const bf16x8 BL = __builtin_bit_cast(bf16x8, bl);
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, BH, acc[b], 0, 0, 0);
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, BL, acc[b], 0, 0, 0);
-      if (RW == 16) {
-        // all 16 rows are tile rows: the low halves of A need their own MFMAs
-        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL, BH, acc[b], 0, 0, 0);
-        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(AL, BL, acc[b], 0, 0, 0);
-      }
-    } else {
+    for (int b = 0; b < NB; ++b) {
       u4 bv;
       const uint32_t sel = (b & 1) ? 0x07060302u : 0x05040100u;
 #pragma unroll
@@ -433,6 +414,55 @@ __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
       }
       acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
           A, __builtin_bit_cast(bf16x8, bv), acc[b], 0, 0, 0);
+    }
+    return;
+  }
+  // split-BF16 (bf16x6): A = A0 + A1 + A2 and B = B0 + B1 + B2 exactly
+  // (split3); the six products A_i B_j with i + j <= 2 are accumulated, the
+  // three dropped ones are below 2^-23 of |a||b| together.  The A operand's
+  // rows are tile rows: RW=16 all 16 (six MFMAs a block); RW=8 rows 0-7,
+  // so rows 8-15 carry a second A term: [A0; A1] against B0, B1 and B2
+  // gives A0 B0, A1 B0, A0 B1, A1 B1, A0 B2 (and A1 B2), and [A2; 0]
+  // against B0 the last product (four MFMAs a block); rows r and r+8 are
+  // added when the window is stored.
+  uint32_t t0[4], t1[4], t2[4];
+#pragma unroll
+  for (int d = 0; d < 4; ++d) split3(a[2 * d], a[2 * d + 1], t0[d], t1[d], t2[d]);
+  const u4 a0 = {t0[0], t0[1], t0[2], t0[3]}, a1 = {t1[0], t1[1], t1[2], t1[3]},
+           a2 = {t2[0], t2[1], t2[2], t2[3]};
+  u4 x0 = a0, x1 = a1;
+  if (RW == 8) {
+    const bool upper = m >= 8;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      x0[d] = upper ? a1[d] : a0[d];
+      x1[d] = upper ? 0u : a2[d];
+    }
+  }
+  const bf16x8 X0 = __builtin_bit_cast(bf16x8, x0);  // RW=8: [A0; A1]; RW=16: A0
+  const bf16x8 X1 = __builtin_bit_cast(bf16x8, x1);  // RW=8: [A2; 0];  RW=16: A1
+  const bf16x8 X2 = __builtin_bit_cast(bf16x8, a2);  // RW=16: A2
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    uint32_t s0[4], s1[4], s2[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) split3(st.b[2 * d].w[b], st.b[2 * d + 1].w[b], s0[d], s1[d], s2[d]);
+    const bf16x8 B0 = __builtin_bit_cast(bf16x8, u4{s0[0], s0[1], s0[2], s0[3]});
+    const bf16x8 B1 = __builtin_bit_cast(bf16x8, u4{s1[0], s1[1], s1[2], s1[3]});
+    const bf16x8 B2 = __builtin_bit_cast(bf16x8, u4{s2[0], s2[1], s2[2], s2[3]});
+    // smallest products first
+    if (RW == 8) {
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B2, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, B0, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B1, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B0, acc[b], 0, 0, 0);
+    } else {
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B2, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, B1, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X2, B0, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B1, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, B0, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B0, acc[b], 0, 0, 0);
     }
   }
 }
@@ -461,8 +491,8 @@ __device__ inline uint64_t out_row(const uint32_t* __restrict__ rmap, uint32_t r
 }
 
 // Writes the accumulated rows of window w to C (split form, RW=8: rows r
-// and r+8 of the MFMA tile hold the A_hi and A_lo halves and are added in
-// place; the caller resets the accumulators afterwards).
+// and r+8 of the MFMA tile hold products of different A terms of tile row
+// r and are added in place; the caller resets the accumulators afterwards).
 template <int FORM, int NB, int RW>
 __device__ inline void store_window(f4* acc, uint32_t w, uint32_t q,
                                     uint32_t feat, uint32_t nvalid, float* C,
@@ -540,13 +570,13 @@ __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
                                  GroupPos pos, uint32_t ngroups, uint32_t q,
                                  uint32_t m, bool a_rows, uint32_t feat,
                                  const void* B, uint32_t ldb, float* lv,
-                                 uint32_t lane) {
+                                 uint32_t lane, bool mask) {
   Meta mA, mB;
   Stage<FORM, NB> dA, dB;
-  const uint32_t* vsrc = FORM == kFP32 ? reinterpret_cast<const uint32_t*>(g.data) : g.vsplit;
+  const uint32_t* vsrc = reinterpret_cast<const uint32_t*>(g.data);
   auto consume = [&](Stage<FORM, NB>& d) {
     if (PACKED && out.direct) out.move_to(d.win);
-    compute_stage<FORM, NB, RW>(d, out.acc, m, lv, lane, vsrc, g.wide != 0u);
+    compute_stage<FORM, NB, RW>(d, out.acc, m, lv, lane, vsrc, g.wide != 0u, mask);
   };
   // pos: position of the most recently loaded metadata
   load_meta<RW>(mA, g, pos, true, q, m);
@@ -592,6 +622,21 @@ __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
   }
 }
 
+// Wave-uniform: whether row 0 of the caller's FP32 B, which padded tile
+// columns read, holds an infinity or NaN in this lane group's features (the
+// loaded words must then be masked).  Only called for a unit with tiles,
+// so B has at least one row.
+template <int FORM, int NB>
+__device__ inline bool row0_needs_mask(const void* B, uint32_t feat, uint32_t K) {
+  if (!raw_b(FORM)) return false;
+  const float* r0 = static_cast<const float*>(B);
+  bool bad = false;
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+    if (feat + i < K) bad |= !__builtin_isfinite(r0[feat + i]);
+  return __builtin_amdgcn_ballot_w64(bad) != 0;
+}
+
 // PACKED: the schedule may hold multi-window units (the loop then flushes C
 // at window changes); without it every direct unit is one window and the
 // loop carries no flush code (fewer registers).
@@ -604,11 +649,11 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
   const uint32_t q = lane >> 4, n = lane & 15u;
   const uint32_t k0 = kc * KC;
   const uint32_t feat = k0 + NB * n;
-  // B loads are full-width unless this chunk straddles K (FP32 form; the
-  // converted BF16-form buffers are zero-padded to a multiple of KC)
+  // B loads are full-width unless this chunk straddles K (FP32 B; the
+  // converted BF16-form buffer is zero-padded to a multiple of KC)
   const bool full = !raw_b(FORM) || k0 + KC <= K;
   // A rows: lanes m = n carry tile row m (RW=16) or m%8 (RW=8; rows 8-15
-  // only in the split form, as the low halves)
+  // only in the split form, as a second A term)
   const uint32_t m = n;
   const bool a_rows = RW == 16 || m < 8 || is_split(FORM);
   const uint32_t nvalid = feat < K ? min((uint32_t)NB, K - feat) : 0u;
@@ -634,12 +679,13 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
     // packed unit: group ranges are whole windows (its first window is
     // not empty when it has groups)
     if (u.slot < 0) pos.tend = g.rwo[pos.w + 1];
+    const bool mask = row0_needs_mask<FORM, NB>(B, feat, K);
     if (full)
       unit_loop<FORM, NB, RW, true, PACKED>(g, out, pos, ngroups, q, m, a_rows, feat, B, ldb,
-                                            lv, lane);
+                                            lv, lane, mask);
     else
       unit_loop<FORM, NB, RW, false, PACKED>(g, out, pos, ngroups, q, m, a_rows, feat, B, ldb,
-                                             lv, lane);
+                                             lv, lane, mask);
   }
 
   if (u.slot < 0) {

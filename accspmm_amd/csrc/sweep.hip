@@ -10,7 +10,7 @@
 //
 // The pieces, their 4-tile groups and the summation order are those of the
 // work-unit schedule with the same (split, pack, panels): a piece
-// accumulates from zero (the split form's hi and lo rows combined at its
+// accumulates from zero (the split form's two A-term rows combined at its
 // end), a window of one piece stores that sum, and a split window's sum is
 // 0 + s_0 + s_1 + ... in slot order, as k_reduce_split computes it.  Both
 // paths therefore give the same C bits, and the batched kernel (which runs
@@ -41,7 +41,7 @@ struct SweepArgs {
 template <int RW>
 constexpr uint32_t slot_lanes() { return RW == 8 ? 32u : 64u; }
 
-// Ends a piece: combines the split form's hi/lo rows and adds the piece sum
+// Ends a piece: combines the split form's two A-term rows and adds the piece sum
 // into the window's LDS slot (first piece: stored, or 0 + s for a split
 // window, matching the work-unit path's direct store and reduction).
 template <int FORM, int NB, int RW>
@@ -109,7 +109,7 @@ template <int FORM, int NB, int RW, bool FULL>
 __device__ void sweep_loop(const SweepArgs& a, f4* acc, f4* slots, GroupPos pos,
                            uint32_t ngroups, uint32_t lane, uint32_t q,
                            uint32_t m, bool a_rows, uint32_t feat,
-                           uint32_t& started, float* lv) {
+                           uint32_t& started, float* lv, bool mask) {
   const SpmmArgs& g = a.g;
   constexpr uint32_t NL = slot_lanes<RW>();
   const void* B = g.B;
@@ -124,13 +124,13 @@ __device__ void sweep_loop(const SweepArgs& a, f4* acc, f4* slots, GroupPos pos,
   };
   Meta mA, mB;
   Stage<FORM, NB> dA, dB;
-  const uint32_t* vsrc = FORM == kFP32 ? reinterpret_cast<const uint32_t*>(g.data) : g.vsplit;
+  const uint32_t* vsrc = reinterpret_cast<const uint32_t*>(g.data);
   auto consume = [&](Stage<FORM, NB>& d) {
     if (d.win != cur) {
       end_piece(cur);
       cur = d.win;
     }
-    compute_stage<FORM, NB, RW>(d, acc, m, lv, lane, vsrc, g.wide != 0u);
+    compute_stage<FORM, NB, RW>(d, acc, m, lv, lane, vsrc, g.wide != 0u, mask);
   };
   load_meta<RW>(mA, g, pos, true, q, m);
   if (ngroups > 1) sweep_next(pos, a.pieces);
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
     const uint32_t* wl = a.wlist + (size_t)idx * a.M;
     for (uint32_t kc = 0; kc < g.n_kc; ++kc) {
       const uint32_t k0 = kc * KC, feat = k0 + NB * n;
-      // B loads are full-width unless the chunk straddles K (FP32 form)
+      // B loads are full-width unless the chunk straddles K (FP32 B)
       const bool full = !raw_b(FORM) || k0 + KC <= g.K;
       const uint32_t nvalid = feat < g.K ? min((uint32_t)NB, g.K - feat) : 0u;
       uint32_t started = 0;
@@ -211,12 +211,13 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
         for (int b = 0; b < NB; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
         GroupPos pos{pb, (uint32_t)__builtin_amdgcn_readfirstlane(a.pieces[3ull * pb]),
                      (uint32_t)__builtin_amdgcn_readfirstlane(a.pieces[3ull * pb + 1])};
+        const bool mask = row0_needs_mask<FORM, NB>(g.B, feat, g.K);
         if (full)
           sweep_loop<FORM, NB, RW, true>(a, acc, slots, pos, ng, lane, q, m, a_rows, feat,
-                                         started, lv);
+                                         started, lv, mask);
         else
           sweep_loop<FORM, NB, RW, false>(a, acc, slots, pos, ng, lane, q, m, a_rows, feat,
-                                          started, lv);
+                                          started, lv, mask);
       }
       for (uint32_t s = 0; s < a.M; ++s) {
         const uint32_t w = __builtin_amdgcn_readfirstlane(wl[s]);
@@ -390,12 +391,10 @@ accspmm_status sweep_launch(const accspmm_metcf* h, const dev::SpmmArgs& g,
   if (h->rw == 8) {
     if (form == kFP32) return launch_sweep_nb<kFP32, 8>(nb, a, cus, wpb, lds, s);
     if (form == kSPLIT) return launch_sweep_nb<kSPLIT, 8>(nb, a, cus, wpb, lds, s);
-    if (form == kSPLITF) return launch_sweep_nb<kSPLITF, 8>(nb, a, cus, wpb, lds, s);
     return launch_sweep_nb<kBF16, 8>(nb, a, cus, wpb, lds, s);
   }
   if (form == kFP32) return launch_sweep_nb<kFP32, 16>(nb, a, cus, wpb, lds, s);
   if (form == kSPLIT) return launch_sweep_nb<kSPLIT, 16>(nb, a, cus, wpb, lds, s);
-  if (form == kSPLITF) return launch_sweep_nb<kSPLITF, 16>(nb, a, cus, wpb, lds, s);
   return launch_sweep_nb<kBF16, 16>(nb, a, cus, wpb, lds, s);
 }
 

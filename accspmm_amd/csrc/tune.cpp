@@ -27,10 +27,6 @@
 namespace accspmm {
 void default_tile(const accspmm_metcf* h, uint32_t K, int form,
                   accspmm_tile* t);
-accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
-                           float* C, uint32_t ldc, uint32_t K, int form,
-                           const accspmm_tile& tile, hipStream_t s,
-                           int phases);
 accspmm_status spmm_fit_tile(const accspmm_metcf* h, const float* B,
                              uint32_t ldb, float* C, uint32_t ldc, uint32_t K,
                              int form, const accspmm_tile* tile,
@@ -158,13 +154,13 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
   ACC_HIP(hipEventCreate(&e1));
   // median of 5 timed launches after 2 warm-up launches; < 0 if rejected
   auto time_tile = [&](const accspmm_tile& c) -> float {
-    if (spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3) != ACCSPMM_OK)
+    if (spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3, nullptr) != ACCSPMM_OK)
       return -1.f;  // e.g. alignment not met
-    for (int i = 0; i < 2; ++i) spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3);
+    for (int i = 0; i < 2; ++i) spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3, nullptr);
     std::vector<float> t;
     for (int i = 0; i < 5; ++i) {
       (void)hipEventRecord(e0, s);
-      spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3);
+      spmm_launch(h, B, ldb, C, ldc, K, (int)form, c, s, 3, nullptr);
       (void)hipEventRecord(e1, s);
       (void)hipEventSynchronize(e1);
       float ms = 0;
@@ -201,7 +197,7 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
   if (bms < 1e30f) {
     const uint32_t KC = 16 * bt.feat_per_lane;
     const uint32_t n_kc = (K + KC - 1) / KC;
-    const uint64_t es = form == ACCSPMM_FORM_FP32 ? 4 : form == ACCSPMM_FORM_SPLIT_BF16 ? 4 : 2;
+    const uint64_t es = form == ACCSPMM_FORM_BF16 ? 2 : 4;
     const uint64_t row_bytes = (bt.xcd_slice == 1 ? KC : (uint64_t)n_kc * KC) * es;
     if ((uint64_t)h->cols * row_bytes > (4ull << 20)) {
       const accspmm_tile base = bt;
@@ -212,15 +208,6 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
           c.split_tiles = sp;
           consider(c);
         }
-    }
-  }
-  // stage 3 (split-BF16): both ways of splitting B, around the best shape
-  if (bms < 1e30f && form == ACCSPMM_FORM_SPLIT_BF16) {
-    const accspmm_tile base = bt;
-    for (uint32_t bs : {1u, 2u}) {
-      accspmm_tile c = base;
-      c.b_split = bs;
-      consider(c);
     }
   }
   (void)hipEventDestroy(e0);
@@ -271,7 +258,7 @@ extern "C" accspmm_status accspmm_autotune_layout(const accspmm_csr_view* csr,
       std::vector<float> ms;
       for (int i = 0; i < 7 && err == ACCSPMM_OK; ++i) {
         (void)hipEventRecord(e0, s);
-        err = spmm_launch(h, B, ldb, C, ldc, K, (int)form, t, s, 3);
+        err = spmm_launch(h, B, ldb, C, ldc, K, (int)form, t, s, 3, nullptr);
         (void)hipEventRecord(e1, s);
         (void)hipEventSynchronize(e1);
         float x = 0.f;

@@ -5,7 +5,10 @@
 // when the file cannot be opened, MM_READ_BANNER_ERROR (-2) for a bad banner,
 // ERROR_READ_MTX (3) for a bad size line or entry.  The COO comes back in row
 // order (the order of the CSR the reference builds from it); symmetric and
-// Hermitian files are expanded as in the reference.
+// Hermitian files are expanded as in the reference.  The COO's row, col and
+// data pointers are overwritten with new malloc'd arrays without reading
+// them, so an unconstructed COO from malloc (the reference driver's use) is
+// valid input.
 #pragma once
 #include <cstdio>
 #include <cstdlib>
@@ -38,12 +41,14 @@ int read_from_mtx(const char* filename, COO<DataType>* coo, bool canonicalize = 
   }
   std::cout << "m_tmp: " << h.rows << ", n_tmp: " << h.cols << ", nnz_tmp: " << h.nnz
             << std::endl;
+  // like the reference's handle_coo (include/mmio.hpp:136-138,160-162) the
+  // array pointers are overwritten, never freed: the reference's driver
+  // passes a COO from malloc whose pointers are uninitialised
+  // (src/mma_tf32.cu:198-199).  Arrays a COO already holds are left to the
+  // caller, as in the reference.
   coo->rows = h.rows;
   coo->cols = h.cols;
   coo->nnz = h.nnz;
-  std::free(coo->row);
-  std::free(coo->col);
-  std::free(coo->data);
   coo->row = (vint*)std::malloc(sizeof(vint) * (h.nnz ? h.nnz : 1));
   coo->col = (vint*)std::malloc(sizeof(vint) * (h.nnz ? h.nnz : 1));
   coo->data = (DataType*)std::malloc(sizeof(DataType) * (h.nnz ? h.nnz : 1));

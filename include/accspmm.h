@@ -81,8 +81,13 @@ typedef struct accspmm_build_opts {
 
 void accspmm_build_opts_default(accspmm_build_opts* o);
 
-/* Opaque device-resident METCF(+AdpBME) graph.  Immutable after build, so it
- * may be shared read-only across streams. */
+/* Opaque device-resident METCF(+AdpBME) graph.  Its format arrays are
+ * immutable after build.  accspmm_spmm keeps one workspace (split-window
+ * partials, the BF16 form's converted B) per stream inside the handle, so
+ * calls on different streams may share a handle concurrently; calls on one
+ * stream are ordered by the stream.  Host threads may call concurrently (an
+ * internal lock serialises the enqueue).  Destroy only after every stream
+ * that used the handle is idle. */
 typedef struct accspmm_metcf accspmm_metcf;
 
 typedef struct accspmm_metcf_info {
@@ -167,7 +172,9 @@ accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** out,
 
 typedef enum accspmm_form {
   ACCSPMM_FORM_FP32 = 0,       /* FP32 MFMA (v_mfma_f32_16x16x4_f32)           */
-  ACCSPMM_FORM_SPLIT_BF16 = 1, /* A,B split hi+lo BF16, 4 products, FP32 acc  */
+  ACCSPMM_FORM_SPLIT_BF16 = 1, /* A,B split exactly into 3 BF16 terms in
+                                  registers, the 6 products of order >= 2^-16
+                                  (bf16x6), FP32 accumulate: <= 1e-5 max-rel */
   ACCSPMM_FORM_BF16 = 2        /* A,B rounded to BF16, FP32 accumulate        */
 } accspmm_form;
 
@@ -191,32 +198,30 @@ typedef struct accspmm_tile {
                               stay in LDS and runs their pieces panel by
                               panel (same pieces and result bits as 0; the
                               work-unit mapping fields are ignored)          */
-  uint32_t b_split;        /* split-BF16 form, how B is split into bf16 hi and
-                              lo: 1 a conversion pass writes a split copy of B
-                              that the kernel gathers; 2 the kernel gathers
-                              the FP32 B and splits it in registers (no copy,
-                              the same C bits); 0 the library's choice.  The
-                              batched and served paths always use 1 (a second
-                              instantiation in the persistent kernel slowed
-                              every problem, measured).                   */
+  uint32_t b_split;        /* reserved, ignored: the split-BF16 form gathers
+                              the caller's FP32 B and splits it in registers
+                              (a converted copy of three bf16 terms would
+                              gather 1.5x the bytes)                      */
 } accspmm_tile;
 
 /* C[rows x K] = A * B[cols x K].  B (ldb >= K) and C (ldc >= K) are
  * caller-owned DEVICE FP32 buffers; C is written exactly (rows x K, no
  * padding rows).  Replaces TF32Compute::compute / adpBalanceCompute
  * (include/tf32_comp.hpp:52-113) and the four kernels of
- * include/mma_tf32.cuh.  The BF16 forms convert B into a workspace owned by
- * the handle (allocated on first use for a given K, reused after), so the
- * conversion is part of each call.  Asynchronous on `stream`. */
+ * include/mma_tf32.cuh.  The BF16 form converts B into the handle's
+ * workspace for `stream` (allocated on first use, grown when a call needs
+ * more, reused after), so the conversion is part of each call; split
+ * windows write their partial sums there too.  Asynchronous on `stream`. */
 accspmm_status accspmm_spmm(const accspmm_metcf* h, const float* B,
                             uint32_t ldb, float* C, uint32_t ldc, uint32_t K,
                             accspmm_form form, const accspmm_tile* tile,
                             void* stream);
 
 /* The two halves of accspmm_spmm, for callers that time or overlap them:
- * accspmm_spmm_prepare_b converts B into the handle's workspace for `form`
- * (a no-op for FP32), accspmm_spmm_prepared runs the SpMM kernel on it.
- * Both must be given the same B, K, form and tile. */
+ * accspmm_spmm_prepare_b converts B into the handle's workspace for
+ * `stream` (BF16 form; a no-op for the others), accspmm_spmm_prepared runs
+ * the SpMM kernel on it.  Both must be given the same B, K, form, tile and
+ * stream. */
 accspmm_status accspmm_spmm_prepare_b(const accspmm_metcf* h, const float* B,
                                       uint32_t ldb, uint32_t K,
                                       accspmm_form form,

@@ -2,6 +2,8 @@
 // include/utils.hpp of AccSpMM), compiled unchanged against the drop-in
 // headers.  Prints the METCF and AdpBME arrays as JSON for the test harness.
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 
 #include "accspmm/mmio.hpp"
 
@@ -15,9 +17,17 @@ static void dump(const char* name, const V& v, bool last = false) {
 
 int main(int argc, char** argv) {
   if (argc < 2) return ERROR_ARGS;
-  COO<MAT_VAL_TYPE>* coo = new COO<MAT_VAL_TYPE>();
+  // the reference driver's sequence (src/mma_tf32.cu:198-213): a COO from
+  // malloc, never constructed, filled by read_from_mtx; CSR from it; freed
+  // without destruction after the CSR is built
+  COO<MAT_VAL_TYPE>* coo = (COO<MAT_VAL_TYPE>*)malloc(sizeof(COO<MAT_VAL_TYPE>));
+  std::memset((void*)coo, 0xA5, sizeof(COO<MAT_VAL_TYPE>));  // garbage pointers
   if (read_from_mtx<MAT_VAL_TYPE>(argv[1], coo) != SUCCESS) return ERROR_READ_MTX;
   CSR<MAT_VAL_TYPE> csr(coo);
+  std::free(coo->row);
+  std::free(coo->col);
+  std::free(coo->data);
+  std::free(coo);
   METCFBit<MAT_VAL_TYPE> metcf_bit;
   metcf_bit.convertFromCSR(csr);
   bool load_balance = balanceStrategy(metcf_bit, metcf_bit.tcOffset.size() - 1,
@@ -39,6 +49,5 @@ int main(int argc, char** argv) {
   dump("groupOffset", adpbme.groupOffset);
   dump("rowIndices", adpbme.rowIndices);
   std::printf("\"balanced\": %d\n}\n", load_balance ? 1 : 0);
-  delete coo;
   return 0;
 }

@@ -9,7 +9,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-TOL = {acc.FORM_FP32: 1e-5, acc.FORM_SPLIT_BF16: 2.0 ** -15 + 1e-6, acc.FORM_BF16: 1e-2}
+TOL = {acc.FORM_FP32: 1e-5, acc.FORM_SPLIT_BF16: 1e-5, acc.FORM_BF16: 1e-2}
 
 
 def _pool(n, seed, nmin=500, nmax=6000):
