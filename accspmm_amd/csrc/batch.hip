@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_batch_convert(
 // whose tasks are prefix[p0] .. prefix[p1] - 1.  One NB per kernel keeps the
 // register budget of the standalone kernel (three NB variants in one body
 // spilled).
-template <int FORM, int RW, int NB>
+template <int FORM, int RW, int NB, bool BUF>
 __global__ __launch_bounds__(256) void k_batch_spmm(
     const SpmmArgs* __restrict__ descs, const uint32_t* __restrict__ prefix,
     uint32_t p0, uint32_t p1, uint32_t* __restrict__ head) {
@@ -106,20 +106,32 @@ __global__ __launch_bounds__(256) void k_batch_spmm(
     const uint32_t local = t - prefix[p];
     const WorkUnit u = g.units[local / g.n_kc];
     const uint32_t kc = local % g.n_kc;
-    run_unit<FORM, NB, RW, true>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv);
+    run_unit<FORM, NB, RW, true, BUF>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv);
   }
 }
 
-template <int FORM, int RW>
-void launch_batch_class(uint32_t nb, dim3 grid, hipStream_t s, const SpmmArgs* descs,
-                        const uint32_t* prefix, uint32_t p0, uint32_t p1, uint32_t* head) {
+template <int FORM, int RW, bool BUF>
+void launch_batch_buf(uint32_t nb, dim3 grid, hipStream_t s, const SpmmArgs* descs,
+                      const uint32_t* prefix, uint32_t p0, uint32_t p1, uint32_t* head) {
   const dim3 blk(256);
   if (nb == 4)
-    hipLaunchKernelGGL((k_batch_spmm<FORM, RW, 4>), grid, blk, 0, s, descs, prefix, p0, p1, head);
+    hipLaunchKernelGGL((k_batch_spmm<FORM, RW, 4, BUF>), grid, blk, 0, s, descs, prefix, p0, p1,
+                       head);
   else if (nb == 2)
-    hipLaunchKernelGGL((k_batch_spmm<FORM, RW, 2>), grid, blk, 0, s, descs, prefix, p0, p1, head);
+    hipLaunchKernelGGL((k_batch_spmm<FORM, RW, 2, BUF>), grid, blk, 0, s, descs, prefix, p0, p1,
+                       head);
   else
-    hipLaunchKernelGGL((k_batch_spmm<FORM, RW, 1>), grid, blk, 0, s, descs, prefix, p0, p1, head);
+    hipLaunchKernelGGL((k_batch_spmm<FORM, RW, 1, BUF>), grid, blk, 0, s, descs, prefix, p0, p1,
+                       head);
+}
+
+template <int FORM, int RW>
+void launch_batch_class(const BatchClass& k, hipStream_t s, const SpmmArgs* descs,
+                        const uint32_t* prefix, uint32_t* head) {
+  if (k.buf)
+    launch_batch_buf<FORM, RW, true>(k.nb, dim3(k.grid), s, descs, prefix, k.p0, k.p1, head);
+  else
+    launch_batch_buf<FORM, RW, false>(k.nb, dim3(k.grid), s, descs, prefix, k.p0, k.p1, head);
 }
 
 template <int RW>
@@ -267,6 +279,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     g.rw = h->rw;
     g.nb = q.nb;
       g.zero_row = h->cols;
+    g.b_rows = form != kBF16 ? h->cols : h->cols + 1;
     g.pad_tile = h->T;
     g.wide = h->rw == 16 && h->max_group_values > dev::kSpanValues ? 1u : 0u;
     g.partials = reinterpret_cast<float*>(A + bc);
@@ -312,6 +325,8 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
       cls[nclass - 1].p1 = j + 1;
     }
     cls[nclass - 1].tasks += q.tasks;
+    // the class gathers by buffer loads when every problem's B allows it
+    cls[nclass - 1].buf &= dev::buffer_ok(g.b_rows, (uint64_t)g.ldb * (form == kBF16 ? 2 : 4));
     PF[j] = run;
     run += (uint32_t)q.tasks;
     SP[j] = nsplit;
@@ -374,15 +389,14 @@ accspmm_status BatchLauncher::launch(hipStream_t s) const {
   for (uint32_t c = 0; c < nclass; ++c) {
     const BatchClass& k = cls[c];
     if (k.tasks == 0) continue;
-    const dim3 g(k.grid);
     if (rw == 8) {
-      if (form == kFP32) launch_batch_class<kFP32, 8>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
-      else if (form == kSPLIT) launch_batch_class<kSPLIT, 8>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
-      else launch_batch_class<kBF16, 8>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
+      if (form == kFP32) launch_batch_class<kFP32, 8>(k, s, descs, prefix, H + c);
+      else if (form == kSPLIT) launch_batch_class<kSPLIT, 8>(k, s, descs, prefix, H + c);
+      else launch_batch_class<kBF16, 8>(k, s, descs, prefix, H + c);
     } else {
-      if (form == kFP32) launch_batch_class<kFP32, 16>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
-      else if (form == kSPLIT) launch_batch_class<kSPLIT, 16>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
-      else launch_batch_class<kBF16, 16>(k.nb, g, s, descs, prefix, k.p0, k.p1, H + c);
+      if (form == kFP32) launch_batch_class<kFP32, 16>(k, s, descs, prefix, H + c);
+      else if (form == kSPLIT) launch_batch_class<kSPLIT, 16>(k, s, descs, prefix, H + c);
+      else launch_batch_class<kBF16, 16>(k, s, descs, prefix, H + c);
     }
     ACC_HIP(hipGetLastError());
   }

@@ -28,6 +28,7 @@ struct BatchClass {
   uint32_t nb, p0, p1;
   uint64_t tasks;
   uint32_t grid = 1;
+  bool buf = true;  // every problem's B gathered by buffer loads
 };
 // NB 4, 2 and 1 (the persistent kernels carry NB <= 4)
 constexpr uint32_t kMaxBatchClasses = 3;

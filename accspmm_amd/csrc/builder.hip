@@ -368,7 +368,9 @@ extern "C" void accspmm_build_opts_default(accspmm_build_opts* o) {
   o->t_write = 2;
   o->target_num = 16;
   o->imbalance_threshold = 8;
-  o->build_adpbme = 1;
+  // the SpMM kernels run work units, not AdpBME groups: groups are built
+  // only for callers that export them (the AdpBME drop-in class)
+  o->build_adpbme = 0;
   o->reorder_rows = 0;
 }
 

@@ -47,11 +47,19 @@
 #include "spmm_device.hpp"
 
 namespace accspmm {
+// whether a launch over g gathers B by buffer loads (every offset fits)
+bool gather_by_buffer(const dev::SpmmArgs& g, int form) {
+  return dev::buffer_ok(g.b_rows, (uint64_t)g.ldb * (form == dev::kBF16 ? 2 : 4));
+}
+
 namespace {
 using namespace dev;
 
-template <int FORM, int NB, int RW, bool PACKED>
-__global__ __launch_bounds__(512) void spmm_kernel(SpmmArgs g) {
+// NB = 8 holds 2 x 64 words of gathered B per lane in flight: blocks of at
+// most 4 waves, so the compiler may give a wave all 512 registers of its
+// SIMD lane (VGPRs and AGPRs).
+template <int FORM, int NB, int RW, bool PACKED, bool BUF>
+__global__ __launch_bounds__(NB == 8 ? 256 : 512) void spmm_kernel(SpmmArgs g) {
   const uint32_t wpb = blockDim.x >> 6;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   uint32_t unit, kc;
@@ -79,7 +87,7 @@ __global__ __launch_bounds__(512) void spmm_kernel(SpmmArgs g) {
   float* lv = lds_vals + wave * wave_lds<RW>();
   if ((threadIdx.x & 63u) == 0) lv[group_values<RW>()] = 0.f;
   const WorkUnit u = g.units[unit];
-  run_unit<FORM, NB, RW, PACKED>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv);
+  run_unit<FORM, NB, RW, PACKED, BUF>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv);
 }
 
 // B conversion for the BF16 form: zero-padded to ldx >= K columns, plus
@@ -285,12 +293,15 @@ namespace {
 template <int FORM, int NB, int RW>
 void launch_t(const SpmmArgs& g, uint32_t blocks, uint32_t wpb, bool packed,
               hipStream_t s) {
-  if (packed)
-    hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, true>), dim3(blocks),
-                       dim3(64 * wpb), 0, s, g);
-  else
-    hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, false>), dim3(blocks),
-                       dim3(64 * wpb), 0, s, g);
+  const bool buf = gather_by_buffer(g, FORM);
+  const dim3 grid(blocks), blk(64 * wpb);
+  if (packed) {
+    if (buf) hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, true, true>), grid, blk, 0, s, g);
+    else hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, true, false>), grid, blk, 0, s, g);
+  } else {
+    if (buf) hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, false, true>), grid, blk, 0, s, g);
+    else hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, false, false>), grid, blk, 0, s, g);
+  }
 }
 
 template <int FORM, int RW>
@@ -455,6 +466,9 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   if (nb != 1 && nb != 2 && nb != 4 && nb != 8) return ACCSPMM_ERR_UNSUPPORTED;
   uint32_t wpb = tile.waves_per_block ? tile.waves_per_block : 4;
   if (wpb > 8) return ACCSPMM_ERR_UNSUPPORTED;
+  // NB = 8 kernels run one wave per SIMD (blocks of at most 4 waves), which
+  // leaves them the whole register file, accumulation registers included
+  if (nb == 8) wpb = std::min<uint32_t>(wpb, 4);
   const uint32_t KC = 16 * nb;
   const uint32_t n_kc = (K + KC - 1) / KC;
   // FP32 B is read with NB-wide vector loads: needs alignment
@@ -495,6 +509,7 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   g.xcd_slice = tile.xcd_slice == 2 ? 2u
                 : (tile.xcd_slice == 1 && (8 % n_kc) == 0) ? 1u : 0u;
   g.zero_row = h->cols;
+  g.b_rows = raw_b(form) ? h->cols : h->cols + 1;
   g.pad_tile = h->T;
   g.wide = h->rw == 16 && h->max_group_values > dev::kSpanValues ? 1u : 0u;
   g.partials = pbytes ? w.partials : nullptr;
@@ -554,6 +569,7 @@ accspmm_status spmm_fit_tile(const accspmm_metcf* h, const float* B,
   }
   // reserved (the split form always splits the FP32 B in registers)
   t.b_split = 0;
+  if (t.feat_per_lane == 8 && t.waves_per_block > 4) t.waves_per_block = 4;
   while (t.feat_per_lane > 1 &&
          ((raw_b(form) &&
            ((ldb % t.feat_per_lane) || ((uintptr_t)B % (4u * t.feat_per_lane)))) ||

@@ -1,6 +1,6 @@
 // spmm_device.hpp — device code shared by the single-graph SpMM kernel
 // (spmm.hip) and the persistent batched kernel (batch.hip): the per-work-unit
-// routine run_unit<FORM, NB, RW>.  Both kernels execute the same arithmetic
+// routine run_unit.  Both kernels execute the same arithmetic
 // in the same order for a given (graph, K, NB, split threshold), so a problem
 // computed alone and inside a batch gives bit-identical C.
 #pragma once
@@ -44,6 +44,7 @@ struct SpmmArgs {
   // would otherwise leak into C; reading a separate zero buffer instead was
   // measured slower on gfx950)
   uint32_t zero_row;
+  uint32_t b_rows;    // rows of the gathered B (buffer loads past them read zeros)
   uint32_t pad_tile;  // the all-padding tile (index T) past the real tiles
   uint32_t ldb, ldc;
   uint32_t rows, K;
@@ -138,18 +139,97 @@ struct BRow {
   uint32_t w[kWords];
 };
 
-// Branch-free gather of one B row segment.  A padded column (kPad) reads the
-// converted B's zero row (BF16 forms) or row 0 of B (FP32 form, masked when
-// the stage is consumed).  Every lane issues the same
+// Where a lane's B row segments come from.  BUF: buffer loads through rsrc
+// (num_records = b_rows * row_bytes) at 32-bit offsets c * row_bytes +
+// feature bytes (one v_mad_u32_u24): a padded column (kPad) is clamped to
+// row b_rows, just past the end, and the hardware returns zeros for it, so
+// no mask and no zero row are needed.  Used when every offset fits 32 bits
+// (buffer_ok).  Otherwise 64-bit addresses from B with stride ldb elements:
+// padded columns read the converted B's zero row (BF16 form) or row 0 of
+// the caller's B (masked when consumed if that row is not finite).
+struct BSrc {
+  __amdgpu_buffer_rsrc_t rsrc;
+  const void* B;
+  uint32_t ldb;        // elements
+  uint32_t row_bytes;  // ldb * element size
+  uint32_t b_rows;     // rows of the gathered B
+  uint32_t zero_row;   // 64-bit path, BF16 form: the all-zero row
+};
+
+// Whether every B offset of the BUF path fits its 32-bit arithmetic: rows
+// and row bytes within v_mad_u32_u24's 24-bit operands, and the clamped
+// row's segment below 2^32.
+__host__ __device__ inline bool buffer_ok(uint64_t b_rows, uint64_t row_bytes) {
+  return b_rows < (1u << 24) && row_bytes < (1u << 24) &&
+         (b_rows + 1) * row_bytes <= 0xFFFFFF00ull;
+}
+
+template <int FORM, int NB, bool BUF>
+__device__ inline BSrc make_bsrc(const void* B, uint32_t ldb, uint32_t b_rows,
+                                 uint32_t zero_row) {
+  BSrc s;
+  constexpr uint32_t es = FORM == kBF16 ? 2u : 4u;
+  s.B = B;
+  s.ldb = ldb;
+  s.row_bytes = ldb * es;
+  s.b_rows = b_rows;
+  s.zero_row = zero_row;
+  if (BUF)
+    s.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(B), (short)0,
+                                               (int)(b_rows * s.row_bytes), 0x00020000);
+  return s;
+}
+
+// Branch-free gather of one B row segment.  Every lane issues the same
 // vector loads and no result is touched until its stage is consumed, which
 // keeps the count of outstanding loads static: the compiler can then wait for
 // one pipeline stage without draining the younger ones.
-template <int FORM, int NB, bool FULL>
-__device__ inline void load_brow(BRow<FORM, NB>& r, const void* B, uint32_t ldb,
-                                 uint32_t col, uint32_t feat, uint32_t K,
-                                 uint32_t zero_row) {
-  const uint32_t c = col != kPad ? col : zero_row;
+template <int FORM, int NB, bool FULL, bool BUF>
+__device__ inline void load_brow(BRow<FORM, NB>& r, const BSrc& src, uint32_t col,
+                                 uint32_t feat, uint32_t K) {
+  constexpr uint32_t es = FORM == kBF16 ? 2u : 4u;
+  if constexpr (BUF) {
+    const uint32_t off = __umul24(min(col, src.b_rows), src.row_bytes) + feat * es;
+    if constexpr (FORM == kBF16) {
+      if constexpr (NB == 8) {
+        const u4 v = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(src.rsrc, off, 0, 0));
+        r.w[0] = v[0]; r.w[1] = v[1]; r.w[2] = v[2]; r.w[3] = v[3];
+      } else if constexpr (NB == 4) {
+        const uint2 v = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(src.rsrc, off, 0, 0));
+        r.w[0] = v.x; r.w[1] = v.y;
+      } else if constexpr (NB == 2) {
+        r.w[0] = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc, off, 0, 0);
+      } else {
+        r.w[0] = __builtin_amdgcn_raw_buffer_load_b16(src.rsrc, off, 0, 0);
+      }
+    } else if constexpr (FULL) {
+      if constexpr (NB >= 4) {
+#pragma unroll
+        for (int i = 0; i < NB / 4; ++i) {
+          const u4 v = __builtin_bit_cast(
+              u4, __builtin_amdgcn_raw_buffer_load_b128(src.rsrc, off + 16u * i, 0, 0));
+          r.w[4 * i] = v[0]; r.w[4 * i + 1] = v[1];
+          r.w[4 * i + 2] = v[2]; r.w[4 * i + 3] = v[3];
+        }
+      } else if constexpr (NB == 2) {
+        const uint2 v = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(src.rsrc, off, 0, 0));
+        r.w[0] = v.x; r.w[1] = v.y;
+      } else {
+        r.w[0] = __builtin_amdgcn_raw_buffer_load_b32(src.rsrc, off, 0, 0);
+      }
+    } else {
+      // feature tail (FP32 B): features past K read past the end (zeros)
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        r.w[i] = __builtin_amdgcn_raw_buffer_load_b32(
+            src.rsrc, feat + i < K ? off + 4u * i : 0xFFFFFFFFu, 0, 0);
+    }
+    return;
+  }
+  const void* B = src.B;
+  const uint32_t ldb = src.ldb;
   if constexpr (FORM == kBF16) {
+    const uint32_t c = col != kPad ? col : src.zero_row;
     const uint16_t* p = static_cast<const uint16_t*>(B) + (uint64_t)c * ldb + feat;
     if constexpr (NB == 8) {
       u4 v = *reinterpret_cast<const u4*>(p);
@@ -292,11 +372,10 @@ __device__ inline void load_meta(Meta& mt, const SpmmArgs& g, const GroupPos& p,
 
 // Gathers the B rows of the tile's 8 columns and the lane's share of the
 // group's value span; records the lane's A row (bitmap byte, first value).
-template <int FORM, int NB, bool FULL, int RW>
+template <int FORM, int NB, bool FULL, int RW, bool BUF>
 __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
                                  const SpmmArgs& g, uint32_t n, bool a_rows,
-                                 uint32_t feat, const void* B, uint32_t ldb,
-                                 uint32_t lane) {
+                                 uint32_t feat, const BSrc& src, uint32_t lane) {
   // columns 0-3 come from the even lane of the pair, 4-7 from the odd one;
   // the bitmap from the even lane, tcOffset[t] from the odd one
   uint32_t col[8];
@@ -319,7 +398,7 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
   uint32_t bm = 0;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    load_brow<FORM, NB, FULL>(st.b[j], B, ldb, col[j], feat, g.K, g.zero_row);
+    load_brow<FORM, NB, FULL, BUF>(st.b[j], src, col[j], feat, g.K);
     if (raw_b(FORM)) bm |= (col[j] != kPad ? 1u : 0u) << j;
   }
   st.bmask = bm;
@@ -343,15 +422,14 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
   if (RW == 16) st.vbase = vbase;
 }
 
-// MFMAs of one 4-tile group.  mask (wave-uniform): the forms that gather
-// the caller's FP32 B zero what padded columns loaded from B's row 0, needed
-// only when that row holds an infinity or NaN (the A value there is zero,
-// and 0 x inf is NaN).
-template <int FORM, int NB, int RW>
+// MFMAs of one 4-tile group.  MASK: the 64-bit gather path of the forms
+// that gather the caller's FP32 B zeroes what padded columns loaded from
+// B's row 0, needed only when that row holds an infinity or NaN (the A value
+// there is zero, and 0 x inf is NaN).
+template <int FORM, int NB, int RW, bool MASK>
 __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
                                      uint32_t m, float* lv, uint32_t lane,
-                                     const uint32_t* __restrict__ vsrc, bool wide,
-                                     bool mask) {
+                                     const uint32_t* __restrict__ vsrc, bool wide) {
   // the group's value span through the wave's LDS buffer: a set bitmap bit
   // j reads the row's next value, a clear one the zero after the values
   *reinterpret_cast<u4*>(lv + 4u * lane) = st.vals;
@@ -377,7 +455,7 @@ __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
     const uint32_t rank = __popc(st.byte & ((1u << j) - 1u));
     a[j] = __builtin_bit_cast(uint32_t, lv[on ? st.base + rank : group_values<RW>()]);
   }
-  if (raw_b(FORM) && mask) {
+  if (raw_b(FORM) && MASK) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const bool bv = (st.bmask >> j) & 1u;
@@ -565,36 +643,35 @@ struct Accum {
 // three groups run as straight-line tail code.  sched_barrier keeps each
 // stage's loads issued before the previous stage's consumer code (the
 // scheduler would otherwise hoist the cheap masking and force early waits).
-template <int FORM, int NB, int RW, bool FULL, bool PACKED>
+template <int FORM, int NB, int RW, bool FULL, bool PACKED, bool BUF, bool MASK>
 __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
                                  GroupPos pos, uint32_t ngroups, uint32_t q,
                                  uint32_t m, bool a_rows, uint32_t feat,
-                                 const void* B, uint32_t ldb, float* lv,
-                                 uint32_t lane, bool mask) {
+                                 const BSrc& src, float* lv, uint32_t lane) {
   Meta mA, mB;
   Stage<FORM, NB> dA, dB;
   const uint32_t* vsrc = reinterpret_cast<const uint32_t*>(g.data);
   auto consume = [&](Stage<FORM, NB>& d) {
     if (PACKED && out.direct) out.move_to(d.win);
-    compute_stage<FORM, NB, RW>(d, out.acc, m, lv, lane, vsrc, g.wide != 0u, mask);
+    compute_stage<FORM, NB, RW, MASK>(d, out.acc, m, lv, lane, vsrc, g.wide != 0u);
   };
   // pos: position of the most recently loaded metadata
   load_meta<RW>(mA, g, pos, true, q, m);
   if (ngroups > 1) next_group<PACKED>(pos, g.rwo);
   load_meta<RW>(mB, g, pos, ngroups > 1, q, m);
-  load_data<FORM, NB, FULL, RW>(dA, mA, g, m, a_rows, feat, B, ldb, lane);
+  load_data<FORM, NB, FULL, RW, BUF>(dA, mA, g, m, a_rows, feat, src, lane);
   uint32_t gi = 0;
   for (; gi + 3 < ngroups; gi += 2) {
     next_group<PACKED>(pos, g.rwo);  // group gi + 2
     load_meta<RW>(mA, g, pos, true, q, m);
-    load_data<FORM, NB, FULL, RW>(dB, mB, g, m, a_rows, feat, B, ldb, lane);
+    load_data<FORM, NB, FULL, RW, BUF>(dB, mB, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
     __builtin_amdgcn_sched_barrier(0);
     const bool v3 = gi + 3 < ngroups;
     if (v3) next_group<PACKED>(pos, g.rwo);  // group gi + 3
     load_meta<RW>(mB, g, pos, v3, q, m);
-    load_data<FORM, NB, FULL, RW>(dA, mA, g, m, a_rows, feat, B, ldb, lane);
+    load_data<FORM, NB, FULL, RW, BUF>(dA, mA, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dB);
     __builtin_amdgcn_sched_barrier(0);
@@ -604,16 +681,16 @@ __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
   if (left == 3) {
     next_group<PACKED>(pos, g.rwo);  // group gi + 2
     load_meta<RW>(mA, g, pos, true, q, m);
-    load_data<FORM, NB, FULL, RW>(dB, mB, g, m, a_rows, feat, B, ldb, lane);
+    load_data<FORM, NB, FULL, RW, BUF>(dB, mB, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
     __builtin_amdgcn_sched_barrier(0);
-    load_data<FORM, NB, FULL, RW>(dA, mA, g, m, a_rows, feat, B, ldb, lane);
+    load_data<FORM, NB, FULL, RW, BUF>(dA, mA, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dB);
     consume(dA);
   } else if (left == 2) {
-    load_data<FORM, NB, FULL, RW>(dB, mB, g, m, a_rows, feat, B, ldb, lane);
+    load_data<FORM, NB, FULL, RW, BUF>(dB, mB, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
     consume(dB);
@@ -639,8 +716,9 @@ __device__ inline bool row0_needs_mask(const void* B, uint32_t feat, uint32_t K)
 
 // PACKED: the schedule may hold multi-window units (the loop then flushes C
 // at window changes); without it every direct unit is one window and the
-// loop carries no flush code (fewer registers).
-template <int FORM, int NB, int RW, bool PACKED>
+// loop carries no flush code (fewer registers).  BUF: B is gathered by
+// buffer loads (g.b_rows rows; buffer_ok holds).
+template <int FORM, int NB, int RW, bool PACKED, bool BUF>
 __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
                          const void* B, uint32_t ldb, float* C, uint32_t ldc,
                          uint32_t K, uint32_t rows, float* lv) {
@@ -679,13 +757,30 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
     // packed unit: group ranges are whole windows (its first window is
     // not empty when it has groups)
     if (u.slot < 0) pos.tend = g.rwo[pos.w + 1];
-    const bool mask = row0_needs_mask<FORM, NB>(B, feat, K);
-    if (full)
-      unit_loop<FORM, NB, RW, true, PACKED>(g, out, pos, ngroups, q, m, a_rows, feat, B, ldb,
-                                            lv, lane, mask);
-    else
-      unit_loop<FORM, NB, RW, false, PACKED>(g, out, pos, ngroups, q, m, a_rows, feat, B, ldb,
-                                             lv, lane, mask);
+    const BSrc src = make_bsrc<FORM, NB, BUF>(B, ldb, g.b_rows, g.zero_row);
+    if (BUF) {
+      // padded columns read zeros past the end of the buffer: no mask
+      if (full)
+        unit_loop<FORM, NB, RW, true, PACKED, BUF, false>(g, out, pos, ngroups, q, m, a_rows,
+                                                          feat, src, lv, lane);
+      else
+        unit_loop<FORM, NB, RW, false, PACKED, BUF, false>(g, out, pos, ngroups, q, m, a_rows,
+                                                           feat, src, lv, lane);
+    } else if (row0_needs_mask<FORM, NB>(B, feat, K)) {
+      if (full)
+        unit_loop<FORM, NB, RW, true, PACKED, BUF, true>(g, out, pos, ngroups, q, m, a_rows,
+                                                         feat, src, lv, lane);
+      else
+        unit_loop<FORM, NB, RW, false, PACKED, BUF, true>(g, out, pos, ngroups, q, m, a_rows,
+                                                          feat, src, lv, lane);
+    } else {
+      if (full)
+        unit_loop<FORM, NB, RW, true, PACKED, BUF, false>(g, out, pos, ngroups, q, m, a_rows,
+                                                          feat, src, lv, lane);
+      else
+        unit_loop<FORM, NB, RW, false, PACKED, BUF, false>(g, out, pos, ngroups, q, m, a_rows,
+                                                           feat, src, lv, lane);
+    }
   }
 
   if (u.slot < 0) {

@@ -105,15 +105,17 @@ __device__ inline void sweep_next(GroupPos& p, const uint32_t* __restrict__ piec
 // The work-unit kernel's three-stage pipeline (metadata g+2, gathers g+1,
 // MFMAs g) over the wave's piece list; a group of a new piece first ends
 // the previous piece into its window's LDS slot.
-template <int FORM, int NB, int RW, bool FULL>
+template <int FORM, int NB, int RW, bool FULL, bool MASK>
 __device__ void sweep_loop(const SweepArgs& a, f4* acc, f4* slots, GroupPos pos,
                            uint32_t ngroups, uint32_t lane, uint32_t q,
                            uint32_t m, bool a_rows, uint32_t feat,
-                           uint32_t& started, float* lv, bool mask) {
+                           uint32_t& started, float* lv) {
   const SpmmArgs& g = a.g;
   constexpr uint32_t NL = slot_lanes<RW>();
-  const void* B = g.B;
-  const uint32_t ldb = g.ldb;
+  // the sweep gathers B with 64-bit addresses (the same C bits as the
+  // buffer loads of the work-unit kernel: padded columns contribute zeros
+  // either way)
+  const BSrc src = make_bsrc<FORM, NB, false>(g.B, g.ldb, g.b_rows, g.zero_row);
   uint32_t cur = pos.w;
   auto end_piece = [&](uint32_t pi) {
     const uint32_t info = __builtin_amdgcn_readfirstlane(a.pieces[3ull * pi + 2]);
@@ -130,24 +132,24 @@ __device__ void sweep_loop(const SweepArgs& a, f4* acc, f4* slots, GroupPos pos,
       end_piece(cur);
       cur = d.win;
     }
-    compute_stage<FORM, NB, RW>(d, acc, m, lv, lane, vsrc, g.wide != 0u, mask);
+    compute_stage<FORM, NB, RW, MASK>(d, acc, m, lv, lane, vsrc, g.wide != 0u);
   };
   load_meta<RW>(mA, g, pos, true, q, m);
   if (ngroups > 1) sweep_next(pos, a.pieces);
   load_meta<RW>(mB, g, pos, ngroups > 1, q, m);
-  load_data<FORM, NB, FULL, RW>(dA, mA, g, m, a_rows, feat, B, ldb, lane);
+  load_data<FORM, NB, FULL, RW, false>(dA, mA, g, m, a_rows, feat, src, lane);
   uint32_t gi = 0;
   for (; gi + 3 < ngroups; gi += 2) {
     sweep_next(pos, a.pieces);  // group gi + 2
     load_meta<RW>(mA, g, pos, true, q, m);
-    load_data<FORM, NB, FULL, RW>(dB, mB, g, m, a_rows, feat, B, ldb, lane);
+    load_data<FORM, NB, FULL, RW, false>(dB, mB, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
     __builtin_amdgcn_sched_barrier(0);
     const bool v3 = gi + 3 < ngroups;
     if (v3) sweep_next(pos, a.pieces);  // group gi + 3
     load_meta<RW>(mB, g, pos, v3, q, m);
-    load_data<FORM, NB, FULL, RW>(dA, mA, g, m, a_rows, feat, B, ldb, lane);
+    load_data<FORM, NB, FULL, RW, false>(dA, mA, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dB);
     __builtin_amdgcn_sched_barrier(0);
@@ -156,16 +158,16 @@ __device__ void sweep_loop(const SweepArgs& a, f4* acc, f4* slots, GroupPos pos,
   if (left == 3) {
     sweep_next(pos, a.pieces);
     load_meta<RW>(mA, g, pos, true, q, m);
-    load_data<FORM, NB, FULL, RW>(dB, mB, g, m, a_rows, feat, B, ldb, lane);
+    load_data<FORM, NB, FULL, RW, false>(dB, mB, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
     __builtin_amdgcn_sched_barrier(0);
-    load_data<FORM, NB, FULL, RW>(dA, mA, g, m, a_rows, feat, B, ldb, lane);
+    load_data<FORM, NB, FULL, RW, false>(dA, mA, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dB);
     consume(dA);
   } else if (left == 2) {
-    load_data<FORM, NB, FULL, RW>(dB, mB, g, m, a_rows, feat, B, ldb, lane);
+    load_data<FORM, NB, FULL, RW, false>(dB, mB, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
     consume(dB);
@@ -212,12 +214,18 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
         GroupPos pos{pb, (uint32_t)__builtin_amdgcn_readfirstlane(a.pieces[3ull * pb]),
                      (uint32_t)__builtin_amdgcn_readfirstlane(a.pieces[3ull * pb + 1])};
         const bool mask = row0_needs_mask<FORM, NB>(g.B, feat, g.K);
-        if (full)
-          sweep_loop<FORM, NB, RW, true>(a, acc, slots, pos, ng, lane, q, m, a_rows, feat,
-                                         started, lv, mask);
+        if (full && mask)
+          sweep_loop<FORM, NB, RW, true, true>(a, acc, slots, pos, ng, lane, q, m, a_rows, feat,
+                                               started, lv);
+        else if (full)
+          sweep_loop<FORM, NB, RW, true, false>(a, acc, slots, pos, ng, lane, q, m, a_rows,
+                                                feat, started, lv);
+        else if (mask)
+          sweep_loop<FORM, NB, RW, false, true>(a, acc, slots, pos, ng, lane, q, m, a_rows,
+                                                feat, started, lv);
         else
-          sweep_loop<FORM, NB, RW, false>(a, acc, slots, pos, ng, lane, q, m, a_rows, feat,
-                                          started, lv, mask);
+          sweep_loop<FORM, NB, RW, false, false>(a, acc, slots, pos, ng, lane, q, m, a_rows,
+                                                 feat, started, lv);
       }
       for (uint32_t s = 0; s < a.M; ++s) {
         const uint32_t w = __builtin_amdgcn_readfirstlane(wl[s]);

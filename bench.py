@@ -16,7 +16,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import statistics
 import sys
@@ -28,32 +27,12 @@ sys.path.insert(0, str(REPO))
 
 HBM_PEAK_GBS = 8000.0
 
-WORKLOADS = {
-    # 232,965-node Chung-Lu, exponent 2.1, mean degree 492, 1/deg values
-    "chung_lu_233k": dict(kind="chung_lu", n=232965, deg=492.0, gamma=2.1, seed=11,
-                          values=1, bseed=12),
-    # 100^3 27-point stencil, U[-1,1] values
-    "stencil27_100": dict(kind="stencil", g=100, seed=13, bseed=14),
-    # 16,384-node Chung-Lu, exponent 2.1, mean degree 16, U(0,1] symmetric values
-    "chung_lu_16k": dict(kind="chung_lu", n=16384, deg=16.0, gamma=2.1, seed=1,
-                         values=0, bseed=2),
-    # one of the serving pool's two 2,000,000-node graphs: mean degree 32,
-    # exponent 2.1, U(0,1] values (graph seed 3100)
-    "chung_lu_2m": dict(kind="chung_lu", n=2000000, deg=32.0, gamma=2.1, seed=3100,
-                        values=0, bseed=3101),
-}
+from accspmm_amd.workloads import (SERVING_KS, WORKLOADS, batch_specs, make_graph,  # noqa: E402
+                                   poisson_gaps, serving_pool, serving_requests)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def make_graph(spec):
-    import accspmm_amd as acc
-    if spec["kind"] == "chung_lu":
-        return acc.gen_chung_lu(spec["n"], spec["deg"], spec["gamma"], spec["seed"],
-                                spec["values"])
-    return acc.gen_stencil27(spec["g"], spec["seed"])
 
 
 def alg_bytes(M, N, nnz, K, form):
@@ -284,16 +263,12 @@ def batch_bench(ngraphs: int, steps: int, warmup: int):
     import torch
 
     import accspmm_amd as acc
-    rng = np.random.default_rng(1000)
     probs, flops = [], 0.0
-    for i in range(ngraphs):
-        n = int(round(math.exp(rng.uniform(math.log(500), math.log(50000)))))
-        deg = float(rng.uniform(4, 64))
-        gam = float(rng.uniform(2.0, 2.5))
-        K = int(rng.choice([16, 64, 128]))
-        csr = acc.gen_chung_lu(n, deg, gam, 1000 + i, 0)
+    for sp in batch_specs(ngraphs):
+        K = sp["K"]
+        csr = acc.gen_chung_lu(sp["n"], sp["deg"], sp["gamma"], sp["seed"], 0)
         g = acc.DeviceGraph(csr, acc.default_build_opts(8, build_adpbme=0))
-        B = torch.from_numpy(acc.gen_dense(2000 + i, csr.cols, K)).cuda()
+        B = torch.from_numpy(acc.gen_dense(sp["bseed"], csr.cols, K)).cuda()
         probs.append((g, B, torch.empty((csr.rows, K), device="cuda"),
                       torch.empty((csr.rows, K), device="cuda")))
         flops += 2.0 * csr.nnz * K
@@ -350,30 +325,19 @@ def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000,
 
     import accspmm_amd as acc
     opts = acc.default_build_opts(row_window, reorder_rows=reorder)
-    rng = np.random.default_rng(3000)
-    pool = []
-    for i in range(64):
-        n = int(round(math.exp(rng.uniform(math.log(500), math.log(50000)))))
-        pool.append(acc.gen_chung_lu(n, float(rng.uniform(4, 64)),
-                                     float(rng.uniform(2.0, 2.5)), 3000 + i, 0))
-    for i in range(2):
-        pool.append(acc.gen_chung_lu(big_nodes, 32.0, 2.1, 3100 + i, 0))
+    pool = serving_pool(big_nodes=big_nodes)
     log(f"serve: pool of {len(pool)} graphs, {sum(c.nnz for c in pool)} nnz")
-    Ks = (32, 128)
     bufs = {}
     for gi, c in enumerate(pool):
-        for K in Ks:
+        for K in SERVING_KS:
             B = torch.rand((c.cols, K), device="cuda") * 2 - 1
             Cs = [torch.empty((c.rows, K), device="cuda") for _ in range(2)]
             bufs[(gi, K)] = (B, Cs)
-    req_rng = np.random.default_rng(seed)
-    gsel = req_rng.integers(0, len(pool), size=requests)
-    ksel = req_rng.choice(Ks, size=requests)
+    gsel, ksel = serving_requests(len(pool), requests, seed)
     flops = np.array([2.0 * pool[g].nnz * k for g, k in zip(gsel, ksel)])
 
     def run(server, qps):
-        arr_rng = np.random.default_rng(5000)
-        gaps = arr_rng.exponential(1.0 / qps, size=requests) if qps else np.zeros(requests)
+        gaps = poisson_gaps(requests, qps)
         rids = []
         t0 = time.perf_counter()
         t_next = t0

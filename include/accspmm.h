@@ -68,7 +68,9 @@ typedef struct accspmm_build_opts {
   uint32_t col_window;          /* must be 8                                */
   uint32_t t_load, t_comp, t_write, target_num; /* AdpBME cost model        */
   uint32_t imbalance_threshold; /* balanceStrategy threshold                */
-  uint32_t build_adpbme;        /* 1: also build AdpBME groups              */
+  uint32_t build_adpbme;        /* 1: also build AdpBME groups for export
+                                   (default 0: no kernel reads them; the
+                                   AdpBME drop-in class sets 1)           */
   /* 0 (default): rows kept in input order, the reference's format.  1: the
    * format is built from the rows in order of decreasing length (stable),
    * so 8-row windows of like rows share more columns: fewer tiles and fewer

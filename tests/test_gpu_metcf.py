@@ -12,7 +12,9 @@ K = kat()
 
 
 def _build(rows, cols, rp, ci, va, rw=8, **kw):
+    # AdpBME groups are built on request (they are exported, not computed on)
     csr = acc.CSR(rows, cols, rp, ci, va)
+    kw.setdefault("build_adpbme", 1)
     g = acc.DeviceGraph(csr, acc.default_build_opts(rw, **kw))
     return g, g.export()
 
@@ -87,7 +89,7 @@ def test_adpbme_cost_params(t):
 
 def test_chung_lu_matches_oracle():
     csr = acc.gen_chung_lu(16384, 16.0, 2.1, 1, 0)
-    g = acc.DeviceGraph(csr)
+    g = acc.DeviceGraph(csr, acc.default_build_opts(8, build_adpbme=1))
     got = g.export()
     exp = O.adpbme_build(8, csr.rows, csr.row_ptr, csr.col_idx, csr.data)
     _assert_same(got, exp, ALL)
@@ -96,7 +98,7 @@ def test_chung_lu_matches_oracle():
 def test_stencil_matches_oracle():
     csr = acc.gen_stencil27(20, 13)
     for rw in (8, 16):
-        g = acc.DeviceGraph(csr, acc.default_build_opts(rw))
+        g = acc.DeviceGraph(csr, acc.default_build_opts(rw, build_adpbme=1))
         exp = O.adpbme_build(rw, csr.rows, csr.row_ptr, csr.col_idx, csr.data)
         _assert_same(g.export(), exp, ALL)
 
@@ -129,7 +131,7 @@ def test_rejects_non_canonical():
 def test_empty_matrix_rows_without_nonzeros():
     rp = np.zeros(20, np.uint32)
     csr = acc.CSR(19, 7, rp, np.zeros(0, np.uint32), np.zeros(0, np.float32))
-    g = acc.DeviceGraph(csr)
+    g = acc.DeviceGraph(csr, acc.default_build_opts(8, build_adpbme=1))
     got = g.export()
     exp = O.adpbme_build(8, 19, rp, [], [])
     _assert_same(got, exp, ALL)

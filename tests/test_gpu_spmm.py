@@ -272,6 +272,31 @@ def test_row0_nonfinite_masked(form):
         assert O.errors(got.cpu().numpy(), C64, ab)[0] <= TOL[form]
 
 
+@pytest.mark.parametrize("form", [acc.FORM_SPLIT_BF16, acc.FORM_BF16], ids=acc.FORM_NAMES.get)
+def test_two_streams_share_a_handle(form):
+    # one handle used from two streams at once with different B: each
+    # stream's calls keep their own workspace (split-window partials, the
+    # BF16 form's converted B), so every C equals its standalone result
+    torch = _torch()
+    rng = np.random.default_rng(64)
+    rows, cols, K = 3000, 4000, 96
+    rp, ci, va = random_csr(rng, rows, cols, 0.02, empty_rows=0.05, heavy_rows=6)
+    g = acc.DeviceGraph(acc.CSR(rows, cols, rp, ci, va))
+    tile = acc.Tile(2, 4, 8, 0, 64, 2)  # split windows and column panels
+    Bs = [torch.from_numpy(acc.gen_dense(s, cols, K)).cuda() for s in (1, 2)]
+    refs = [g.spmm(B, None, form, tile).clone() for B in Bs]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = [[torch.full((rows, K), float("nan"), device="cuda") for _ in range(8)]
+            for _ in Bs]
+    for rep in range(8):
+        for s, B, Cl in zip(streams, Bs, outs):
+            g.spmm(B, Cl[rep], form, tile, stream=s)
+    torch.cuda.synchronize()
+    for Cl, R in zip(outs, refs):
+        assert all(torch.equal(C, R) for C in Cl)
+
+
 def test_unaligned_b_view_narrows_nb():
     # a B view with ldb % NB != 0 cannot take NB-wide loads: the library
     # narrows NB (ldb = 66 -> NB 2), within tolerance of the oracle
