@@ -53,6 +53,7 @@ class Status:
     ERR_NOT_READY = 10
     ERR_SHAPE = 11
     ERR_FILE = 12
+    ERR_SCRATCH = 13
 
 
 class AccSpMMError(RuntimeError):
@@ -147,6 +148,9 @@ _SIGS = {
     "accspmm_spmm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Tile),
                                C.c_void_p]),
+    "accspmm_spmm_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                       C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Tile),
+                                       C.c_void_p]),
     "accspmm_spmm_prepare_b": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                          C.c_int, C.POINTER(Tile), C.c_void_p]),
     "accspmm_spmm_prepared": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,

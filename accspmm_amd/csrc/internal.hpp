@@ -100,6 +100,7 @@ struct Schedule {
   DevBuf units;             // WorkUnit[num_units]
   DevBuf split_info;        // per split window: window, first slot, piece count
   std::vector<WorkUnit> h_units;  // host copy (the sweep plan is derived from it)
+  std::vector<uint32_t> h_sinfo;  // host copy of split_info
 };
 
 // Workspace of one accspmm_spmm call: split-window partials and the BF16
@@ -145,6 +146,10 @@ struct accspmm_metcf {
   mutable std::map<std::tuple<uint32_t, uint32_t, uint32_t>,
                    std::unique_ptr<accspmm::Schedule>>
       schedules;
+  // host-only schedules of accspmm_spmm_noalloc (no device buffers), same key
+  mutable std::map<std::tuple<uint32_t, uint32_t, uint32_t>,
+                   std::unique_ptr<accspmm::Schedule>>
+      host_schedules;
   // keyed by (split_tiles, pack_tiles, col_panels, waves, slots)
   mutable std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>,
                    std::unique_ptr<accspmm::SweepPlan>>

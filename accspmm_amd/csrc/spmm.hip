@@ -152,18 +152,19 @@ constexpr uint32_t kPanelMinTiles = 8;
 //     `pack_tiles` tiles (a heavy window closes the open unit first); runs
 //     of empty windows form units of their own, which write zero rows.
 // Unit order: whole-window units first, then panel 0's pieces, panel 1's...
-accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
-                            hipStream_t s, const Schedule** out) {
+// plan_schedule fills the host side (h_units, h_sinfo, counts); the panel
+// cut search runs on the GPU into `bounds` (W * (P - 1) words of device
+// memory the caller provides).
+uint64_t panel_bound_words(const accspmm_metcf* h, const accspmm_tile& tile) {
+  const uint32_t P = std::max<uint32_t>(1, std::min<uint32_t>(tile.col_panels, 1024));
+  return P > 1 && h->W > 0 && h->T > 0 ? (uint64_t)h->W * (P - 1) : 0;
+}
+
+accspmm_status plan_schedule(const accspmm_metcf* h, const accspmm_tile& tile, hipStream_t s,
+                             uint32_t* bounds_dev, Schedule* sch) {
   const uint32_t split_tiles = tile.split_tiles ? tile.split_tiles : 0xFFFFFFFFu;
   const uint32_t pack_tiles = tile.pack_tiles;
   const uint32_t P = std::max<uint32_t>(1, std::min<uint32_t>(tile.col_panels, 1024));
-  const auto key = std::make_tuple(split_tiles, pack_tiles, P);
-  auto it = h->schedules.find(key);
-  if (it != h->schedules.end()) {
-    *out = it->second.get();
-    return ACCSPMM_OK;
-  }
-  auto sch = std::make_unique<Schedule>();
   sch->split_tiles = split_tiles;
   sch->pack_tiles = pack_tiles;
   sch->col_panels = P;
@@ -172,18 +173,16 @@ accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
   // panel cut positions (tiles below each boundary), per window
   std::vector<uint32_t> bounds;
   const uint32_t pc = (uint32_t)(((uint64_t)h->cols + P - 1) / P);
-  if (P > 1 && h->W > 0 && h->T > 0) {
-    const uint64_t n = (uint64_t)h->W * (P - 1);
-    DevBuf d;
-    ACC_TRY(d.alloc(n * 4));
-    const uint64_t blocks = (n + 255) / 256;
+  const uint64_t nb = panel_bound_words(h, tile);
+  if (nb) {
+    const uint64_t blocks = (nb + 255) / 256;
     if (blocks > 0x7FFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
     hipLaunchKernelGGL(k_panel_bounds, dim3((uint32_t)blocks), dim3(256), 0, s,
                        h->rwo.as<uint32_t>(), h->a2b.as<uint32_t>(), h->W, P,
-                       pc, d.as<uint32_t>());
+                       pc, bounds_dev);
     ACC_HIP(hipGetLastError());
-    bounds.resize(n);
-    ACC_HIP(hipMemcpyAsync(bounds.data(), d.p, n * 4, hipMemcpyDeviceToHost, s));
+    bounds.resize(nb);
+    ACC_HIP(hipMemcpyAsync(bounds.data(), bounds_dev, nb * 4, hipMemcpyDeviceToHost, s));
     ACC_HIP(hipStreamSynchronize(s));
   }
 
@@ -269,23 +268,70 @@ accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
   close_open();
   for (auto& v : by_panel) units.insert(units.end(), v.begin(), v.end());
   sch->num_units = (uint32_t)units.size();
-  sch->h_units = units;
   sch->packed = false;
   for (const WorkUnit& u : units) sch->packed |= (u.slot < 0 && u.nwin > 1 && u.ngroups > 0);
+  sch->h_units = std::move(units);
+  sch->h_sinfo = std::move(sinfo);
   sch->num_pieces = pieces;
   sch->num_split_windows = split_windows;
-  ACC_TRY(sch->units.alloc(std::max<size_t>(units.size(), 1) * sizeof(WorkUnit)));
-  if (!units.empty())
-    ACC_HIP(hipMemcpyAsync(sch->units.p, units.data(), units.size() * sizeof(WorkUnit),
+  return ACCSPMM_OK;
+}
+
+// Device bytes of a schedule's work units and split-window records.
+uint64_t schedule_unit_bytes(const Schedule& sch) {
+  return (uint64_t)std::max<size_t>(sch.h_units.size(), 1) * sizeof(WorkUnit);
+}
+uint64_t schedule_sinfo_bytes(const Schedule& sch) {
+  return (uint64_t)std::max<size_t>(sch.h_sinfo.size(), 3) * 4;
+}
+
+// Copies a schedule's work units and split-window records to device memory
+// (units: schedule_unit_bytes, sinfo: schedule_sinfo_bytes); asynchronous,
+// from the schedule's host vectors.
+accspmm_status upload_schedule(const Schedule& sch, void* units, void* sinfo, hipStream_t s) {
+  if (!sch.h_units.empty())
+    ACC_HIP(hipMemcpyAsync(units, sch.h_units.data(), sch.h_units.size() * sizeof(WorkUnit),
                            hipMemcpyHostToDevice, s));
-  ACC_TRY(sch->split_info.alloc(std::max<size_t>(sinfo.size(), 3) * 4));
-  if (!sinfo.empty())
-    ACC_HIP(hipMemcpyAsync(sch->split_info.p, sinfo.data(), sinfo.size() * 4,
+  if (!sch.h_sinfo.empty())
+    ACC_HIP(hipMemcpyAsync(sinfo, sch.h_sinfo.data(), sch.h_sinfo.size() * 4,
                            hipMemcpyHostToDevice, s));
+  return ACCSPMM_OK;
+}
+
+// The handle's cached schedule for the tile's (split, pack, panels), built
+// and uploaded to the handle's own device buffers on first use.
+accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
+                            hipStream_t s, const Schedule** out) {
+  const uint32_t split_tiles = tile.split_tiles ? tile.split_tiles : 0xFFFFFFFFu;
+  const uint32_t P = std::max<uint32_t>(1, std::min<uint32_t>(tile.col_panels, 1024));
+  const auto key = std::make_tuple(split_tiles, tile.pack_tiles, P);
+  auto it = h->schedules.find(key);
+  if (it != h->schedules.end()) {
+    *out = it->second.get();
+    return ACCSPMM_OK;
+  }
+  auto sch = std::make_unique<Schedule>();
+  {
+    DevBuf bounds;
+    const uint64_t nb = panel_bound_words(h, tile);
+    if (nb) ACC_TRY(bounds.alloc(nb * 4));
+    ACC_TRY(plan_schedule(h, tile, s, bounds.as<uint32_t>(), sch.get()));
+  }
+  ACC_TRY(sch->units.alloc(schedule_unit_bytes(*sch)));
+  ACC_TRY(sch->split_info.alloc(schedule_sinfo_bytes(*sch)));
+  ACC_TRY(upload_schedule(*sch, sch->units.p, sch->split_info.p, s));
   ACC_HIP(hipStreamSynchronize(s));
   *out = sch.get();
   h->schedules[key] = std::move(sch);
   return ACCSPMM_OK;
+}
+
+// The handle's cached schedule for the tile, or null (no allocation).
+const Schedule* find_schedule(const accspmm_metcf* h, const accspmm_tile& tile) {
+  const uint32_t split_tiles = tile.split_tiles ? tile.split_tiles : 0xFFFFFFFFu;
+  const uint32_t P = std::max<uint32_t>(1, std::min<uint32_t>(tile.col_panels, 1024));
+  auto it = h->schedules.find(std::make_tuple(split_tiles, tile.pack_tiles, P));
+  return it == h->schedules.end() ? nullptr : it->second.get();
 }
 
 namespace {
@@ -455,40 +501,24 @@ accspmm_status stream_workspace(const accspmm_metcf* h, hipStream_t s, uint64_t 
   return ACCSPMM_OK;
 }
 
-// phases: bit 0 = convert B (BF16 form), bit 1 = SpMM kernel.  ws: the
-// call's workspace (at least spmm_workspace_need's bytes), or null for the
-// handle's workspace of stream s.
-accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
-                           float* C, uint32_t ldc, uint32_t K, int form,
-                           const accspmm_tile& tile, hipStream_t s,
-                           int phases, const SpmmWorkspace* ws) {
-  uint32_t nb = tile.feat_per_lane;
-  if (nb != 1 && nb != 2 && nb != 4 && nb != 8) return ACCSPMM_ERR_UNSUPPORTED;
+// One SpMM call on a given schedule: d_units / d_sinfo are its work units
+// and split-window records in device memory, ws the call's workspace (at
+// least spmm_workspace_need's bytes).  phases: bit 0 = convert B (BF16
+// form), bit 1 = SpMM kernel.  Caller holds h->mu.  Allocates nothing except
+// through the sweep (tile.sweep), whose plan the handle caches.
+accspmm_status spmm_run(const accspmm_metcf* h, const Schedule* sch, const WorkUnit* d_units,
+                        const uint32_t* d_sinfo, const float* B, uint32_t ldb, float* C,
+                        uint32_t ldc, uint32_t K, int form, const accspmm_tile& tile,
+                        hipStream_t s, int phases, const SpmmWorkspace& w) {
+  const uint32_t nb = tile.feat_per_lane;
   uint32_t wpb = tile.waves_per_block ? tile.waves_per_block : 4;
-  if (wpb > 8) return ACCSPMM_ERR_UNSUPPORTED;
   // NB = 8 kernels run one wave per SIMD (blocks of at most 4 waves), which
   // leaves them the whole register file, accumulation registers included
   if (nb == 8) wpb = std::min<uint32_t>(wpb, 4);
   const uint32_t KC = 16 * nb;
   const uint32_t n_kc = (K + KC - 1) / KC;
-  // FP32 B is read with NB-wide vector loads: needs alignment
-  if (raw_b(form)) {
-    if ((ldb % nb) != 0 || ((uintptr_t)B % (4u * nb)) != 0) return ACCSPMM_ERR_UNSUPPORTED;
-  }
-  if ((phases & 2) && ((ldc % nb) != 0 || ((uintptr_t)C % (4u * nb)) != 0))
-    return ACCSPMM_ERR_UNSUPPORTED;
-
-  std::lock_guard<std::mutex> lock(h->mu);
-  const Schedule* sch = nullptr;
-  ACC_TRY(get_schedule(h, tile, s, &sch));
   uint64_t pbytes = 0, cbytes = 0;
   spmm_workspace_need(h, sch, K, form, tile, &pbytes, &cbytes);
-  SpmmWorkspace w{};
-  if (ws) {
-    w = *ws;
-  } else {
-    ACC_TRY(stream_workspace(h, s, (phases & 2) ? pbytes : 0, cbytes, &w));
-  }
 
   SpmmArgs g{};
   g.rwo = h->rwo.as<uint32_t>();
@@ -496,8 +526,8 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   g.bits = h->bits.as<unsigned long long>();
   g.a2b = h->a2b.as<uint32_t>();
   g.data = h->data.as<float>();
-  g.units = sch->units.as<WorkUnit>();
-  g.split_info = sch->split_info.as<uint32_t>();
+  g.units = d_units;
+  g.split_info = d_sinfo;
   g.C = C;
   g.row_map = h->row_perm.p ? h->row_perm.as<uint32_t>() : nullptr;
   g.ldc = ldc;
@@ -537,15 +567,50 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
   if (sch->num_split_windows) {
     if (h->rw == 8)
       hipLaunchKernelGGL(k_reduce_split<8>, dim3(sch->num_split_windows), dim3(256), 0, s,
-                         sch->split_info.as<uint32_t>(), g.partials, g.kc_padded, K,
-                         C, ldc, h->rows, g.row_map);
+                         d_sinfo, g.partials, g.kc_padded, K, C, ldc, h->rows, g.row_map);
     else
       hipLaunchKernelGGL(k_reduce_split<16>, dim3(sch->num_split_windows), dim3(256), 0, s,
-                         sch->split_info.as<uint32_t>(), g.partials, g.kc_padded, K,
-                         C, ldc, h->rows, g.row_map);
+                         d_sinfo, g.partials, g.kc_padded, K, C, ldc, h->rows, g.row_map);
     ACC_HIP(hipGetLastError());
   }
   return ACCSPMM_OK;
+}
+
+// Operand checks shared by the entry points: NB-wide vector loads of the
+// caller's FP32 B and stores of C need their alignment.
+accspmm_status spmm_check_operands(const float* B, uint32_t ldb, const float* C, uint32_t ldc,
+                                   int form, const accspmm_tile& tile, int phases) {
+  const uint32_t nb = tile.feat_per_lane;
+  if (nb != 1 && nb != 2 && nb != 4 && nb != 8) return ACCSPMM_ERR_UNSUPPORTED;
+  if (tile.waves_per_block > 8) return ACCSPMM_ERR_UNSUPPORTED;
+  if (raw_b(form)) {
+    if ((ldb % nb) != 0 || ((uintptr_t)B % (4u * nb)) != 0) return ACCSPMM_ERR_UNSUPPORTED;
+  }
+  if ((phases & 2) && ((ldc % nb) != 0 || ((uintptr_t)C % (4u * nb)) != 0))
+    return ACCSPMM_ERR_UNSUPPORTED;
+  return ACCSPMM_OK;
+}
+
+// accspmm_spmm's body: the handle's cached schedule and, unless ws is
+// given, the handle's workspace of stream s.
+accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
+                           float* C, uint32_t ldc, uint32_t K, int form,
+                           const accspmm_tile& tile, hipStream_t s,
+                           int phases, const SpmmWorkspace* ws) {
+  ACC_TRY(spmm_check_operands(B, ldb, C, ldc, form, tile, phases));
+  std::lock_guard<std::mutex> lock(h->mu);
+  const Schedule* sch = nullptr;
+  ACC_TRY(get_schedule(h, tile, s, &sch));
+  uint64_t pbytes = 0, cbytes = 0;
+  spmm_workspace_need(h, sch, K, form, tile, &pbytes, &cbytes);
+  SpmmWorkspace w{};
+  if (ws) {
+    w = *ws;
+  } else {
+    ACC_TRY(stream_workspace(h, s, (phases & 2) ? pbytes : 0, cbytes, &w));
+  }
+  return spmm_run(h, sch, sch->units.as<WorkUnit>(), sch->split_info.as<uint32_t>(), B, ldb,
+                  C, ldc, K, form, tile, s, phases, w);
 }
 
 }  // namespace accspmm

@@ -301,6 +301,7 @@ extern "C" const char* accspmm_status_string(accspmm_status s) {
     case ACCSPMM_ERR_NOT_READY: return "not ready";
     case ACCSPMM_ERR_SHAPE: return "operand shapes do not agree";
     case ACCSPMM_ERR_FILE: return "file not writable, or METCF file unreadable, corrupt or of another format";
+    case ACCSPMM_ERR_SCRATCH: return "allocation-free call needs more than the scratch array holds";
   }
   return "unknown status";
 }

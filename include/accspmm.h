@@ -36,8 +36,10 @@ typedef enum accspmm_status {
   ACCSPMM_ERR_MTX = 9,            /* Matrix Market parse error                */
   ACCSPMM_ERR_NOT_READY = 10,     /* serving: request still in flight         */
   ACCSPMM_ERR_SHAPE = 11,         /* operand shapes do not agree              */
-  ACCSPMM_ERR_FILE = 12           /* file unwritable; METCF file unreadable,
+  ACCSPMM_ERR_FILE = 12,          /* file unwritable; METCF file unreadable,
                                      corrupt or of another version           */
+  ACCSPMM_ERR_SCRATCH = 13        /* accspmm_spmm_noalloc: the call needs more
+                                     than ACCSPMM_NOALLOC_SCRATCH_BYTES      */
 } accspmm_status;
 
 const char* accspmm_status_string(accspmm_status s);
@@ -232,6 +234,30 @@ accspmm_status accspmm_spmm_prepared(const accspmm_metcf* h, const float* B,
                                      uint32_t ldb, float* C, uint32_t ldc,
                                      uint32_t K, accspmm_form form,
                                      const accspmm_tile* tile, void* stream);
+
+/* Bytes of the per-GPU scratch array of accspmm_spmm_noalloc (64 MiB). */
+#define ACCSPMM_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* accspmm_spmm that allocates nothing: no hipMalloc, hipFree or other
+ * allocating routine, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).  No
+ * counterpart in the reference.  Same operands, tile choice, kernels and C
+ * bits as accspmm_spmm; the call's device buffers (split-window partials,
+ * the BF16 form's converted B and, when the handle holds no device schedule
+ * for the tile yet, the work units) live in one file-scope device array of
+ * ACCSPMM_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  A
+ * call that needs more returns ACCSPMM_ERR_SCRATCH (choose a tile with fewer
+ * split windows and column panels); tile.sweep is refused with
+ * ACCSPMM_ERR_UNSUPPORTED.  The first call for a tile on a handle without a
+ * device schedule plans it on the host (one stream synchronisation, host
+ * memory kept in the handle); later calls copy that plan into the array.
+ * All calls on one GPU share the array and keep the last call's contents:
+ * calls that overlap on one GPU race, so order them (one stream, or events
+ * between streams).  Asynchronous on `stream` otherwise. */
+accspmm_status accspmm_spmm_noalloc(const accspmm_metcf* h, const float* B,
+                                    uint32_t ldb, float* C, uint32_t ldc,
+                                    uint32_t K, accspmm_form form,
+                                    const accspmm_tile* tile, void* stream);
 
 /* Chooses the tile configuration the library would use for (h, K, form):
  * the autotuner's cached pick if one exists for this degree histogram and
