@@ -65,10 +65,11 @@ __device__ inline uint32_t last_le(const uint32_t* __restrict__ a, uint32_t n,
   return lo;
 }
 
-// Every problem's B conversion in one launch (BF16 form): the blocks of
-// problem p convert its chunks [(b - prefix[p]) * kConvChunks, ...) of the
-// (rows + 1) * ldx / 4 (convert_chunk, the single-graph conversion's code,
-// so the converted bits are the same).
+// Every problem's B conversion in one launch (the converted forms): the
+// blocks of problem p convert its chunks [(b - prefix[p]) * kConvChunks,
+// ...) of the (rows + 1) * ldx / 4 (convert_chunk, the single-graph
+// conversion's code, so the converted bits are the same).
+template <int FORM>
 __global__ __launch_bounds__(256) void k_batch_convert(
     const ConvDesc* __restrict__ d, uint32_t nconv,
     const uint32_t* __restrict__ prefix) {
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(256) void k_batch_convert(
   const uint64_t c0 = (uint64_t)(blockIdx.x - prefix[p]) * kConvChunks;
   const uint64_t c1 = min<uint64_t>(c0 + kConvChunks, total);
   for (uint64_t ch = c0 + threadIdx.x; ch < c1; ch += blockDim.x)
-    convert_chunk(c.src, c.rows, c.K, c.ldb, c.dst, c.ldx, c.vec, ch);
+    convert_chunk<FORM>(c.src, c.rows, c.K, c.ldb, c.dst, c.ldx, c.vec, ch);
 }
 
 // The persistent kernel of one (form, row window, NB) class: problems
@@ -174,10 +175,14 @@ accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
     ACC_TRY(get_schedule(h, t, s, &out->sched));
   }
   out->tasks = (uint64_t)out->sched->num_units * out->n_kc;
+  out->b_inreg = form == kSPLIT && t.b_split == 2;
   out->wpb = t.waves_per_block ? t.waves_per_block : 4;
   out->xcd = t.xcd_slice;
+  // the persistent kernel gathers a converted B in the split form (a batch
+  // of one problem whose tile splits B in registers leaves it unused)
   out->bconv_bytes =
-      form != kBF16 ? 0 : align256(((uint64_t)h->cols + 1) * out->kc_padded * 2);
+      form == kFP32 ? 0
+                    : align256(((uint64_t)h->cols + 1) * out->kc_padded * (form == kSPLIT ? 4 : 2));
   out->partial_bytes =
       align256((uint64_t)out->sched->num_pieces * h->rw * out->kc_padded * 4);
   return ACCSPMM_OK;
@@ -279,15 +284,15 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     g.rw = h->rw;
     g.nb = q.nb;
       g.zero_row = h->cols;
-    g.b_rows = form != kBF16 ? h->cols : h->cols + 1;
+    g.b_rows = form == kFP32 ? h->cols : h->cols + 1;
     g.pad_tile = h->T;
     g.wide = h->rw == 16 && h->max_group_values > dev::kSpanValues ? 1u : 0u;
     g.partials = reinterpret_cast<float*>(A + bc);
     bc += q.partial_bytes;
-    // a batch of one split-BF16 problem (the server's large requests) runs
-    // the single-graph kernel, which keeps the standalone call's register
-    // budget
-    single = n == 1 && form == kSPLIT;
+    // a batch of one split-BF16 problem whose tile splits B in registers
+    // (the server's large requests) runs the single-graph kernel with the
+    // batch's own workspace
+    single = n == 1 && q.b_inreg;
     if (single) {
       // the single-graph kernel with the batch's own workspace
       g.xcd_slice = q.xcd == 2 ? 2u : (q.xcd == 1 && q.n_kc >= 2 && (8 % q.n_kc) == 0) ? 1u : 0u;
@@ -300,7 +305,8 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
       single_nkc = q.n_kc;
       single_packed = q.sched->packed;
     }
-    if (form != kBF16) {
+    if (single) g.b_rows = h->cols;
+    if (form == kFP32 || single) {
       g.B = p.B;
       g.ldb = p.ldb;
     } else {
@@ -327,6 +333,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
     cls[nclass - 1].tasks += q.tasks;
     // the class gathers by buffer loads when every problem's B allows it
     cls[nclass - 1].buf &= dev::buffer_ok(g.b_rows, (uint64_t)g.ldb * (form == kBF16 ? 2 : 4));
+    if (single) single_g.b_rows = g.b_rows;
     PF[j] = run;
     run += (uint32_t)q.tasks;
     SP[j] = nsplit;
@@ -361,7 +368,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
 accspmm_status BatchLauncher::launch(hipStream_t s) const {
   if (nprob == 0) return ACCSPMM_OK;
   if (single) {
-    ACC_TRY(launch_single(single_g, kSPLIT, rw, single_nb, single_wpb, single_packed,
+    ACC_TRY(launch_single(single_g, kSPLITF, rw, single_nb, single_wpb, single_packed,
                           single_units, single_nkc, s));
     if (total_split) {
       const dim3 blk(256);
@@ -378,9 +385,13 @@ accspmm_status BatchLauncher::launch(hipStream_t s) const {
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, head.as<uint32_t>(),
                      (uint64_t)kMaxBatchClasses, 0u);
   ACC_HIP(hipGetLastError());
-  if (form == kBF16 && conv_blocks) {
-    hipLaunchKernelGGL(k_batch_convert, dim3(conv_blocks), dim3(256), 0, s,
-                       static_cast<const ConvDesc*>(conv), num_conv, conv_prefix);
+  if (form != kFP32 && conv_blocks) {
+    if (form == kSPLIT)
+      hipLaunchKernelGGL(k_batch_convert<kSPLIT>, dim3(conv_blocks), dim3(256), 0, s,
+                         static_cast<const ConvDesc*>(conv), num_conv, conv_prefix);
+    else
+      hipLaunchKernelGGL(k_batch_convert<kBF16>, dim3(conv_blocks), dim3(256), 0, s,
+                         static_cast<const ConvDesc*>(conv), num_conv, conv_prefix);
     ACC_HIP(hipGetLastError());
   }
   if (total_tasks == 0) return ACCSPMM_OK;

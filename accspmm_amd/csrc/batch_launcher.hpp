@@ -12,8 +12,10 @@ struct ProblemPlan {
   uint32_t nb = 1, n_kc = 1, kc_padded = 16;
   const Schedule* sched = nullptr;
   uint64_t tasks = 0;          // units x feature chunks
-  uint64_t bconv_bytes = 0;    // converted B (BF16 form)
-  // the single-graph call's choices: waves per block and XCD mapping
+  uint64_t bconv_bytes = 0;    // converted B (split-BF16 and BF16 forms)
+  // the single-graph call's choices: B split in registers (split form),
+  // waves per block and XCD mapping
+  bool b_inreg = false;
   uint32_t wpb = 4, xcd = 0;
   uint64_t partial_bytes = 0;  // split-window partial sums
   uint64_t workspace_bytes() const;
@@ -55,7 +57,8 @@ struct BatchLauncher {
   uint32_t nprob = 0, num_conv = 0, conv_blocks = 0;
   BatchClass cls[kMaxBatchClasses] = {};
   uint32_t nclass = 0;
-  // a batch of one split-form problem runs the single-graph kernel
+  // a batch of one split-form problem whose B is split in registers runs
+  // the single-graph kernel
   bool single = false;
   dev::SpmmArgs single_g{};
   uint32_t single_nb = 1, single_wpb = 4, single_units = 0, single_nkc = 1;

@@ -52,6 +52,12 @@ bool gather_by_buffer(const dev::SpmmArgs& g, int form) {
   return dev::buffer_ok(g.b_rows, (uint64_t)g.ldb * (form == dev::kBF16 ? 2 : 4));
 }
 
+// The kernel form of a call: split-BF16 with b_split 2 splits the caller's
+// FP32 B in registers (kSPLITF), otherwise it gathers the converted copy.
+int kernel_form(int form, const accspmm_tile& tile) {
+  return form == dev::kSPLIT && tile.b_split == 2 ? (int)dev::kSPLITF : form;
+}
+
 namespace {
 using namespace dev;
 
@@ -90,16 +96,18 @@ __global__ __launch_bounds__(NB == 8 ? 256 : 512) void spmm_kernel(SpmmArgs g) {
   run_unit<FORM, NB, RW, PACKED, BUF>(g, u, kc, g.B, g.ldb, g.C, g.ldc, g.K, g.rows, lv);
 }
 
-// B conversion for the BF16 form: zero-padded to ldx >= K columns, plus
-// one all-zero row N that padded tile columns read: grid-stride over the
-// (N + 1) * ldx / 4 four-feature chunks (convert_chunk).
+// B conversion for the converted forms (split-BF16 by pass, BF16):
+// zero-padded to ldx >= K columns, plus one all-zero row N that padded tile
+// columns read: grid-stride over the (N + 1) * ldx / 4 four-feature chunks
+// (convert_chunk).
+template <int FORM>
 __global__ __launch_bounds__(256) void convert_b(const float* __restrict__ B, uint32_t N,
                                                  uint32_t K, uint32_t ldb, void* __restrict__ out,
                                                  uint32_t ldx, uint32_t vec) {
   const uint64_t total = (uint64_t)(N + 1) * (ldx / 4);
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total;
        c += (uint64_t)gridDim.x * blockDim.x)
-    convert_chunk(B, N, K, ldb, out, ldx, vec, c);
+    convert_chunk<FORM>(B, N, K, ldb, out, ldx, vec, c);
 }
 
 // For window w and panel boundary p in [1, P): the number of the window's
@@ -369,10 +377,12 @@ accspmm_status launch_form(int form, uint32_t rw, uint32_t nb,
   if (rw == 8) {
     if (form == kFP32) return launch_nb<kFP32, 8>(nb, g, blocks, wpb, packed, s);
     if (form == kSPLIT) return launch_nb<kSPLIT, 8>(nb, g, blocks, wpb, packed, s);
+    if (form == kSPLITF) return launch_nb<kSPLITF, 8>(nb, g, blocks, wpb, packed, s);
     return launch_nb<kBF16, 8>(nb, g, blocks, wpb, packed, s);
   }
   if (form == kFP32) return launch_nb<kFP32, 16>(nb, g, blocks, wpb, packed, s);
   if (form == kSPLIT) return launch_nb<kSPLIT, 16>(nb, g, blocks, wpb, packed, s);
+  if (form == kSPLITF) return launch_nb<kSPLITF, 16>(nb, g, blocks, wpb, packed, s);
   return launch_nb<kBF16, 16>(nb, g, blocks, wpb, packed, s);
 }
 
@@ -446,6 +456,18 @@ void default_tile(const accspmm_metcf* h, uint32_t K, int form,
   }
 }
 
+// Split-BF16 B path when the tile leaves it open.  The conversion pass costs
+// one read and one write of B; splitting in registers costs four VALU
+// operations per pair of gathered elements instead, i.e. per tile column.
+// With few gathers per B row (banded matrices: ~14 on the 27-point stencil)
+// the in-register split wins; with many (~380 on the 232,965-node power-law
+// graph) the conversion pass does.  Threshold: 64 gathered tile columns per
+// B row.
+uint32_t default_b_split(const accspmm_metcf* h, uint32_t K) {
+  (void)K;  // both costs scale with K
+  return (uint64_t)h->T * kColWindow < 64ull * h->cols ? 2u : 1u;
+}
+
 // The single-graph SpMM kernel over g's work units: blocks of wpb waves,
 // mapped to XCDs as g.xcd_slice says (multiples of 8 blocks for modes 1, 2).
 accspmm_status launch_single(const SpmmArgs& g, int form, uint32_t rw, uint32_t nb,
@@ -481,8 +503,9 @@ void spmm_workspace_need(const accspmm_metcf* h, const Schedule* sch, uint32_t K
                          uint64_t* bconv_bytes) {
   const uint32_t KC = 16 * tile.feat_per_lane;
   const uint64_t kcp = (uint64_t)((K + KC - 1) / KC) * KC;
+  const int kform = kernel_form(form, tile);
   const uint64_t pb = tile.sweep ? 0 : (uint64_t)sch->num_pieces * h->rw * kcp * 4;
-  const uint64_t cb = form == kBF16 ? ((uint64_t)h->cols + 1) * kcp * 2 : 0;
+  const uint64_t cb = raw_b(kform) ? 0 : ((uint64_t)h->cols + 1) * kcp * (kform == kBF16 ? 2 : 4);
   *partial_bytes = (pb + 255) / 256 * 256;
   *bconv_bytes = (cb + 255) / 256 * 256;
 }
@@ -511,6 +534,7 @@ accspmm_status spmm_run(const accspmm_metcf* h, const Schedule* sch, const WorkU
                         uint32_t ldc, uint32_t K, int form, const accspmm_tile& tile,
                         hipStream_t s, int phases, const SpmmWorkspace& w) {
   const uint32_t nb = tile.feat_per_lane;
+  const int kform = kernel_form(form, tile);
   uint32_t wpb = tile.waves_per_block ? tile.waves_per_block : 4;
   // NB = 8 kernels run one wave per SIMD (blocks of at most 4 waves), which
   // leaves them the whole register file, accumulation registers included
@@ -539,12 +563,12 @@ accspmm_status spmm_run(const accspmm_metcf* h, const Schedule* sch, const WorkU
   g.xcd_slice = tile.xcd_slice == 2 ? 2u
                 : (tile.xcd_slice == 1 && (8 % n_kc) == 0) ? 1u : 0u;
   g.zero_row = h->cols;
-  g.b_rows = raw_b(form) ? h->cols : h->cols + 1;
+  g.b_rows = raw_b(kform) ? h->cols : h->cols + 1;
   g.pad_tile = h->T;
   g.wide = h->rw == 16 && h->max_group_values > dev::kSpanValues ? 1u : 0u;
   g.partials = pbytes ? w.partials : nullptr;
 
-  if (raw_b(form)) {
+  if (raw_b(kform)) {
     g.B = B;
     g.ldb = ldb;
   } else {
@@ -553,17 +577,21 @@ accspmm_status spmm_run(const accspmm_metcf* h, const Schedule* sch, const WorkU
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((chunks + 255) / 256, 8192);
     const uint32_t vec = (ldb % 4 == 0 && ((uintptr_t)B % 16) == 0) ? 1u : 0u;
     if (phases & 1) {
-      hipLaunchKernelGGL(convert_b, dim3(blocks), dim3(256), 0, s, B, h->cols, K, ldb,
-                         w.bconv, ldx, vec);
+      if (kform == kSPLIT)
+        hipLaunchKernelGGL(convert_b<kSPLIT>, dim3(blocks), dim3(256), 0, s, B, h->cols, K, ldb,
+                           w.bconv, ldx, vec);
+      else
+        hipLaunchKernelGGL(convert_b<kBF16>, dim3(blocks), dim3(256), 0, s, B, h->cols, K, ldb,
+                           w.bconv, ldx, vec);
       ACC_HIP(hipGetLastError());
     }  // else converted by an earlier prepare call on this stream
     g.B = w.bconv;
     g.ldb = ldx;
   }
   if (!(phases & 2)) return ACCSPMM_OK;
-  if (tile.sweep) return sweep_launch(h, g, tile, nb, form, sch, s);
+  if (tile.sweep) return sweep_launch(h, g, tile, nb, kform, sch, s);
 
-  ACC_TRY(launch_single(g, form, h->rw, nb, wpb, sch->packed, sch->num_units, n_kc, s));
+  ACC_TRY(launch_single(g, kform, h->rw, nb, wpb, sch->packed, sch->num_units, n_kc, s));
   if (sch->num_split_windows) {
     if (h->rw == 8)
       hipLaunchKernelGGL(k_reduce_split<8>, dim3(sch->num_split_windows), dim3(256), 0, s,
@@ -583,7 +611,7 @@ accspmm_status spmm_check_operands(const float* B, uint32_t ldb, const float* C,
   const uint32_t nb = tile.feat_per_lane;
   if (nb != 1 && nb != 2 && nb != 4 && nb != 8) return ACCSPMM_ERR_UNSUPPORTED;
   if (tile.waves_per_block > 8) return ACCSPMM_ERR_UNSUPPORTED;
-  if (raw_b(form)) {
+  if (raw_b(kernel_form(form, tile))) {
     if ((ldb % nb) != 0 || ((uintptr_t)B % (4u * nb)) != 0) return ACCSPMM_ERR_UNSUPPORTED;
   }
   if ((phases & 2) && ((ldc % nb) != 0 || ((uintptr_t)C % (4u * nb)) != 0))
@@ -632,14 +660,17 @@ accspmm_status spmm_fit_tile(const accspmm_metcf* h, const float* B,
   } else {
     ACC_TRY(accspmm_tile_select(h, K, (accspmm_form)form, &t));
   }
-  // reserved (the split form always splits the FP32 B in registers)
-  t.b_split = 0;
+  if (form != kSPLIT) t.b_split = 0;
+  else if (t.b_split == 0 || t.b_split > 2) t.b_split = default_b_split(h, K);
   if (t.feat_per_lane == 8 && t.waves_per_block > 4) t.waves_per_block = 4;
   while (t.feat_per_lane > 1 &&
-         ((raw_b(form) &&
+         ((form == kFP32 &&
            ((ldb % t.feat_per_lane) || ((uintptr_t)B % (4u * t.feat_per_lane)))) ||
           (C && ((ldc % t.feat_per_lane) || ((uintptr_t)C % (4u * t.feat_per_lane))))))
     t.feat_per_lane /= 2;
+  // the in-register split reads B like the FP32 form; otherwise convert
+  if (t.b_split == 2 && ((ldb % t.feat_per_lane) || ((uintptr_t)B % (4u * t.feat_per_lane))))
+    t.b_split = 1;
   *out = t;
   return ACCSPMM_OK;
 }

@@ -16,13 +16,16 @@ typedef float f2 __attribute__((__vector_size__(2 * sizeof(float))));
 typedef float f4 __attribute__((__vector_size__(4 * sizeof(float))));
 typedef uint32_t u4 __attribute__((__vector_size__(4 * sizeof(uint32_t))));
 
-// Forms.  kFP32 and kSPLIT gather the caller's FP32 B (padded tile columns
-// read row 0, masked when that row is not finite); kBF16 gathers a converted
-// bf16 copy of B with an all-zero row for padded columns.  Both BF16 forms
-// read the FP32 A values and round or split them in registers.
-enum : int { kFP32 = 0, kSPLIT = 1, kBF16 = 2 };
-constexpr bool is_split(int f) { return f == kSPLIT; }
-constexpr bool raw_b(int f) { return f != kBF16; }
+// Forms.  kFP32 and kSPLITF gather the caller's FP32 B (padded tile columns
+// read past the end of a buffer, or row 0 masked when that row is not
+// finite); kSPLIT and kBF16 gather a converted copy of B (split-BF16: bf16
+// hi | lo << 16 per element; BF16: bf16) with an all-zero last row.  kSPLIT
+// and kSPLITF are the split-BF16 form with B split by the conversion pass or
+// in registers, bit for bit the same split.  A's FP32 values are split or
+// rounded in registers.
+enum : int { kFP32 = 0, kSPLIT = 1, kBF16 = 2, kSPLITF = 3 };
+constexpr bool is_split(int f) { return f == kSPLIT || f == kSPLITF; }
+constexpr bool raw_b(int f) { return f == kFP32 || f == kSPLITF; }
 
 struct SpmmArgs {
   const uint32_t* __restrict__ rwo;
@@ -33,15 +36,15 @@ struct SpmmArgs {
   const WorkUnit* __restrict__ units;
   const uint32_t* __restrict__ split_info;  // per split window: (window, first slot, count)
   float* __restrict__ partials;
-  const void* __restrict__ B;  // FP32 and split-BF16: float; BF16: u16
+  const void* __restrict__ B;  // FP32, SPLITF: float; SPLIT: u32 (hi | lo << 16); BF16: u16
   float* __restrict__ C;
   // C row of format row r: row_map[r] when the graph was built with its
   // rows reordered, r when row_map is null
   const uint32_t* __restrict__ row_map;
-  // what padded columns read: BF16 form: row zero_row of the converted B
-  // (its last row, all zero); FP32 and split-BF16: row 0 of the caller's B,
-  // masked after the load when that row holds an infinity or NaN (0 x inf
-  // would otherwise leak into C; reading a separate zero buffer instead was
+  // what padded columns read on the 64-bit gather path: converted B: row
+  // zero_row (its last row, all zero); the caller's FP32 B: row 0, masked
+  // after the load when that row holds an infinity or NaN (0 x inf would
+  // otherwise leak into C; reading a separate zero buffer instead was
   // measured slower on gfx950)
   uint32_t zero_row;
   uint32_t b_rows;    // rows of the gathered B (buffer loads past them read zeros)
@@ -86,9 +89,19 @@ __device__ inline float sub_bf16(uint32_t p, float acc) {
                                          __builtin_bit_cast(bf16x2, c), acc, false);
 }
 
-// Three-term bf16 split of two floats x, y (the bf16x6 scheme of Henry, Tang
-// and Heinecke, "Leveraging the bfloat16 Artificial Intelligence Datatype
-// For Higher-Precision Computations", ARITH 2019): t0 = rne(x),
+// Two-term bf16 split of two floats x, y: hi = rne(x), lo = rne(x - hi),
+// each a bf16 pair (x in the low half).  x - hi is exact, and the remainder
+// x - hi - lo is at most 2^-17 |x| (2^e <= |x|: |x - hi| <= 2^(e-8), and
+// below that lo keeps 8 significant bits).  Four VALU operations per pair.
+__device__ inline void split2(uint32_t x, uint32_t y, uint32_t& hi, uint32_t& lo) {
+  const float fx = __builtin_bit_cast(float, x), fy = __builtin_bit_cast(float, y);
+  hi = pack_bf16(fx, fy);
+  lo = pack_bf16(sub_bf16<true>(hi, fx), sub_bf16<false>(hi, fy));
+}
+
+// Three-term bf16 split of two floats x, y (the three-way split of Henry,
+// Tang and Heinecke, "Leveraging the bfloat16 Artificial Intelligence
+// Datatype For Higher-Precision Computations", ARITH 2019): t0 = rne(x),
 // t1 = rne(x - t0), t2 = x - t0 - t1, each a bf16 pair (x in the low half).
 // x - t0 carries at most 16 significant bits and x - t0 - t1 at most 8, so
 // t2 is exact and x = t0 + t1 + t2 with no error for every finite x whose
@@ -105,11 +118,13 @@ __device__ inline void split3(uint32_t x, uint32_t y, uint32_t& t0, uint32_t& t1
                              __builtin_bit_cast(uint32_t, sx), 0x07060302u);
 }
 
-// B conversion for the BF16 form, one chunk of four consecutive features:
-// chunk c of the (N + 1) x ldx / 4 chunks of the converted B (ldx a multiple
-// of 16, zero past K and in the all-zero row N that padded tile columns
-// read).  A 16-byte load when the segment lies inside K and B is 16-byte
-// aligned with ldb % 4 == 0 (vec); one 8-byte store.
+// B conversion for the converted forms, one chunk of four consecutive
+// features: chunk c of the (N + 1) x ldx / 4 chunks of the converted B (ldx
+// a multiple of 16, zero past K and in the all-zero row N that padded tile
+// columns read).  A 16-byte load when the segment lies inside K and B is
+// 16-byte aligned with ldb % 4 == 0 (vec); one 16-byte (split: hi | lo << 16
+// per feature, split2's rounding) or 8-byte (BF16) store.
+template <int FORM>
 __device__ inline void convert_chunk(const float* __restrict__ B, uint32_t N, uint32_t K,
                                      uint32_t ldb, void* __restrict__ out, uint32_t ldx,
                                      uint32_t vec, uint64_t c) {
@@ -126,10 +141,22 @@ __device__ inline void convert_chunk(const float* __restrict__ B, uint32_t N, ui
       for (int i = 0; i < 4; ++i) x[i] = k + i < K ? src[i] : 0.f;
     }
   }
-  uint2 o;
-  o.x = pack_bf16(x[0], x[1]);
-  o.y = pack_bf16(x[2], x[3]);
-  reinterpret_cast<uint2*>(out)[c] = o;
+  if (FORM == kSPLIT) {
+    u4 o;
+#pragma unroll
+    for (int i = 0; i < 4; i += 2) {
+      uint32_t hi, lo;
+      split2(__builtin_bit_cast(uint32_t, x[i]), __builtin_bit_cast(uint32_t, x[i + 1]), hi, lo);
+      o[i] = __builtin_amdgcn_perm(lo, hi, 0x05040100u);      // x[i]: hi | lo << 16
+      o[i + 1] = __builtin_amdgcn_perm(lo, hi, 0x07060302u);  // x[i+1]
+    }
+    reinterpret_cast<u4*>(out)[c] = o;
+  } else {
+    uint2 o;
+    o.x = pack_bf16(x[0], x[1]);
+    o.y = pack_bf16(x[2], x[3]);
+    reinterpret_cast<uint2*>(out)[c] = o;
+  }
 }
 
 // Elements each lane gathers per B row: NB features, as 32-bit words.
@@ -202,7 +229,7 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const BSrc& src, uint32_t co
       } else {
         r.w[0] = __builtin_amdgcn_raw_buffer_load_b16(src.rsrc, off, 0, 0);
       }
-    } else if constexpr (FULL) {
+    } else if constexpr (FORM == kSPLIT || FULL) {
       if constexpr (NB >= 4) {
 #pragma unroll
         for (int i = 0; i < NB / 4; ++i) {
@@ -242,10 +269,12 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const BSrc& src, uint32_t co
     } else {
       r.w[0] = *p;
     }
-  } else if constexpr (FULL) {
-    // the caller's FP32 B: padded columns read row 0 (masked when consumed
-    // if that row is not finite)
-    const uint32_t row = col != kPad ? col : 0u;
+  } else if constexpr (FORM == kSPLIT || FULL) {
+    // the converted split B: padded columns read its zero row; the caller's
+    // FP32 B: they read row 0 (masked when consumed if that row is not
+    // finite)
+    const uint32_t row = FORM == kSPLIT ? (col != kPad ? col : src.zero_row)
+                                        : (col != kPad ? col : 0u);
     const uint32_t* p = static_cast<const uint32_t*>(B) + (uint64_t)row * ldb + feat;
     if constexpr (NB >= 4) {
 #pragma unroll
@@ -495,52 +524,61 @@ __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
     }
     return;
   }
-  // split-BF16 (bf16x6): A = A0 + A1 + A2 and B = B0 + B1 + B2 exactly
-  // (split3); the six products A_i B_j with i + j <= 2 are accumulated, the
-  // three dropped ones are below 2^-23 of |a||b| together.  The A operand's
-  // rows are tile rows: RW=16 all 16 (six MFMAs a block); RW=8 rows 0-7,
-  // so rows 8-15 carry a second A term: [A0; A1] against B0, B1 and B2
-  // gives A0 B0, A1 B0, A0 B1, A1 B1, A0 B2 (and A1 B2), and [A2; 0]
-  // against B0 the last product (four MFMAs a block); rows r and r+8 are
-  // added when the window is stored.
+  // split-BF16: A = A0 + A1 + A2 exactly (split3) and B = B0 + B1 + r with
+  // |r| <= 2^-17 |b| (split2, by the conversion pass or in registers); the
+  // five products A_i B_j with i + j <= 2 (j <= 1) are accumulated, so each
+  // product a b is off by |a r| + |A2 B1| <= (2^-17 + 2^-25) |a b| before
+  // FP32 accumulation.  The A operand's rows are tile rows: RW=16 all 16
+  // (five MFMAs a block); RW=8 rows 0-7, so rows 8-15 carry a second A term:
+  // [A0; A1] against B0 and B1 gives A0 B0, A1 B0, A0 B1, A1 B1, and
+  // [A2; 0] against B0 the last product (three MFMAs a block); rows r and
+  // r+8 are added when the window is stored.  Smallest products first.
   uint32_t t0[4], t1[4], t2[4];
 #pragma unroll
   for (int d = 0; d < 4; ++d) split3(a[2 * d], a[2 * d + 1], t0[d], t1[d], t2[d]);
-  const u4 a0 = {t0[0], t0[1], t0[2], t0[3]}, a1 = {t1[0], t1[1], t1[2], t1[3]},
-           a2 = {t2[0], t2[1], t2[2], t2[3]};
-  u4 x0 = a0, x1 = a1;
+  u4 x0 = {t0[0], t0[1], t0[2], t0[3]}, x1 = {t1[0], t1[1], t1[2], t1[3]};
+  const u4 x2 = {t2[0], t2[1], t2[2], t2[3]};
   if (RW == 8) {
     const bool upper = m >= 8;
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
-      x0[d] = upper ? a1[d] : a0[d];
-      x1[d] = upper ? 0u : a2[d];
+      const uint32_t a1 = x1[d];
+      x1[d] = upper ? 0u : x2[d];
+      x0[d] = upper ? a1 : x0[d];
     }
   }
   const bf16x8 X0 = __builtin_bit_cast(bf16x8, x0);  // RW=8: [A0; A1]; RW=16: A0
   const bf16x8 X1 = __builtin_bit_cast(bf16x8, x1);  // RW=8: [A2; 0];  RW=16: A1
-  const bf16x8 X2 = __builtin_bit_cast(bf16x8, a2);  // RW=16: A2
+  const bf16x8 X2 = __builtin_bit_cast(bf16x8, x2);  // RW=16: A2
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    uint32_t s0[4], s1[4], s2[4];
+    u4 bh, bl;
 #pragma unroll
-    for (int d = 0; d < 4; ++d) split3(st.b[2 * d].w[b], st.b[2 * d + 1].w[b], s0[d], s1[d], s2[d]);
-    const bf16x8 B0 = __builtin_bit_cast(bf16x8, u4{s0[0], s0[1], s0[2], s0[3]});
-    const bf16x8 B1 = __builtin_bit_cast(bf16x8, u4{s1[0], s1[1], s1[2], s1[3]});
-    const bf16x8 B2 = __builtin_bit_cast(bf16x8, u4{s2[0], s2[1], s2[2], s2[3]});
-    // smallest products first
+    for (int d = 0; d < 4; ++d) {
+      const uint32_t x = st.b[2 * d].w[b], y = st.b[2 * d + 1].w[b];
+      if (FORM == kSPLITF) {
+        uint32_t h, l;
+        split2(x, y, h, l);
+        bh[d] = h;
+        bl[d] = l;
+      } else {
+        // converted words hold hi | lo << 16 per feature
+        bh[d] = __builtin_amdgcn_perm(y, x, 0x05040100u);
+        bl[d] = __builtin_amdgcn_perm(y, x, 0x07060302u);
+      }
+    }
+    const bf16x8 BH = __builtin_bit_cast(bf16x8, bh);
+    const bf16x8 BL = __builtin_bit_cast(bf16x8, bl);
     if (RW == 8) {
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B2, acc[b], 0, 0, 0);
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, B0, acc[b], 0, 0, 0);
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B1, acc[b], 0, 0, 0);
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B0, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, BH, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, BL, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, BH, acc[b], 0, 0, 0);
     } else {
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B2, acc[b], 0, 0, 0);
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, B1, acc[b], 0, 0, 0);
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X2, B0, acc[b], 0, 0, 0);
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B1, acc[b], 0, 0, 0);
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, B0, acc[b], 0, 0, 0);
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, B0, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X2, BH, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, BL, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, BL, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, BH, acc[b], 0, 0, 0);
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, BH, acc[b], 0, 0, 0);
     }
   }
 }

@@ -399,10 +399,12 @@ accspmm_status sweep_launch(const accspmm_metcf* h, const dev::SpmmArgs& g,
   if (h->rw == 8) {
     if (form == kFP32) return launch_sweep_nb<kFP32, 8>(nb, a, cus, wpb, lds, s);
     if (form == kSPLIT) return launch_sweep_nb<kSPLIT, 8>(nb, a, cus, wpb, lds, s);
+    if (form == kSPLITF) return launch_sweep_nb<kSPLITF, 8>(nb, a, cus, wpb, lds, s);
     return launch_sweep_nb<kBF16, 8>(nb, a, cus, wpb, lds, s);
   }
   if (form == kFP32) return launch_sweep_nb<kFP32, 16>(nb, a, cus, wpb, lds, s);
   if (form == kSPLIT) return launch_sweep_nb<kSPLIT, 16>(nb, a, cus, wpb, lds, s);
+  if (form == kSPLITF) return launch_sweep_nb<kSPLITF, 16>(nb, a, cus, wpb, lds, s);
   return launch_sweep_nb<kBF16, 16>(nb, a, cus, wpb, lds, s);
 }
 

@@ -210,6 +210,15 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
         }
     }
   }
+  // stage 3 (split-BF16): both ways of splitting B, around the best shape
+  if (bms < 1e30f && form == ACCSPMM_FORM_SPLIT_BF16) {
+    const accspmm_tile base = bt;
+    for (uint32_t bs : {1u, 2u}) {
+      accspmm_tile c = base;
+      c.b_split = bs;
+      consider(c);
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (bms >= 1e30f) return ACCSPMM_ERR_UNSUPPORTED;

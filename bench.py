@@ -40,6 +40,94 @@ def alg_bytes(M, N, nnz, K, form):
     return 4 * (M + 1) + nnz * (4 + s) + N * K * s + M * K * 4
 
 
+# sources of the SpMM kernels: a traffic profile is valid for the build it
+# was measured on, identified by this digest
+KERNEL_SOURCES = ("accspmm_amd/csrc/spmm_device.hpp", "accspmm_amd/csrc/spmm.hip",
+                  "accspmm_amd/csrc/internal.hpp", "accspmm_amd/csrc/builder.hip")
+
+
+def kernel_source_stamp() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        h.update((REPO / rel).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def sig(x: float, digits: int = 4) -> float:
+    """x rounded to `digits` significant digits (errors like 3e-7 stay visible)."""
+    return float(f"{x:.{digits}g}") if x else 0.0
+
+
+def traffic_file(workload: str, K: int, form: str) -> Path:
+    """Traffic profile of the workload at K in the form, in bench.py's
+    default layout and library tile."""
+    return REPO / "profiles" / f"traffic_{workload}_k{K}_{form}.json"
+
+
+def read_counters(path: Path) -> dict:
+    """Per-kernel mean of each counter per dispatch, from a rocprofv3
+    counter-collection CSV (KiB for FETCH_SIZE / WRITE_SIZE)."""
+    import csv
+    acc_v, disp = {}, {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            name = row["Kernel_Name"].replace("void ", "").split("(")[0].strip()
+            key = (name, row["Counter_Name"])
+            acc_v[key] = acc_v.get(key, 0.0) + float(row["Counter_Value"])
+            disp.setdefault(key, set()).add(row["Dispatch_Id"])
+    return {k: (v / len(disp[k]), len(disp[k])) for k, v in acc_v.items()}
+
+
+def summarize_traffic(d: Path, workload: str, K: int, form: str) -> dict:
+    """The traffic JSON of one SpMM launch from four rocprofv3 --pmc runs
+    under `d` (fetch/write over `bench.py --profile-leg`, cfetch/cwrite over
+    `bench.py --calibrate-copy`): FETCH_SIZE and WRITE_SIZE of the SpMM
+    kernel and the split-window reduction per dispatch, scaled by what the
+    same counters read for a 1 GiB device copy."""
+    def find(tag):
+        hits = sorted(d.rglob(f"{tag}_counter_collection.csv"))
+        if not hits:
+            raise SystemExit(f"no {tag}_counter_collection.csv under {d}")
+        return read_counters(hits[0])
+    fetch, write, cf, cw = find("fetch"), find("write"), find("cfetch"), find("cwrite")
+    copy = "__amd_rocclr_copyBuffer"
+    cfk = max(v[0] for (n, c), v in cf.items() if n.startswith(copy) and c == "FETCH_SIZE")
+    cwk = max(v[0] for (n, c), v in cw.items() if n.startswith(copy) and c == "WRITE_SIZE")
+    gib = float(1 << 30)
+    ff, wf = gib / (cfk * 1024), gib / (cwk * 1024)
+    kern = {}
+    for src, cname in ((fetch, "FETCH_SIZE"), (write, "WRITE_SIZE")):
+        for (n, c), (v, nd) in src.items():
+            if c == cname and (n.startswith("spmm_kernel") or n.startswith("k_reduce_split")):
+                kern.setdefault(n, {})[cname] = round(v, 2)
+                kern[n]["dispatches"] = nd
+    total = sum(k.get("FETCH_SIZE", 0) * 1024 * ff + k.get("WRITE_SIZE", 0) * 1024 * wf
+                for k in kern.values())
+    return {"workload": workload, "K": K, "form": form,
+            "source_stamp": kernel_source_stamp(),
+            "launch": "one SpMM step of the head form at the library tile "
+                      "(bench.py --profile-leg)",
+            "counters_kib_per_dispatch": kern,
+            "calibration": {"kernel": copy + ", 1 GiB device-to-device copy",
+                            "FETCH_SIZE_kib": cfk, "WRITE_SIZE_kib": cwk,
+                            "fetch_factor": round(ff, 4), "write_factor": round(wf, 4)},
+            "hbm_bytes_per_launch": int(total),
+            "formula": "sum over the SpMM kernel and k_reduce_split of FETCH_SIZE KiB * 1024 * "
+                       "fetch_factor + WRITE_SIZE KiB * 1024 * write_factor",
+            "commands": [
+                f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir> -o fetch -- "
+                f"python3 bench.py --workload {workload} --K {K} --forms {form} --profile-leg",
+                f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir> -o write -- "
+                f"python3 bench.py --workload {workload} --K {K} --forms {form} --profile-leg",
+                "rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir> -o cfetch -- "
+                "python3 bench.py --calibrate-copy",
+                "rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir> -o cwrite -- "
+                "python3 bench.py --calibrate-copy",
+                f"python3 bench.py --workload {workload} --K {K} --forms {form} "
+                f"--traffic-from <dir>"]}
+
+
 def time_form(g, B, Cd, form, tile, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -252,13 +340,15 @@ def ingest_bench(workload: str) -> dict:
     return out
 
 
-def batch_bench(ngraphs: int, steps: int, warmup: int):
+def batch_bench(ngraphs: int, steps: int, warmup: int, tune: bool = True):
     """Batching: ngraphs Chung-Lu graphs (500-50,000 nodes log-uniform, mean
     degree U[4,64], exponent U[2.0,2.5], U(0,1] values, graph seed 1000+i),
-    K uniform in {16, 64, 128}, B U[-1,1] (seed 2000+i), split-BF16.  Times
-    one persistent launch over all of them (eager and hipGraph replay)
-    against the same problems as back-to-back standalone calls, and checks
-    that every batched C equals its standalone C bit for bit."""
+    K uniform in {16, 64, 128}, B U[-1,1] (seed 2000+i), split-BF16.  With
+    tune, every problem is autotuned first (the batch then plans with the
+    tuned tiles; the library-tile batch is timed too).  Times one persistent
+    launch over all of them (eager and hipGraph replay) against the same
+    problems as back-to-back standalone calls, and checks that every batched
+    C equals its standalone C bit for bit."""
     import numpy as np
     import torch
 
@@ -273,7 +363,31 @@ def batch_bench(ngraphs: int, steps: int, warmup: int):
                       torch.empty((csr.rows, K), device="cuda")))
         flops += 2.0 * csr.nnz * K
     form = acc.FORM_SPLIT_BF16
-    batch = acc.Batch([(g, B, Cb) for g, B, Cb, _ in probs], form)
+    # library tiles first, then every problem autotuned (the picks go to the
+    # tuning cache, which the batch planner and the standalone calls read)
+    untuned = acc.Batch([(g, B, Cb) for g, B, Cb, _ in probs], form)
+    untuned.run()
+    torch.cuda.synchronize()
+    untuned_ms = None
+    if tune:
+        for _ in range(warmup):
+            untuned.run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            untuned.run()
+        e1.record()
+        torch.cuda.synchronize()
+        untuned_ms = e0.elapsed_time(e1) / steps
+        untuned.close()
+        t0 = time.perf_counter()
+        for g, B, Cb, _ in probs:
+            g.autotune(B, Cb, form)
+        log(f"batch: autotuned {ngraphs} graphs in {time.perf_counter() - t0:.1f} s")
+        batch = acc.Batch([(g, B, Cb) for g, B, Cb, _ in probs], form)
+    else:
+        batch = untuned
 
     def timed(fn):
         for _ in range(warmup):
@@ -304,6 +418,8 @@ def batch_bench(ngraphs: int, steps: int, warmup: int):
            "sequential_ms": round(seq_ms, 4), "batched_ms": round(eager_ms, 4),
            "graph_replay_ms": round(graph_ms, 4),
            "speedup_vs_sequential": round(seq_ms / min(eager_ms, graph_ms), 3),
+           "tiles": "autotuned per graph" if tune else "library default",
+           "batched_ms_library_tiles": round(untuned_ms, 4) if untuned_ms else None,
            "batched_gflops": round(flops / (min(eager_ms, graph_ms) * 1e-3) / 1e9, 1),
            "bitwise_equal_to_standalone": bool(same)}
     log(f"batch: {ngraphs} graphs, {nnz} nnz: sequential {seq_ms:.3f} ms, one launch "
@@ -411,10 +527,30 @@ def main():
     ap.add_argument("--ingest", action="store_true",
                     help="only measure Matrix Market ingest and canonicalisation of the "
                          "workload graph (prints its own JSON line)")
+    ap.add_argument("--profile-leg", action="store_true",
+                    help="one warm-up and one timed step of the head form only "
+                         "(the run rocprofv3 --pmc profiles)")
+    ap.add_argument("--traffic-from", default="",
+                    help="write the traffic JSON under profiles/ from rocprofv3 counter runs "
+                         "in this directory (no GPU work)")
+    ap.add_argument("--k-sweep", default="32,64,128,256,512",
+                    help="K values timed on the workload per form (empty: none)")
+    ap.add_argument("--stencil", type=int, default=1,
+                    help="also time the 100^3 stencil at K (1) or not (0)")
+    ap.add_argument("--batch-tune", type=int, default=1,
+                    help="autotune every graph of the batch leg (1) or use library tiles (0)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+
+    if args.traffic_from:
+        head = [f for f in args.forms.split(",") if f][0]
+        out = summarize_traffic(Path(args.traffic_from), args.workload, args.K, head)
+        dst = traffic_file(args.workload, args.K, head)
+        dst.write_text(json.dumps(out, indent=1) + "\n")
+        log(f"bench: wrote {dst} ({out['hbm_bytes_per_launch'] / 1e9:.2f} GB per launch)")
+        return
 
     if args.calibrate_copy:
         import torch
@@ -477,6 +613,13 @@ def main():
     B = torch.from_numpy(Bh).cuda()
     Cd = torch.empty((csr.rows, K), dtype=torch.float32, device="cuda")
 
+    if args.profile_leg:
+        form = name_to_form[forms[0]]
+        tile = g.tile_select(K, form)
+        time_form(g, B, Cd, form, tile, 1, 1)
+        log(f"bench: profile leg {forms[0]} tile={tile.as_tuple()}")
+        return
+
     per_form = {}
     outputs = {}
     for fname in forms:
@@ -502,6 +645,29 @@ def main():
         if not args.no_check:
             torch.cuda.synchronize()
             outputs[fname] = Cd.cpu().numpy()
+
+    k_sweep = None
+    if args.k_sweep:
+        k_sweep = {}
+        for K2 in [int(x) for x in args.k_sweep.split(",") if x]:
+            B2 = torch.from_numpy(acc.gen_dense(spec["bseed"] + K2, csr.cols, K2)).cuda()
+            C2 = torch.empty((csr.rows, K2), dtype=torch.float32, device="cuda")
+            row = {}
+            for fname in forms:
+                form = name_to_form[fname]
+                tile = g.tile_select(K2, form)
+                r = time_form(g, B2, C2, form, tile, max(5, args.steps // 2), 2)
+                row[fname] = {"ms_per_step": sig(r["ms_per_step"]),
+                              "gflops": round(2.0 * csr.nnz * K2 / (r["ms_per_step"] * 1e-3)
+                                              / 1e9, 1),
+                              "tile": list(tile.as_tuple())}
+            if "split_bf16" in row and "fp32" in row:
+                row["split_over_fp32"] = round(row["split_bf16"]["gflops"] /
+                                               row["fp32"]["gflops"], 3)
+            k_sweep[str(K2)] = row
+            log(f"bench: K={K2}: " + ", ".join(f"{f} {v['gflops']:.0f}" for f, v in row.items()
+                                                 if isinstance(v, dict)) + " GFLOP/s")
+            del B2, C2
 
     diag = format_diagnostics(g, K, name_to_form[forms[0]])
     log(f"bench: format {diag['format_bytes'] / 1e9:.3f} GB, window gathers "
@@ -549,9 +715,39 @@ def main():
     torch.cuda.empty_cache()
     copy_gbs = copy_bandwidth()
     log(f"bench: device copy {copy_gbs:.0f} GB/s (read + write)")
+    stencil = None
+    if args.stencil and args.workload != "stencil27_100":
+        st_spec = WORKLOADS["stencil27_100"]
+        sc = make_graph(st_spec)
+        sg = acc.DeviceGraph(sc, acc.default_build_opts(args.row_window,
+                                                         reorder_rows=args.reorder))
+        SB = torch.from_numpy(acc.gen_dense(st_spec["bseed"] + K, sc.cols, K)).cuda()
+        SC = torch.empty((sc.rows, K), dtype=torch.float32, device="cuda")
+        stencil = {"rows": sc.rows, "nnz": sc.nnz, "K": K, "row_window": sg.info.row_window,
+                   "reorder_rows": sg.info.reordered}
+        for fname in forms:
+            form = name_to_form[fname]
+            tile = sg.tile_select(K, form)
+            r = time_form(sg, SB, SC, form, tile, args.steps, args.warmup)
+            stencil[fname] = {"ms_per_step": sig(r["ms_per_step"]),
+                              "kernel_ms_mean": sig(r["kernel_ms_mean"]),
+                              "gflops": round(2.0 * sc.nnz * K / (r["ms_per_step"] * 1e-3)
+                                              / 1e9, 1),
+                              "hbm_frac_kernel": sig(alg_bytes(sc.rows, sc.cols, sc.nnz, K, form)
+                                                     / (r["kernel_ms_mean"] * 1e-3) / 1e9
+                                                     / HBM_PEAK_GBS),
+                              "tile": list(tile.as_tuple())}
+            log(f"bench: stencil27_100 {fname}: {r['ms_per_step']:.3f} ms/step, "
+                f"{stencil[fname]['gflops']:.0f} GFLOP/s")
+        if "split_bf16" in stencil and "fp32" in stencil:
+            stencil["split_over_fp32"] = round(stencil["split_bf16"]["gflops"] /
+                                               stencil["fp32"]["gflops"], 3)
+        del sg, SB, SC, sc
+        torch.cuda.empty_cache()
     batching = None
     if args.batch_graphs > 0:
-        batching = batch_bench(args.batch_graphs, max(3, args.steps // 2), 2)
+        batching = batch_bench(args.batch_graphs, max(3, args.steps // 2), 2,
+                               tune=bool(args.batch_tune))
         torch.cuda.empty_cache()
     serving = None
     if args.serve_requests > 0:
@@ -563,12 +759,19 @@ def main():
     head_form = name_to_form[head]
     ab = alg_bytes(csr.rows, csr.cols, csr.nnz, K, head_form)
     achieved = ab / (hr["kernel_ms_mean"] * 1e-3) / 1e9
-    traffic = None
-    variant = "" if (inf.row_window == 8 and not inf.reordered) else \
-        f"_rw{inf.row_window}" + ("_reordered" if inf.reordered else "")
-    tfile = REPO / "profiles" / f"traffic_{args.workload}_k{K}_{head}{variant}.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+    # the profiled traffic counts only for the kernels it was measured on:
+    # the profile carries the digest of their sources, and the tile
+    traffic, traffic_note = None, "no profile"
+    tfile = traffic_file(args.workload, K, head)
+    if tfile.exists() and not args.tile and not args.tune and args.row_window == 0 \
+            and args.reorder == 2:
+        prof = json.loads(tfile.read_text())
+        if prof.get("source_stamp") == kernel_source_stamp():
+            traffic = prof.get("hbm_bytes_per_launch")
+            traffic_note = f"{tfile.relative_to(REPO)} (source stamp {prof['source_stamp']})"
+        else:
+            traffic_note = (f"stale: {tfile.relative_to(REPO)} was measured at source stamp "
+                            f"{prof.get('source_stamp')}, this build is {kernel_source_stamp()}")
     out = {
         "metric": "SpMM GFLOP/s (2*nnz*K/t)",
         "value": round(hr["gflops"], 2),
@@ -592,6 +795,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
+                     "traffic_profile": traffic_note,
                      "kernel": "spmm_kernel+k_reduce_split",
                      "stream_copy_gbs": round(copy_gbs, 1),
                      # the profiled HBM/MALL bytes per launch over this run's
@@ -609,8 +813,10 @@ def main():
         "serving": serving,
         "build_ms": round(build_ms, 1),
         "input_order": input_order,
-        "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv)
+        "forms": {k: {kk: (sig(vv) if isinstance(vv, float) else vv)
                       for kk, vv in v.items()} for k, v in per_form.items()},
+        "k_sweep": k_sweep,
+        "stencil27_100": stencil,
     }
     if "fp32" in per_form and head != "fp32":
         out["vs_fp32_form"] = round(hr["gflops"] / per_form["fp32"]["gflops"], 3)

@@ -176,9 +176,11 @@ accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** out,
 
 typedef enum accspmm_form {
   ACCSPMM_FORM_FP32 = 0,       /* FP32 MFMA (v_mfma_f32_16x16x4_f32)           */
-  ACCSPMM_FORM_SPLIT_BF16 = 1, /* A,B split exactly into 3 BF16 terms in
-                                  registers, the 6 products of order >= 2^-16
-                                  (bf16x6), FP32 accumulate: <= 1e-5 max-rel */
+  ACCSPMM_FORM_SPLIT_BF16 = 1, /* A split exactly into 3 BF16 terms, B into 2
+                                  (remainder <= 2^-17 |b|), the 5 products
+                                  A_i B_j (j <= 1), FP32 accumulate: each
+                                  product within 2^-17 + 2^-25, <= 1e-5
+                                  max-rel                                    */
   ACCSPMM_FORM_BF16 = 2        /* A,B rounded to BF16, FP32 accumulate        */
 } accspmm_form;
 
@@ -202,10 +204,13 @@ typedef struct accspmm_tile {
                               stay in LDS and runs their pieces panel by
                               panel (same pieces and result bits as 0; the
                               work-unit mapping fields are ignored)          */
-  uint32_t b_split;        /* reserved, ignored: the split-BF16 form gathers
-                              the caller's FP32 B and splits it in registers
-                              (a converted copy of three bf16 terms would
-                              gather 1.5x the bytes)                      */
+  uint32_t b_split;        /* split-BF16 form, how B is split into bf16 hi and
+                              lo: 1 a conversion pass writes a split copy of B
+                              (4 bytes an element) that the kernel gathers;
+                              2 the kernel gathers the FP32 B and splits it in
+                              registers (no copy, the same C bits); 0 the
+                              library's choice.  Multi-problem batches always
+                              convert.                                     */
 } accspmm_tile;
 
 /* C[rows x K] = A * B[cols x K].  B (ldb >= K) and C (ldc >= K) are

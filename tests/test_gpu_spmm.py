@@ -2,11 +2,12 @@
 
 Tolerances (componentwise max-rel against |A||B|, FP64 oracle):
   FP32-MFMA: 1e-5;  split-BF16: 1e-5;  single-BF16: 1e-2.
-Split-BF16 splits A and B exactly into three bf16 terms and accumulates the
-six products of order >= 2^-16 (bf16x6), so one product is off by at most
-~3e-7 relative: rows with one nonzero (test_split_single_products) expose
-that bound directly, since no other term hides it.  The full-size workloads
-are in test_gpu_large.py.
+Split-BF16 splits A exactly into three bf16 terms and B into two (remainder
+at most 2^-17 |b|) and accumulates the five products A_i B_j, j <= 1, so one
+product is off by at most 2^-17 + 2^-25 relative (7.7e-6) before FP32
+rounding: rows with one nonzero (test_split_single_products) expose that
+bound directly, since no other term hides it.  The full-size workloads are in
+test_gpu_large.py.
 """
 import os
 from pathlib import Path
@@ -218,11 +219,13 @@ def test_invalid_arguments():
 
 
 @pytest.mark.parametrize("rw", [8, 16])
-def test_split_single_products(rw):
+@pytest.mark.parametrize("b_split", [1, 2])
+def test_split_single_products(rw, b_split):
     # rows of one or two nonzeros: C is one or two products, so the split's
-    # representation error is not averaged away.  A two-term split (or a
-    # three-term one missing any product of order 2^-16) is off by up to
-    # ~1.5e-5 on such rows; bf16x6 stays below 1e-6.
+    # representation error is not averaged away.  Two-term splits of both A
+    # and B (or a missing A2 B0 product) are off by up to ~1.5e-5 on such
+    # rows; three-term A times two-term B stays within 2^-17 + 2^-25 plus
+    # the FP32 rounding of a five-term sum (< 8.5e-6).
     torch = _torch()
     rng = np.random.default_rng(70 + rw)
     rows = cols = 40000
@@ -235,13 +238,15 @@ def test_split_single_products(rw):
     csr = acc.CSR(rows, cols, rp, ci, va)
     B = (rng.uniform(1, 2, (cols, 64)) * np.exp2(rng.integers(-20, 20, (cols, 64)))).astype(np.float32)
     B *= np.where(rng.random((cols, 64)) < 0.5, -1, 1).astype(np.float32)
-    for tile in (None, acc.Tile(1, 4, 0, 0, 0), acc.Tile(4, 1, 8, 1, 64, 2)):
+    for tile in (acc.Tile(1, 4, 0, 0, 0, 0, 0, b_split),
+                 acc.Tile(4, 1, 8, 1, 64, 2, 0, b_split), acc.Tile(8, 1, 0, 0, 64, 0, 0, b_split)):
         mx, fr, _ = run_case(csr, 64, acc.FORM_SPLIT_BF16, tile=tile, rw=rw, B=B)
-        assert mx <= 1e-6, (rw, tile and tile.as_tuple(), mx)
+        assert mx <= 8.5e-6, (rw, tile.as_tuple(), mx)
 
 
-@pytest.mark.parametrize("form", [acc.FORM_FP32, acc.FORM_SPLIT_BF16], ids=acc.FORM_NAMES.get)
-def test_row0_nonfinite_masked(form):
+@pytest.mark.parametrize("form,b_split", [(acc.FORM_FP32, 0), (acc.FORM_SPLIT_BF16, 1),
+                                          (acc.FORM_SPLIT_BF16, 2)])
+def test_row0_nonfinite_masked(form, b_split):
     # padded tile columns gather B's row 0 (their A value is zero); an
     # infinity or NaN there must not reach C.  Column 0 holds no nonzero, so
     # C equals the product with row 0 zeroed, bit for bit, on every schedule
@@ -261,8 +266,11 @@ def test_row0_nonfinite_masked(form):
             Bz[0, :] = 0.0
             Bh[0, :] = np.inf
             Bh[0, ::3] = np.nan
-            for tile in (acc.Tile(1, 4, 0, 0, 0), acc.Tile(2, 4, 16, 0, 64), acc.Tile(4, 4, 0, 0, 64, 4),
-                         acc.Tile(8, 2, 24, 0, 64, 2), acc.Tile(4, 4, 0, 0, 64, 4, 1)):
+            for tile in (acc.Tile(1, 4, 0, 0, 0, 0, 0, b_split),
+                         acc.Tile(2, 4, 16, 0, 64, 0, 0, b_split),
+                         acc.Tile(4, 4, 0, 0, 64, 4, 0, b_split),
+                         acc.Tile(8, 2, 24, 0, 64, 2, 0, b_split),
+                         acc.Tile(4, 4, 0, 0, 64, 4, 1, b_split)):
                 ref = g.spmm(torch.from_numpy(Bz).cuda(), None, form, tile).clone()
                 got = g.spmm(torch.from_numpy(Bh).cuda(), None, form, tile).clone()
                 torch.cuda.synchronize()
@@ -295,6 +303,32 @@ def test_two_streams_share_a_handle(form):
     torch.cuda.synchronize()
     for Cl, R in zip(outs, refs):
         assert all(torch.equal(C, R) for C in Cl)
+
+
+@pytest.mark.parametrize("rw", [8, 16])
+def test_split_in_registers_equals_conversion(rw):
+    # b_split 2 (the kernel splits the FP32 B) and 1 (a conversion pass)
+    # round B the same way: C must be bit-identical, K tails included
+    torch = _torch()
+    rng = np.random.default_rng(60 + rw)
+    rows, cols = 1501, 2300
+    rp, ci, va = random_csr(rng, rows, cols, 0.03, empty_rows=0.1, heavy_rows=3)
+    g = acc.DeviceGraph(acc.CSR(rows, cols, rp, ci, va), acc.default_build_opts(rw))
+    for K in (8, 17, 64, 100, 128):
+        Bh = acc.gen_dense(K, cols, K)
+        B = torch.from_numpy(Bh).cuda()
+        for nb, split, panels, sweep in ((1, 0, 0, 0), (2, 16, 0, 0), (4, 0, 4, 0),
+                                         (8, 24, 2, 0), (4, 0, 4, 1)):
+            outs = []
+            for bs in (1, 2):
+                tile = acc.Tile(nb, 4, split, 0, 64, panels, sweep, bs)
+                outs.append(g.spmm(B, None, acc.FORM_SPLIT_BF16, tile).clone())
+            torch.cuda.synchronize()
+            assert torch.equal(outs[0], outs[1]), (K, nb, split, panels, sweep)
+        C64 = O.spmm_f64(rp, ci, va, Bh)
+        ab = O.spmm_abs_f64(rp, ci, va, Bh)
+        mx, fr = O.errors(outs[1].cpu().numpy(), C64, ab)
+        assert mx <= TOL[acc.FORM_SPLIT_BF16], (K, mx)
 
 
 def test_unaligned_b_view_narrows_nb():
@@ -362,15 +396,17 @@ def test_dense_groups(form):
     rp, ci, va = dense_block_csr(rng, 300, 900, 120, 0.9)
     csr = acc.CSR(300, 900, rp, ci, va)
     full = acc.CSR(64, 64, *dense_block_csr(rng, 64, 64, 64, 1.0))
+    bs = (0,) if form != acc.FORM_SPLIT_BF16 else (1, 2)
     for rw in (16, 8):
         g = acc.DeviceGraph(csr, acc.default_build_opts(rw))
         assert g.info.max_group_values == max_group_values(rw, 300, rp, ci, va)
         assert (g.info.max_group_values > 256) == (rw == 16)
         for K in (16, 100):
-            for tile in (acc.Tile(2, 4, 0, 0, 64), acc.Tile(4, 4, 8, 0, 0),
-                         acc.Tile(1, 8, 0, 1, 64, 3), acc.Tile(8, 2, 4, 2, 64)):
-                mx, fr, _ = run_case(csr, K, form, tile=tile, rw=rw, seed=K, graph=g)
-                assert mx <= TOL[form], (rw, K, tile.as_tuple(), mx)
+            for b in bs:
+                for tile in (acc.Tile(2, 4, 0, 0, 64, 0, 0, b), acc.Tile(4, 4, 8, 0, 0, 0, 0, b),
+                             acc.Tile(1, 8, 0, 1, 64, 3, 0, b), acc.Tile(8, 2, 4, 2, 64, 0, 0, b)):
+                    mx, fr, _ = run_case(csr, K, form, tile=tile, rw=rw, seed=K, graph=g)
+                    assert mx <= TOL[form], (rw, K, tile.as_tuple(), mx)
         B = torch.from_numpy(acc.gen_dense(7, 900, 64)).cuda()
         outs = [g.spmm(B, None, form, acc.Tile(2, 4, 8, 0, 64, 2, sw)).clone() for sw in (0, 1)]
         torch.cuda.synchronize()
@@ -421,9 +457,10 @@ def test_prepare_then_prepared_equals_spmm(form):
     rng = np.random.default_rng(93)
     rp, ci, va = random_csr(rng, 800, 1100, 0.02, empty_rows=0.1, heavy_rows=3)
     g = acc.DeviceGraph(acc.CSR(800, 1100, rp, ci, va))
+    bs = (0,) if form != acc.FORM_SPLIT_BF16 else (1, 2)
     for K in (24, 128):
         B = torch.from_numpy(acc.gen_dense(K + 1, 1100, K)).cuda()
-        tiles = [None, acc.Tile(2, 4, 16, 0, 64, 2, 0)]
+        tiles = [None] + [acc.Tile(2, 4, 16, 0, 64, 2, 0, b) for b in bs]
         for tile in tiles:
             ref = g.spmm(B, None, form, tile).clone()
             Cm = torch.full((800, K), float("nan"), device="cuda")
