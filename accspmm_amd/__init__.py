@@ -148,6 +148,7 @@ _SIGS = {
     "accspmm_spmm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Tile),
                                C.c_void_p]),
+    "accspmm_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     "accspmm_spmm_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                        C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Tile),
                                        C.c_void_p]),
@@ -622,6 +623,14 @@ class Batch:
     def run(self, stream=None, graph: bool = False):
         fn = lib().accspmm_batch_run_graph if graph else lib().accspmm_batch_run
         _check(fn(self.handle, _stream(stream)), "batch_run")
+
+    def autotune(self, stream=None) -> float:
+        """Times batch-wide choices (NB cap, short-window packing) and keeps
+        the fastest; returns its ms per run.  Runs overwrite the C buffers."""
+        ms = C.c_float()
+        _check(lib().accspmm_batch_autotune(self.handle, _stream(stream), C.byref(ms)),
+               "batch_autotune")
+        return ms.value
 
     @property
     def num_tasks(self) -> int:

@@ -151,21 +151,27 @@ inline uint64_t align256(uint64_t x) { return (x + 255) / 256 * 256; }
 
 // Per-problem plan: tile, schedule and workspace sizes.
 accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
-                            ProblemPlan* out) {
+                            ProblemPlan* out, const BatchPolicy& policy) {
   if (!p.graph || !p.B || !p.C || p.K == 0) return ACCSPMM_ERR_INVALID_ARG;
   if (p.ldb < p.K || p.ldc < p.K) return ACCSPMM_ERR_SHAPE;
   const accspmm_metcf* h = p.graph;
   accspmm_tile t;
   // the same tile (after alignment narrowing) as the single-graph call
   ACC_TRY(spmm_fit_tile(h, p.B, p.ldb, p.C, p.ldc, p.K, form, nullptr, &t));
-  // the persistent kernel carries NB <= 4 (register budget)
-  while (t.feat_per_lane > 4) t.feat_per_lane /= 2;
-  // it also packs short windows into units of >= 64 tiles whatever the
-  // single-graph call does: its parallelism comes from the many problems,
-  // and each task costs a queue pop (256-graph batch: 2.65 ms packed, 5.33
-  // unpacked).  Packing does not change C: a packed unit sums each of its
-  // windows from zero, group by group, as a unit of that window alone does.
-  if (t.pack_tiles == 0 && h->W && h->T / h->W < 64) t.pack_tiles = 64;
+  // the persistent kernel carries NB <= 4 (register budget), or the
+  // policy's narrower cap; NB does not change C's bits
+  const uint32_t cap = std::min<uint32_t>(4, std::max<uint32_t>(1, policy.nb_cap));
+  while (t.feat_per_lane > cap) t.feat_per_lane /= 2;
+  // it also packs short windows into units of >= 64 tiles (or the policy's
+  // size) whatever the single-graph call does: its parallelism comes from
+  // the many problems, and each task costs a queue pop (256-graph batch:
+  // 2.65 ms packed, 5.33 unpacked).  Packing does not change C: a packed
+  // unit sums each of its windows from zero, group by group, as a unit of
+  // that window alone does.
+  if (h->W && h->T / h->W < 64) {
+    if (policy.pack) t.pack_tiles = policy.pack;
+    else if (t.pack_tiles == 0) t.pack_tiles = 64;
+  }
   const uint32_t KC = 16 * t.feat_per_lane;
   out->nb = t.feat_per_lane;
   out->n_kc = (p.K + KC - 1) / KC;
@@ -432,6 +438,7 @@ struct accspmm_batch {
   // 8-row graphs go to L[0], those on 16-row graphs to L[1]
   BatchLauncher L[2];
   int form = 0;
+  BatchPolicy policy;
   std::vector<accspmm_problem> probs;
   std::vector<ProblemPlan> plans;
   hipStream_t cap_stream = nullptr;
@@ -480,7 +487,7 @@ extern "C" accspmm_status accspmm_batch_create(const accspmm_problem* problems,
   b->plans.resize(count);
   accspmm_status st = ACCSPMM_OK;
   for (uint32_t i = 0; i < count && st == ACCSPMM_OK; ++i)
-    st = problem_plan(problems[i], (int)form, s, &b->plans[i]);
+    st = problem_plan(problems[i], (int)form, s, &b->plans[i], b->policy);
   if (st == ACCSPMM_OK) st = b->prepare_groups(s);
   if (st == ACCSPMM_OK && hipStreamSynchronize(s) != hipSuccess) st = ACCSPMM_ERR_HIP;
   if (st != ACCSPMM_OK) {
@@ -502,7 +509,7 @@ extern "C" accspmm_status accspmm_batch_rebind(accspmm_batch* b,
   for (uint32_t i = 0; i < count; ++i) {
     if (problems[i].graph != b->probs[i].graph || problems[i].K != b->probs[i].K)
       return ACCSPMM_ERR_SHAPE;
-    ACC_TRY(problem_plan(problems[i], b->form, s, &plans[i]));
+    ACC_TRY(problem_plan(problems[i], b->form, s, &plans[i], b->policy));
     // the new operands must keep the planned tile (captured graphs replay it)
     if (plans[i].nb != b->plans[i].nb) return ACCSPMM_ERR_UNSUPPORTED;
   }
@@ -538,6 +545,76 @@ extern "C" accspmm_status accspmm_batch_run_graph(accspmm_batch* b, void* stream
     ACC_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
   ACC_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return ACCSPMM_OK;
+}
+
+namespace {
+// Re-plans every problem of the batch under policy p and uploads the
+// descriptors (no launch); synchronises the stream.
+accspmm_status replan(accspmm_batch* b, const BatchPolicy& p, hipStream_t s) {
+  std::vector<ProblemPlan> plans(b->probs.size());
+  for (size_t i = 0; i < b->probs.size(); ++i)
+    ACC_TRY(problem_plan(b->probs[i], b->form, s, &plans[i], p));
+  b->policy = p;
+  b->plans = plans;
+  ACC_TRY(b->prepare_groups(s));
+  ACC_HIP(hipStreamSynchronize(s));
+  return ACCSPMM_OK;
+}
+}  // namespace
+
+extern "C" accspmm_status accspmm_batch_autotune(accspmm_batch* b, void* stream,
+                                                 float* best_ms) {
+  if (!b) return ACCSPMM_ERR_INVALID_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  // a captured graph may hold buffers that re-planning replaces
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ACC_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return ACCSPMM_ERR_HIP;
+  }
+  BatchPolicy best = b->policy;
+  float bms = 1e30f;
+  accspmm_status st = ACCSPMM_OK;
+  for (uint32_t cap : {4u, 2u, 1u}) {
+    for (uint32_t pack : {0u, 32u, 128u, 512u}) {
+      BatchPolicy p;
+      p.nb_cap = cap;
+      p.pack = pack;
+      st = replan(b, p, s);
+      if (st != ACCSPMM_OK) break;
+      for (int i = 0; i < 2 && st == ACCSPMM_OK; ++i) st = b->launch(s);
+      if (st != ACCSPMM_OK) break;
+      if (hipEventRecord(e0, s) != hipSuccess) { st = ACCSPMM_ERR_HIP; break; }
+      for (int i = 0; i < 5 && st == ACCSPMM_OK; ++i) st = b->launch(s);
+      if (st != ACCSPMM_OK) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        st = ACCSPMM_ERR_HIP;
+        break;
+      }
+      if (ms / 5 < bms) {
+        bms = ms / 5;
+        best = p;
+      }
+    }
+    if (st != ACCSPMM_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ACC_TRY(st);
+  ACC_TRY(replan(b, best, s));
+  if (best_ms) *best_ms = bms;
   return ACCSPMM_OK;
 }
 

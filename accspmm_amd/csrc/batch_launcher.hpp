@@ -21,8 +21,17 @@ struct ProblemPlan {
   uint64_t workspace_bytes() const;
 };
 
+// Batch-wide choices on top of each problem's own tile, neither of which
+// changes C's bits: the widest NB the persistent kernel runs (nb_cap) and
+// the packing of short windows (pack: 0 keeps the tile's packing, or 64
+// tiles when it has none; otherwise units of at least `pack` tiles).
+struct BatchPolicy {
+  uint32_t nb_cap = 4;
+  uint32_t pack = 0;
+};
+
 accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
-                            ProblemPlan* out);
+                            ProblemPlan* out, const BatchPolicy& policy = BatchPolicy());
 
 // Problems of one NB, consecutive in the batch's order: one persistent
 // kernel each.

@@ -417,16 +417,30 @@ void default_tile(const accspmm_metcf* h, uint32_t K, int form,
   const uint64_t kp = ((uint64_t)K + 16 * nb - 1) / (16 * nb) * (16 * nb);
   const uint64_t b_bytes = ((uint64_t)h->cols + 1) * kp * (form == kBF16 ? 2 : 4);
   if (mean >= 64 && h->row_perm.p) {
-    const uint32_t n_kc = (uint32_t)(kp / (16 * nb));
-    const uint32_t panels = b_bytes > (32ull << 20) ? 4 : b_bytes > (8ull << 20) ? 2 : 0;
+    if (K >= 128) {
+      // one 128-feature chunk per wave (one wave per SIMD, all registers):
+      // A and the tile metadata are read once per launch instead of once
+      // per 64-feature chunk (232,965-node graph, K = 128: split-BF16 2.95
+      // against 3.07 ms, FP32 3.54 against 3.64 ms)
+      nb = 8;
+      t->feat_per_lane = nb;
+    }
+    const uint64_t kp8 = ((uint64_t)K + 16 * nb - 1) / (16 * nb) * (16 * nb);
+    const uint64_t bb = ((uint64_t)h->cols + 1) * kp8 * (form == kBF16 ? 2 : 4);
+    const uint32_t n_kc = (uint32_t)(kp8 / (16 * nb));
+    const uint32_t panels = bb > (32ull << 20) ? 4 : bb > (8ull << 20) ? 2 : 0;
     t->xcd_slice = 0;
     t->split_tiles = h->max_window_tiles > mean ? std::max<uint32_t>(64, mean) : 0;
-    if (form == kSPLIT) {
+    if (form == kSPLIT && nb == 8) {
+      // split-BF16, one chunk: 2 column panels (0 / 2 / 4 panels: 3.28 /
+      // 2.95 / 3.12 ms at K = 128)
+      t->col_panels = bb > (32ull << 20) ? 2 : 0;
+    } else if (form == kSPLIT) {
       // split-BF16 (B 4 bytes a feature): 4 panels above 64 MiB of B, 2
-      // above 32 MiB (K = 32 / 64 / 128 on the 232,965-node graph: 0 / 2 /
-      // 4 panels, 1.16 / 1.69 / 3.03 ms, against 1.27 / 1.79 ms with the
-      // thresholds of the other forms)
-      t->col_panels = b_bytes > (64ull << 20) ? 4 : b_bytes > (32ull << 20) ? 2 : 0;
+      // above 32 MiB (K = 32 / 64 on the 232,965-node graph: 0 / 2 panels,
+      // 1.16 / 1.69 ms, against 1.27 / 1.79 ms with the thresholds of the
+      // other forms)
+      t->col_panels = bb > (64ull << 20) ? 4 : bb > (32ull << 20) ? 2 : 0;
       if (n_kc >= 2 && 8 % n_kc == 0) t->xcd_slice = 1;
     } else if (form == kFP32) {
       t->col_panels = 0;

@@ -166,6 +166,24 @@ struct BRow {
   uint32_t w[kWords];
 };
 
+// Features of a lane's B segment.  Lane n of a quarter covers NB features
+// of the chunk at k0: k0 + NB n + i for word i, except NB = 8 with 4-byte
+// elements (FP32 B, split-BF16): words 0-3 are features k0 + 4n + i and
+// words 4-7 features k0 + 64 + 4n + i - 4, so each of the lane's two
+// 16-byte loads covers 256 contiguous bytes across the quarter's 16 lanes
+// (consecutive 32-byte runs per lane would use every cache line of the row
+// half in each load).  The output follows the same map.
+template <int FORM, int NB>
+constexpr bool split_halves() { return NB == 8 && FORM != kBF16; }
+template <int FORM, int NB>
+constexpr uint32_t word_feat(int i) {
+  return split_halves<FORM, NB>() && i >= 4 ? 60u + (uint32_t)i : (uint32_t)i;
+}
+template <int FORM, int NB>
+__device__ inline uint32_t lane_feat(uint32_t k0, uint32_t n) {
+  return k0 + (split_halves<FORM, NB>() ? 4u : (uint32_t)NB) * n;
+}
+
 // Where a lane's B row segments come from.  BUF: buffer loads through rsrc
 // (num_records = b_rows * row_bytes) at 32-bit offsets c * row_bytes +
 // feature bytes (one v_mad_u32_u24): a padded column (kPad) is clamped to
@@ -234,7 +252,8 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const BSrc& src, uint32_t co
 #pragma unroll
         for (int i = 0; i < NB / 4; ++i) {
           const u4 v = __builtin_bit_cast(
-              u4, __builtin_amdgcn_raw_buffer_load_b128(src.rsrc, off + 16u * i, 0, 0));
+              u4, __builtin_amdgcn_raw_buffer_load_b128(src.rsrc, off + 4u * word_feat<FORM, NB>(4 * i),
+                                                        0, 0));
           r.w[4 * i] = v[0]; r.w[4 * i + 1] = v[1];
           r.w[4 * i + 2] = v[2]; r.w[4 * i + 3] = v[3];
         }
@@ -247,9 +266,11 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const BSrc& src, uint32_t co
     } else {
       // feature tail (FP32 B): features past K read past the end (zeros)
 #pragma unroll
-      for (int i = 0; i < NB; ++i)
+      for (int i = 0; i < NB; ++i) {
+        const uint32_t wf = word_feat<FORM, NB>(i);
         r.w[i] = __builtin_amdgcn_raw_buffer_load_b32(
-            src.rsrc, feat + i < K ? off + 4u * i : 0xFFFFFFFFu, 0, 0);
+            src.rsrc, feat + wf < K ? off + 4u * wf : 0xFFFFFFFFu, 0, 0);
+      }
     }
     return;
   }
@@ -279,7 +300,7 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const BSrc& src, uint32_t co
     if constexpr (NB >= 4) {
 #pragma unroll
       for (int i = 0; i < NB / 4; ++i) {
-        u4 v = *reinterpret_cast<const u4*>(p + 4 * i);
+        u4 v = *reinterpret_cast<const u4*>(p + word_feat<FORM, NB>(4 * i));
         r.w[4 * i] = v[0]; r.w[4 * i + 1] = v[1];
         r.w[4 * i + 2] = v[2]; r.w[4 * i + 3] = v[3];
       }
@@ -294,7 +315,7 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const BSrc& src, uint32_t co
     const uint32_t* row = static_cast<const uint32_t*>(B) + (uint64_t)(col != kPad ? col : 0u) * ldb;
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-      const uint32_t f = feat + i;
+      const uint32_t f = feat + word_feat<FORM, NB>(i);
       const uint32_t x = row[min(f, K - 1)];
       r.w[i] = f < K ? x : 0u;
     }
@@ -477,12 +498,18 @@ __device__ inline void compute_stage(Stage<FORM, NB>& st, f4* acc,
       *reinterpret_cast<u4*>(lv + kSpanValues + 4u * lane) = u4{v.x, v.y, v.z, v.w};
     }
   }
+  // the row's values are consecutive from st.base: column j takes the next
+  // one when its bitmap bit is set (then steps on) and zero otherwise.  The
+  // sign-extended bit masks the value and steps the read position; a read
+  // past the row's last value stays inside the buffer (at most the zero
+  // after the group's values) and is masked.
   uint32_t a[8];
+  const char* vp = reinterpret_cast<const char*>(lv + st.base);
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const bool on = (st.byte >> j) & 1u;
-    const uint32_t rank = __popc(st.byte & ((1u << j) - 1u));
-    a[j] = __builtin_bit_cast(uint32_t, lv[on ? st.base + rank : group_values<RW>()]);
+    const int32_t m = -(int32_t)((st.byte >> j) & 1u);
+    a[j] = *reinterpret_cast<const uint32_t*>(vp) & (uint32_t)m;
+    vp += __mul24(m, -4);
   }
   if (raw_b(FORM) && MASK) {
 #pragma unroll
@@ -606,12 +633,24 @@ __device__ inline uint64_t out_row(const uint32_t* __restrict__ rmap, uint32_t r
   return rmap ? rmap[r] : r;
 }
 
+// Stores a lane's NB values of one output row (row points at its feature 0)
+// at features feat + word_feat(i), those below lim.
+template <int FORM, int NB>
+__device__ inline void store_lane(float* row, uint32_t feat, uint32_t lim, const float* v) {
+  if constexpr (split_halves<FORM, NB>()) {
+    store_vec<4>(row + feat, v, feat < lim ? min(4u, lim - feat) : 0u);
+    store_vec<4>(row + feat + 64, v + 4, feat + 64 < lim ? min(4u, lim - feat - 64) : 0u);
+  } else {
+    store_vec<NB>(row + feat, v, feat < lim ? min((uint32_t)NB, lim - feat) : 0u);
+  }
+}
+
 // Writes the accumulated rows of window w to C (split form, RW=8: rows r
 // and r+8 of the MFMA tile hold products of different A terms of tile row
 // r and are added in place; the caller resets the accumulators afterwards).
 template <int FORM, int NB, int RW>
 __device__ inline void store_window(f4* acc, uint32_t w, uint32_t q,
-                                    uint32_t feat, uint32_t nvalid, float* C,
+                                    uint32_t feat, uint32_t K, float* C,
                                     uint32_t ldc, uint32_t rows,
                                     const uint32_t* rmap) {
   if (is_split(FORM) && RW == 8) {
@@ -624,18 +663,18 @@ __device__ inline void store_window(f4* acc, uint32_t w, uint32_t q,
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const uint32_t r = w * RW + 4 * q + i;
-    if (r < rows && nvalid) {
+    if (r < rows) {
       float v[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) v[b] = acc[b][i];
-      store_vec<NB>(C + out_row(rmap, r) * ldc + feat, v, nvalid);
+      store_lane<FORM, NB>(C + out_row(rmap, r) * ldc, feat, K, v);
     }
   }
 }
 
-template <int NB, int RW>
+template <int FORM, int NB, int RW>
 __device__ inline void zero_window(uint32_t w, uint32_t q, uint32_t feat,
-                                   uint32_t nvalid, float* C, uint32_t ldc,
+                                   uint32_t K, float* C, uint32_t ldc,
                                    uint32_t rows, const uint32_t* rmap) {
   if (RW == 8 && q >= 2) return;
   float v[NB];
@@ -644,7 +683,7 @@ __device__ inline void zero_window(uint32_t w, uint32_t q, uint32_t feat,
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const uint32_t r = w * RW + 4 * q + i;
-    if (r < rows && nvalid) store_vec<NB>(C + out_row(rmap, r) * ldc + feat, v, nvalid);
+    if (r < rows) store_lane<FORM, NB>(C + out_row(rmap, r) * ldc, feat, K, v);
   }
 }
 
@@ -656,7 +695,7 @@ struct Accum {
   f4 acc[NB];
   uint32_t cur_w;
   bool direct;  // packed unit: flush to C; split piece: keep for the partial
-  uint32_t q, feat, nvalid, ldc, rows;
+  uint32_t q, feat, K, ldc, rows;
   float* C;
   const uint32_t* rmap;
   __device__ inline void reset() {
@@ -667,7 +706,7 @@ struct Accum {
   // empty windows except as a whole unit, so no zero-fill is needed here)
   __device__ inline void move_to(uint32_t w) {
     if (w == cur_w) return;
-    store_window<FORM, NB, RW>(acc, cur_w, q, feat, nvalid, C, ldc, rows, rmap);
+    store_window<FORM, NB, RW>(acc, cur_w, q, feat, K, C, ldc, rows, rmap);
     reset();
     cur_w = w;
   }
@@ -748,7 +787,8 @@ __device__ inline bool row0_needs_mask(const void* B, uint32_t feat, uint32_t K)
   bool bad = false;
 #pragma unroll
   for (int i = 0; i < NB; ++i)
-    if (feat + i < K) bad |= !__builtin_isfinite(r0[feat + i]);
+    if (feat + word_feat<FORM, NB>(i) < K)
+      bad |= !__builtin_isfinite(r0[feat + word_feat<FORM, NB>(i)]);
   return __builtin_amdgcn_ballot_w64(bad) != 0;
 }
 
@@ -764,7 +804,7 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t q = lane >> 4, n = lane & 15u;
   const uint32_t k0 = kc * KC;
-  const uint32_t feat = k0 + NB * n;
+  const uint32_t feat = lane_feat<FORM, NB>(k0, n);
   // B loads are full-width unless this chunk straddles K (FP32 B; the
   // converted BF16-form buffer is zero-padded to a multiple of KC)
   const bool full = !raw_b(FORM) || k0 + KC <= K;
@@ -772,7 +812,6 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
   // only in the split form, as a second A term)
   const uint32_t m = n;
   const bool a_rows = RW == 16 || m < 8 || is_split(FORM);
-  const uint32_t nvalid = feat < K ? min((uint32_t)NB, K - feat) : 0u;
 
   Accum<FORM, NB, RW> out;
   out.reset();
@@ -780,7 +819,7 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
   out.direct = u.slot < 0;
   out.q = q;
   out.feat = feat;
-  out.nvalid = nvalid;
+  out.K = K;
   out.C = C;
   out.ldc = ldc;
   out.rows = rows;
@@ -823,9 +862,9 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
 
   if (u.slot < 0) {
     // last window of the unit; a unit of empty windows writes zero rows
-    store_window<FORM, NB, RW>(out.acc, out.cur_w, q, feat, nvalid, C, ldc, rows, g.row_map);
+    store_window<FORM, NB, RW>(out.acc, out.cur_w, q, feat, K, C, ldc, rows, g.row_map);
     for (uint32_t e = out.cur_w + 1; e < u.window + u.nwin; ++e)
-      zero_window<NB, RW>(e, q, feat, nvalid, C, ldc, rows, g.row_map);
+      zero_window<FORM, NB, RW>(e, q, feat, K, C, ldc, rows, g.row_map);
     return;
   }
   // split piece: write partial (RW rows x kc_padded floats per slot)
@@ -836,15 +875,15 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[b][i] += __shfl_xor(acc[b][i], 32, 64);
   }
-  const bool store_lane = RW == 16 || q < 2;
+  const bool holds_rows = RW == 16 || q < 2;
   float* part = g.partials + (uint64_t)u.slot * RW * g.kc_padded;
-  if (store_lane) {
+  if (holds_rows) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       float v[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) v[b] = acc[b][i];
-      store_vec<NB>(part + (uint64_t)(4 * q + i) * g.kc_padded + NB * n + k0, v, NB);
+      store_lane<FORM, NB>(part + (uint64_t)(4 * q + i) * g.kc_padded, feat, g.kc_padded, v);
     }
   }
   // the partials are combined by reduce_split_window in a second launch

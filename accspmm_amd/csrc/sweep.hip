@@ -69,13 +69,13 @@ __device__ inline void finish_piece(f4* acc, f4* slot, uint32_t lane, bool first
 }
 
 // Writes a window's rows from its LDS slot to C.
-template <int NB, int RW>
+template <int FORM, int NB, int RW>
 __device__ inline void store_slot(const f4* slot, uint32_t lane, uint32_t q,
-                                  uint32_t w, uint32_t feat, uint32_t nvalid,
+                                  uint32_t w, uint32_t feat, uint32_t K,
                                   float* C, uint32_t ldc, uint32_t rows,
                                   const uint32_t* rmap) {
   constexpr uint32_t NL = slot_lanes<RW>();
-  if (lane >= NL || !nvalid) return;
+  if (lane >= NL) return;
   f4 acc[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) acc[b] = slot[b * NL + lane];
@@ -86,7 +86,7 @@ __device__ inline void store_slot(const f4* slot, uint32_t lane, uint32_t q,
       float v[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) v[b] = acc[b][i];
-      store_vec<NB>(C + out_row(rmap, r) * ldc + feat, v, nvalid);
+      store_lane<FORM, NB>(C + out_row(rmap, r) * ldc, feat, K, v);
     }
   }
 }
@@ -202,10 +202,9 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
     const uint32_t ng = __builtin_amdgcn_readfirstlane(a.ngroups[idx]);
     const uint32_t* wl = a.wlist + (size_t)idx * a.M;
     for (uint32_t kc = 0; kc < g.n_kc; ++kc) {
-      const uint32_t k0 = kc * KC, feat = k0 + NB * n;
+      const uint32_t k0 = kc * KC, feat = lane_feat<FORM, NB>(k0, n);
       // B loads are full-width unless the chunk straddles K (FP32 B)
       const bool full = !raw_b(FORM) || k0 + KC <= g.K;
-      const uint32_t nvalid = feat < g.K ? min((uint32_t)NB, g.K - feat) : 0u;
       uint32_t started = 0;
       if (ng) {
         f4 acc[NB];
@@ -231,10 +230,10 @@ __global__ __launch_bounds__(512) void sweep_kernel(SweepArgs a) {
         const uint32_t w = __builtin_amdgcn_readfirstlane(wl[s]);
         if (w == kPad) continue;
         if ((started >> s) & 1u)
-          store_slot<NB, RW>(slots + (size_t)s * NB * NL, lane, q, w, feat, nvalid, g.C,
+          store_slot<FORM, NB, RW>(slots + (size_t)s * NB * NL, lane, q, w, feat, g.K, g.C,
                              g.ldc, g.rows, g.row_map);
         else
-          zero_window<NB, RW>(w, q, feat, nvalid, g.C, g.ldc, g.rows, g.row_map);
+          zero_window<FORM, NB, RW>(w, q, feat, g.K, g.C, g.ldc, g.rows, g.row_map);
       }
     }
   }

@@ -325,6 +325,14 @@ accspmm_status accspmm_batch_rebind(accspmm_batch* b,
 /* One persistent launch over the whole queue (plus the B conversion pass
  * for the BF16 forms).  Asynchronous on `stream`. */
 accspmm_status accspmm_batch_run(accspmm_batch* b, void* stream);
+/* Batch autotuner: times the batch (each run overwrites the problems' C)
+ * under batch-wide choices on top of each problem's tile — the widest
+ * feature chunk the persistent kernel runs (NB 4, 2, 1) and the packing of
+ * short windows into work units (the tile's own, or at least 32, 128, 512
+ * tiles) — and keeps the fastest; neither changes any C bit.  *best_ms
+ * receives its time per run.  A captured graph of the batch is dropped (the
+ * next accspmm_batch_run_graph captures again).  No reference counterpart. */
+accspmm_status accspmm_batch_autotune(accspmm_batch* b, void* stream, float* best_ms);
 /* Captures accspmm_batch_run into a hipGraph once and replays it. */
 accspmm_status accspmm_batch_run_graph(accspmm_batch* b, void* stream);
 accspmm_status accspmm_batch_destroy(accspmm_batch* b);

@@ -173,3 +173,29 @@ def test_batch_dense_groups(form):
             for Cm, A in zip(Cs[sel], alone[sel]):
                 assert torch.equal(Cm, A), (sel, graph)
         batch.close()
+
+
+@pytest.mark.parametrize("form", [acc.FORM_FP32, acc.FORM_SPLIT_BF16, acc.FORM_BF16],
+                         ids=acc.FORM_NAMES.get)
+def test_batch_autotune_keeps_bits(form):
+    # the batch autotuner's choices (NB cap, short-window packing) leave
+    # every C bit-equal to its standalone call, eager and under a graph
+    # captured after the tuning
+    import torch
+    pool = _pool(10, 30 + form)
+    graphs = [acc.DeviceGraph(c) for c, _, _ in pool]
+    Bs = [torch.from_numpy(acc.gen_dense(s, c.cols, K)).cuda() for c, K, s in pool]
+    Cs = [torch.full((c.rows, K), float("nan"), device="cuda") for c, K, _ in pool]
+    alone = [g.spmm(B, None, form).clone() for g, B in zip(graphs, Bs)]
+    batch = acc.Batch(list(zip(graphs, Bs, Cs)), form)
+    batch.run(graph=True)
+    ms = batch.autotune()
+    assert ms > 0
+    for graph in (False, True, True):
+        for c in Cs:
+            c.fill_(float("nan"))
+        batch.run(graph=graph)
+        torch.cuda.synchronize()
+        for Cm, A in zip(Cs, alone):
+            assert torch.equal(Cm, A), graph
+    batch.close()
