@@ -690,26 +690,33 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
   // 27-point stencil 100^3, split-BF16 at K=128: 0.591 ms against 0.674 ms;
   // FP32 0.712 against 0.822 ms.  Small graphs keep 8-row windows (many
   // small dense ones share columns too, unmeasured there).
+  // The 8-row build validated the CSR, which the tile count needs.  The
+  // count is optional: if it fails the 8-row graph stands.  The 8-row
+  // arrays are released before the 16-row build (peak memory: one graph
+  // and the count's scratch, never two graphs); should that build fail,
+  // the 8-row graph is built again.
   if (st == ACCSPMM_OK && auto_window && o.row_window == 8 && !o.reorder_rows &&
       (uint64_t)h->T >= 64ull * 8192) {
     uint64_t t16 = 0;
-    st = count_tiles(csr, 16, (hipStream_t)stream, &t16);
-    if (st == ACCSPMM_OK && t16 * 100 <= (uint64_t)h->T * 92) {
-      auto* h16 = new (std::nothrow) accspmm_metcf();
-      if (!h16) {
-        st = ACCSPMM_ERR_OUT_OF_MEMORY;
-      } else {
-        accspmm_build_opts o16 = o;
-        o16.row_window = 16;
-        st = build_impl(csr, o16, h16, (hipStream_t)stream);
-        if (st == ACCSPMM_OK) {
-          delete h;
-          h = h16;
-        } else {
-          (void)hipStreamSynchronize((hipStream_t)stream);
-          delete h16;
-        }
+    const uint64_t t8 = h->T;
+    if (count_tiles(csr, 16, (hipStream_t)stream, &t16) == ACCSPMM_OK &&
+        t16 * 100 <= t8 * 92) {
+      (void)hipStreamSynchronize((hipStream_t)stream);
+      delete h;
+      h = new (std::nothrow) accspmm_metcf();
+      if (!h) return ACCSPMM_ERR_OUT_OF_MEMORY;
+      accspmm_build_opts o16 = o;
+      o16.row_window = 16;
+      st = build_impl(csr, o16, h, (hipStream_t)stream);
+      if (st != ACCSPMM_OK) {
+        (void)hipStreamSynchronize((hipStream_t)stream);
+        delete h;
+        h = new (std::nothrow) accspmm_metcf();
+        if (!h) return ACCSPMM_ERR_OUT_OF_MEMORY;
+        st = build_impl(csr, o, h, (hipStream_t)stream);
       }
+    } else {
+      (void)hipGetLastError();
     }
   }
   if (st != ACCSPMM_OK) {
