@@ -65,6 +65,16 @@ def traffic_file(workload: str, K: int, form: str) -> Path:
     return REPO / "profiles" / f"traffic_{workload}_k{K}_{form}.json"
 
 
+def kernel_short_name(full: str) -> str:
+    """`spmm_kernel<1, 8, 16, false, true>` from the demangled name rocprofv3
+    writes (return type, namespaces and parameter list dropped)."""
+    s = full.replace("void ", "", 1) if full.startswith("void ") else full
+    s = s.replace("(anonymous namespace)::", "")
+    s = s.split("(")[0].strip()
+    head, sep, tail = s.partition("<")
+    return head.split("::")[-1] + sep + tail
+
+
 def read_counters(path: Path) -> dict:
     """Per-kernel mean of each counter per dispatch, from a rocprofv3
     counter-collection CSV (KiB for FETCH_SIZE / WRITE_SIZE)."""
@@ -72,7 +82,7 @@ def read_counters(path: Path) -> dict:
     acc_v, disp = {}, {}
     with open(path, newline="") as f:
         for row in csv.DictReader(f):
-            name = row["Kernel_Name"].replace("void ", "").split("(")[0].strip()
+            name = kernel_short_name(row["Kernel_Name"])
             key = (name, row["Counter_Name"])
             acc_v[key] = acc_v.get(key, 0.0) + float(row["Counter_Value"])
             disp.setdefault(key, set()).add(row["Dispatch_Id"])
@@ -99,11 +109,16 @@ def summarize_traffic(d: Path, workload: str, K: int, form: str) -> dict:
     kern = {}
     for src, cname in ((fetch, "FETCH_SIZE"), (write, "WRITE_SIZE")):
         for (n, c), (v, nd) in src.items():
-            if c == cname and (n.startswith("spmm_kernel") or n.startswith("k_reduce_split")):
+            if c == cname and n.startswith(("spmm_kernel", "k_reduce_split", "convert_b")):
                 kern.setdefault(n, {})[cname] = round(v, 2)
                 kern[n]["dispatches"] = nd
-    total = sum(k.get("FETCH_SIZE", 0) * 1024 * ff + k.get("WRITE_SIZE", 0) * 1024 * wf
-                for k in kern.values())
+    if not any(n.startswith("spmm_kernel") for n in kern):
+        raise SystemExit(f"no spmm_kernel dispatch in the counter runs under {d}")
+
+    def hbm(names):
+        return int(sum(k.get("FETCH_SIZE", 0) * 1024 * ff + k.get("WRITE_SIZE", 0) * 1024 * wf
+                       for n, k in kern.items() if n.startswith(names)))
+    total = hbm(("spmm_kernel", "k_reduce_split"))
     return {"workload": workload, "K": K, "form": form,
             "source_stamp": kernel_source_stamp(),
             "launch": "one SpMM step of the head form at the library tile "
@@ -112,9 +127,11 @@ def summarize_traffic(d: Path, workload: str, K: int, form: str) -> dict:
             "calibration": {"kernel": copy + ", 1 GiB device-to-device copy",
                             "FETCH_SIZE_kib": cfk, "WRITE_SIZE_kib": cwk,
                             "fetch_factor": round(ff, 4), "write_factor": round(wf, 4)},
-            "hbm_bytes_per_launch": int(total),
-            "formula": "sum over the SpMM kernel and k_reduce_split of FETCH_SIZE KiB * 1024 * "
-                       "fetch_factor + WRITE_SIZE KiB * 1024 * write_factor",
+            "hbm_bytes_per_launch": total,
+            "hbm_bytes_per_step": hbm(("spmm_kernel", "k_reduce_split", "convert_b")),
+            "formula": "per launch: sum over the SpMM kernel and k_reduce_split of FETCH_SIZE "
+                       "KiB * 1024 * fetch_factor + WRITE_SIZE KiB * 1024 * write_factor; per "
+                       "step: the same with the B conversion pass (convert_b) added",
             "commands": [
                 f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir> -o fetch -- "
                 f"python3 bench.py --workload {workload} --K {K} --forms {form} --profile-leg",
