@@ -29,7 +29,7 @@ __all__ = [
     "lib", "Status", "AccSpMMError", "FORM_FP32", "FORM_SPLIT_BF16", "FORM_BF16",
     "BuildOpts", "Tile", "CSR", "read_mtx", "canonicalize_device", "autotune_layout", "METCFBit", "AdpBME",
     "balanceStrategy", "DeviceGraph", "Batch", "Server", "gen_chung_lu", "gen_stencil27",
-    "gen_dense", "MetcfInfo",
+    "gen_dense", "MetcfInfo", "order_rcm",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -74,7 +74,7 @@ class BuildOpts(C.Structure):
                 ("t_load", C.c_uint32), ("t_comp", C.c_uint32),
                 ("t_write", C.c_uint32), ("target_num", C.c_uint32),
                 ("imbalance_threshold", C.c_uint32), ("build_adpbme", C.c_uint32),
-                ("reorder_rows", C.c_uint32)]
+                ("reorder_rows", C.c_uint32), ("symmetric_order", C.c_uint32)]
 
 
 class MetcfInfo(C.Structure):
@@ -83,7 +83,8 @@ class MetcfInfo(C.Structure):
                 ("num_tiles", C.c_uint32), ("num_groups", C.c_uint32),
                 ("ibd", C.c_uint32), ("balanced", C.c_uint32),
                 ("max_window_tiles", C.c_uint32), ("device_bytes", C.c_uint64),
-                ("reordered", C.c_uint32), ("max_group_values", C.c_uint32)]
+                ("reordered", C.c_uint32), ("max_group_values", C.c_uint32),
+                ("symmetric_order", C.c_uint32)]
 
 
 class Tile(C.Structure):
@@ -139,6 +140,8 @@ _SIGS = {
     "accspmm_metcf_get_info": (C.c_int, [C.c_void_p, C.POINTER(MetcfInfo)]),
     "accspmm_metcf_export": (C.c_int, [C.c_void_p] + [C.c_void_p] * 7),
     "accspmm_metcf_row_order": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "accspmm_metcf_col_order": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "accspmm_order_rcm": (C.c_int, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "accspmm_autotune_layout": (C.c_int, [C.POINTER(CsrView), C.c_void_p, C.c_uint32,
                                           C.c_void_p, C.c_uint32, C.c_uint32, C.c_int,
                                           C.POINTER(BuildOpts), C.POINTER(C.c_float),
@@ -375,6 +378,20 @@ def default_build_opts(row_window: int = 8, **kw) -> BuildOpts:
     return o
 
 
+def order_rcm(row_ptr, col_idx, n: int | None = None) -> np.ndarray:
+    """Reverse Cuthill-McKee order of an n x n pattern (host arrays):
+    perm[i] = the node placed at position i (accspmm_order_rcm)."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    ci = np.ascontiguousarray(col_idx, dtype=np.uint32)
+    n = len(rp) - 1 if n is None else n
+    if len(rp) != n + 1:
+        raise ValueError("row_ptr must have n + 1 entries")
+    perm = np.empty(n, np.uint32)
+    _check(lib().accspmm_order_rcm(n, _ptr(rp), _ptr(ci) if len(ci) else 0, _ptr(perm)),
+           "order_rcm")
+    return perm
+
+
 def autotune_layout(csr: CSR, B, C_out, form: int = FORM_FP32, stream=None):
     """Fastest of the four layouts (row window 8/16 x input/reordered rows)
     for (csr, B, form): returns (BuildOpts, ms) (accspmm_autotune_layout)."""
@@ -484,9 +501,17 @@ class DeviceGraph:
         return out
 
     def row_order(self) -> np.ndarray:
-        """Input row of each format row (identity unless reorder_rows)."""
+        """Input row of each format row (identity unless reorder_rows or
+        symmetric_order)."""
         out = np.empty(self.info.rows, np.uint32)
         _check(lib().accspmm_metcf_row_order(self.handle, _ptr(out)), "metcf_row_order")
+        return out
+
+    def col_order(self) -> np.ndarray:
+        """Input column (B row) of each format column (identity unless
+        symmetric_order)."""
+        out = np.empty(self.info.cols, np.uint32)
+        _check(lib().accspmm_metcf_col_order(self.handle, _ptr(out)), "metcf_col_order")
         return out
 
     def tile_select(self, K: int, form: int = FORM_FP32) -> Tile:

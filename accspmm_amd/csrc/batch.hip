@@ -41,6 +41,7 @@ using namespace dev;
 struct ConvDesc {
   const float* src;
   void* dst;
+  const uint32_t* perm;  // the graph's column order (null: none)
   uint32_t ldb, K, rows, ldx;
   uint32_t vec;  // 16-byte loads allowed (ldb % 4 == 0, src 16-byte aligned)
 };
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(256) void k_batch_convert(
   const uint64_t c0 = (uint64_t)(blockIdx.x - prefix[p]) * kConvChunks;
   const uint64_t c1 = min<uint64_t>(c0 + kConvChunks, total);
   for (uint64_t ch = c0 + threadIdx.x; ch < c1; ch += blockDim.x)
-    convert_chunk<FORM>(c.src, c.rows, c.K, c.ldb, c.dst, c.ldx, c.vec, ch);
+    convert_chunk<FORM>(c.src, c.rows, c.K, c.ldb, c.dst, c.ldx, c.vec, ch, c.perm);
 }
 
 // The persistent kernel of one (form, row window, NB) class: problems
@@ -181,14 +182,18 @@ accspmm_status problem_plan(const accspmm_problem& p, int form, hipStream_t s,
     ACC_TRY(get_schedule(h, t, s, &out->sched));
   }
   out->tasks = (uint64_t)out->sched->num_units * out->n_kc;
-  out->b_inreg = form == kSPLIT && t.b_split == 2;
+  // a graph with a column order gathers a permuted copy of B in every form
+  // (the split from the conversion pass is bit for bit the in-register one)
+  out->permuted = h->col_perm.p != nullptr;
+  out->b_inreg = form == kSPLIT && t.b_split == 2 && !out->permuted;
   out->wpb = t.waves_per_block ? t.waves_per_block : 4;
   out->xcd = t.xcd_slice;
   // the persistent kernel gathers a converted B in the split form (a batch
   // of one problem whose tile splits B in registers leaves it unused)
   out->bconv_bytes =
-      form == kFP32 ? 0
-                    : align256(((uint64_t)h->cols + 1) * out->kc_padded * (form == kSPLIT ? 4 : 2));
+      form == kFP32 && !out->permuted
+          ? 0
+          : align256(((uint64_t)h->cols + 1) * out->kc_padded * (form == kBF16 ? 2 : 4));
   out->partial_bytes =
       align256((uint64_t)out->sched->num_pieces * h->rw * out->kc_padded * 4);
   return ACCSPMM_OK;
@@ -312,7 +317,7 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
       single_packed = q.sched->packed;
     }
     if (single) g.b_rows = h->cols;
-    if (form == kFP32 || single) {
+    if ((form == kFP32 && !q.permuted) || single) {
       g.B = p.B;
       g.ldb = p.ldb;
     } else {
@@ -321,8 +326,9 @@ accspmm_status BatchLauncher::prepare(const accspmm_problem* P, uint32_t n,
       g.B = dst;
       g.ldb = q.kc_padded;
       const uint32_t vec = (p.ldb % 4 == 0 && ((uintptr_t)p.B % 16) == 0) ? 1u : 0u;
-      CV[nconv] = ConvDesc{static_cast<const float*>(p.B), dst, p.ldb, p.K, h->cols,
-                           q.kc_padded, vec};
+      CV[nconv] = ConvDesc{static_cast<const float*>(p.B), dst,
+                           h->col_perm.p ? h->col_perm.as<uint32_t>() : nullptr, p.ldb, p.K,
+                           h->cols, q.kc_padded, vec};
       CP[nconv] = cblocks;
       const uint64_t chunks = ((uint64_t)h->cols + 1) * (q.kc_padded / 4);
       const uint64_t nblk = (chunks + kConvChunks - 1) / kConvChunks;
@@ -391,8 +397,11 @@ accspmm_status BatchLauncher::launch(hipStream_t s) const {
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, head.as<uint32_t>(),
                      (uint64_t)kMaxBatchClasses, 0u);
   ACC_HIP(hipGetLastError());
-  if (form != kFP32 && conv_blocks) {
-    if (form == kSPLIT)
+  if (conv_blocks) {
+    if (form == kFP32)
+      hipLaunchKernelGGL(k_batch_convert<kFP32>, dim3(conv_blocks), dim3(256), 0, s,
+                         static_cast<const ConvDesc*>(conv), num_conv, conv_prefix);
+    else if (form == kSPLIT)
       hipLaunchKernelGGL(k_batch_convert<kSPLIT>, dim3(conv_blocks), dim3(256), 0, s,
                          static_cast<const ConvDesc*>(conv), num_conv, conv_prefix);
     else

@@ -16,6 +16,8 @@ struct ProblemPlan {
   // the single-graph call's choices: B split in registers (split form),
   // waves per block and XCD mapping
   bool b_inreg = false;
+  bool permuted = false;       // the graph has a column order: B is permuted
+                               // by the conversion pass in every form
   uint32_t wpb = 4, xcd = 0;
   uint64_t partial_bytes = 0;  // split-window partial sums
   uint64_t workspace_bytes() const;

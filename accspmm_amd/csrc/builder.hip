@@ -372,6 +372,7 @@ extern "C" void accspmm_build_opts_default(accspmm_build_opts* o) {
   // only for callers that export them (the AdpBME drop-in class)
   o->build_adpbme = 0;
   o->reorder_rows = 0;
+  o->symmetric_order = 0;
 }
 
 namespace {
@@ -654,7 +655,8 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
     return ACCSPMM_ERR_UNSUPPORTED;
   if (o.col_window != 8) return ACCSPMM_ERR_UNSUPPORTED;
   if (o.build_adpbme && o.t_load + o.t_comp == 0) return ACCSPMM_ERR_UNSUPPORTED;
-  if (o.reorder_rows > 2) return ACCSPMM_ERR_UNSUPPORTED;
+  if (o.reorder_rows > 2 || o.symmetric_order > 2) return ACCSPMM_ERR_UNSUPPORTED;
+  if (o.symmetric_order && csr->rows != csr->cols) return ACCSPMM_ERR_UNSUPPORTED;
   if (csr->rows == 0 || csr->cols == 0) return ACCSPMM_ERR_INVALID_ARG;
   if (csr->nnz > kMaxNnz) return ACCSPMM_ERR_UNSUPPORTED;
   // automatic choices (row_window 0, reorder_rows 2): 16-row windows and
@@ -666,18 +668,32 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
   const bool auto_window = o.row_window == 0;
   if (o.row_window == 0) o.row_window = long_rows ? 16 : 8;
   if (o.reorder_rows == 2) o.reorder_rows = long_rows ? 1 : 0;
+  // a symmetric ordering fixes the rows' order itself
+  if (o.symmetric_order) o.reorder_rows = 0;
   if (!csr->row_ptr || (csr->nnz && (!csr->col_idx || !csr->val)))
     return ACCSPMM_ERR_INVALID_ARG;
   auto* h = new (std::nothrow) accspmm_metcf();
   if (!h) return ACCSPMM_ERR_OUT_OF_MEMORY;
-  // reorder_rows: build from the rows in order of decreasing length; the
-  // handle keeps the permutation and the kernels map C rows through it
-  DevBuf prp, pci, pva;
+  // reorder_rows: build from the rows in order of decreasing length;
+  // symmetric_order: build from P A P^T.  The handle keeps the permutation
+  // (rows; columns for P A P^T): the kernels write C rows through it and
+  // each call permutes B through it.
+  DevBuf prp, pci, pva, rperm, cperm;
   accspmm_csr_view pv;
   const accspmm_csr_view* src = csr;
   accspmm_status st = ACCSPMM_OK;
-  if (o.reorder_rows) {
-    st = reorder_rows_by_length(csr, prp, pci, pva, h->row_perm, (hipStream_t)stream);
+  if (o.reorder_rows || o.symmetric_order) {
+    st = o.symmetric_order
+             ? symmetric_permute(csr, o.symmetric_order, prp, pci, pva, rperm,
+                                 (hipStream_t)stream)
+             : reorder_rows_by_length(csr, prp, pci, pva, rperm, (hipStream_t)stream);
+    if (st == ACCSPMM_OK && o.symmetric_order) {
+      st = cperm.alloc((size_t)csr->cols * 4);
+      if (st == ACCSPMM_OK &&
+          hipMemcpyAsync(cperm.p, rperm.p, (size_t)csr->cols * 4, hipMemcpyDeviceToDevice,
+                         (hipStream_t)stream) != hipSuccess)
+        st = ACCSPMM_ERR_HIP;
+    }
     pv = accspmm_csr_view{csr->rows, csr->cols, csr->nnz, prp.as<uint32_t>(),
                           pci.as<uint32_t>(), pva.as<float>()};
     src = &pv;
@@ -699,7 +715,7 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
       (uint64_t)h->T >= 64ull * 8192) {
     uint64_t t16 = 0;
     const uint64_t t8 = h->T;
-    if (count_tiles(csr, 16, (hipStream_t)stream, &t16) == ACCSPMM_OK &&
+    if (count_tiles(src, 16, (hipStream_t)stream, &t16) == ACCSPMM_OK &&
         t16 * 100 <= t8 * 92) {
       (void)hipStreamSynchronize((hipStream_t)stream);
       delete h;
@@ -707,13 +723,13 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
       if (!h) return ACCSPMM_ERR_OUT_OF_MEMORY;
       accspmm_build_opts o16 = o;
       o16.row_window = 16;
-      st = build_impl(csr, o16, h, (hipStream_t)stream);
+      st = build_impl(src, o16, h, (hipStream_t)stream);
       if (st != ACCSPMM_OK) {
         (void)hipStreamSynchronize((hipStream_t)stream);
         delete h;
         h = new (std::nothrow) accspmm_metcf();
         if (!h) return ACCSPMM_ERR_OUT_OF_MEMORY;
-        st = build_impl(csr, o, h, (hipStream_t)stream);
+        st = build_impl(src, o, h, (hipStream_t)stream);
       }
     } else {
       (void)hipGetLastError();
@@ -724,6 +740,9 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
     delete h;
     return st;
   }
+  h->row_perm.swap(rperm);
+  h->col_perm.swap(cperm);
+  h->opts.symmetric_order = o.symmetric_order;
   *out = h;
   return ACCSPMM_OK;
 }
@@ -770,6 +789,7 @@ extern "C" accspmm_status accspmm_metcf_get_info(const accspmm_metcf* h,
   info->device_bytes = h->device_bytes();
   info->reordered = h->row_perm.p ? 1u : 0u;
   info->max_group_values = h->max_group_values;
+  info->symmetric_order = h->col_perm.p ? h->opts.symmetric_order : 0u;
   return ACCSPMM_OK;
 }
 
@@ -782,6 +802,18 @@ extern "C" accspmm_status accspmm_metcf_row_order(const accspmm_metcf* h,
   }
   ACC_HIP(hipSetDevice(h->device));
   ACC_HIP(hipMemcpy(row_order, h->row_perm.p, (size_t)h->rows * 4, hipMemcpyDeviceToHost));
+  return ACCSPMM_OK;
+}
+
+extern "C" accspmm_status accspmm_metcf_col_order(const accspmm_metcf* h,
+                                                  uint32_t* col_order) {
+  if (!h || !col_order) return ACCSPMM_ERR_INVALID_ARG;
+  if (!h->col_perm.p) {
+    for (uint32_t j = 0; j < h->cols; ++j) col_order[j] = j;
+    return ACCSPMM_OK;
+  }
+  ACC_HIP(hipSetDevice(h->device));
+  ACC_HIP(hipMemcpy(col_order, h->col_perm.p, (size_t)h->cols * 4, hipMemcpyDeviceToHost));
   return ACCSPMM_OK;
 }
 

@@ -17,7 +17,8 @@
 // output positions, and let the run heads add their runs in order.
 //
 // Also here: the row reordering the builder applies on request (rows by
-// decreasing length, a stable radix sort of the complemented lengths).
+// decreasing length, a stable radix sort of the complemented lengths) and
+// the symmetric permutation P A P^T of its locality orderings.
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -210,30 +211,56 @@ __global__ __launch_bounds__(kThreads) void k_copy_rows(const uint32_t* rp, cons
   }
 }
 
-}  // namespace
+// flags kBadCol / kUnsorted for a column >= cols or a row that is not
+// strictly increasing (row_ptr already checked)
+__global__ __launch_bounds__(kThreads) void k_check_canonical(const uint32_t* rp,
+                                                              const uint32_t* col, uint32_t rows,
+                                                              uint32_t cols, uint32_t nnz,
+                                                              uint32_t* flags) {
+  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  bool bad_col = false, unsorted = false;
+  if (i < nnz) {
+    const uint32_t r = find_row(rp, rows, (uint32_t)i);
+    const uint32_t c = col[i];
+    bad_col = c >= cols;
+    unsorted = i > rp[r] && col[i - 1] >= c;
+  }
+  const bool any_bad = __syncthreads_or(bad_col);
+  const bool any_unsorted = __syncthreads_or(unsorted);
+  if (threadIdx.x == 0 && (any_bad || any_unsorted))
+    atomicOr(flags, (any_bad ? kBadCol : 0u) | (any_unsorted ? kUnsorted : 0u));
+}
 
-namespace accspmm {
+// inv[perm[i]] = i
+__global__ __launch_bounds__(kThreads) void k_invert(const uint32_t* perm, uint32_t n,
+                                                     uint32_t* inv) {
+  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) inv[perm[i]] = (uint32_t)i;
+}
 
-accspmm_status reorder_rows_by_length(const accspmm_csr_view* in, DevBuf& rp, DevBuf& ci,
-                                      DevBuf& va, DevBuf& perm, hipStream_t s) {
-  const uint32_t rows = in->rows, nnz = in->nnz;
-  // the copies below index by row_ptr: it must run from 0 to nnz, in order
-  DevBuf flags, keys, keys_out, ids, len, temp;
-  ACC_TRY(flags.alloc(sizeof(uint32_t)));
-  ACC_HIP(hipMemsetAsync(flags.p, 0, sizeof(uint32_t), s));
-  k_check_rows<<<blocks_for((uint64_t)rows + 1), kThreads, 0, s>>>(in->row_ptr, rows, nnz,
-                                                                    flags.as<uint32_t>());
-  ACC_HIP(hipGetLastError());
-  uint32_t h_flags = 0;
-  ACC_HIP(hipMemcpyAsync(&h_flags, flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  ACC_HIP(hipStreamSynchronize(s));
-  if (h_flags) return ACCSPMM_ERR_INVALID_ARG;
+// One wave per output row: input row perm[i] with its columns relabelled
+// through inv (the row comes out unsorted).
+__global__ __launch_bounds__(kThreads) void k_copy_rows_relabel(
+    const uint32_t* rp, const uint32_t* ci, const float* va, const uint32_t* perm,
+    const uint32_t* inv, const uint32_t* nrp, uint32_t rows, uint32_t* nci, float* nva) {
+  const uint64_t i = (uint64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64;
+  if (i >= rows) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t src = rp[perm[i]], dst = nrp[i], len = nrp[i + 1] - dst;
+  for (uint32_t j = lane; j < len; j += 64u) {
+    nci[dst + j] = inv[ci[src + j]];
+    nva[dst + j] = va[src + j];
+  }
+}
 
+// perm = rows by decreasing length (stable); perm must hold `rows` words
+accspmm_status length_order(const uint32_t* row_ptr, uint32_t rows, DevBuf& perm,
+                            hipStream_t s) {
+  DevBuf keys, keys_out, ids, temp;
   ACC_TRY(keys.alloc(sizeof(uint32_t) * (size_t)rows));
   ACC_TRY(keys_out.alloc(sizeof(uint32_t) * (size_t)rows));
   ACC_TRY(ids.alloc(sizeof(uint32_t) * (size_t)rows));
-  ACC_TRY(perm.alloc(sizeof(uint32_t) * (size_t)rows));
-  k_length_keys<<<blocks_for(rows), kThreads, 0, s>>>(in->row_ptr, rows, keys.as<uint32_t>(),
+  k_length_keys<<<blocks_for(rows), kThreads, 0, s>>>(row_ptr, rows, keys.as<uint32_t>(),
                                                       ids.as<uint32_t>());
   ACC_HIP(hipGetLastError());
   size_t temp_bytes = 0;
@@ -244,6 +271,99 @@ accspmm_status reorder_rows_by_length(const accspmm_csr_view* in, DevBuf& rp, De
   ACC_HIP(rocprim::radix_sort_pairs(temp.p, temp_bytes, keys.as<uint32_t>(),
                                     keys_out.as<uint32_t>(), ids.as<uint32_t>(),
                                     perm.as<uint32_t>(), rows, 0u, 32u, s));
+  ACC_HIP(hipStreamSynchronize(s));
+  return ACCSPMM_OK;
+}
+
+}  // namespace
+
+namespace accspmm {
+
+accspmm_status symmetric_permute(const accspmm_csr_view* in, uint32_t order, DevBuf& rp,
+                                 DevBuf& ci, DevBuf& va, DevBuf& perm, hipStream_t s) {
+  const uint32_t n = in->rows, nnz = in->nnz;
+  if (in->rows != in->cols || (order != 1 && order != 2)) return ACCSPMM_ERR_UNSUPPORTED;
+  // the input must be canonical, as for any build (the relabelled rows are
+  // sorted below, which would otherwise hide duplicates or disorder)
+  DevBuf flags;
+  ACC_TRY(flags.alloc(sizeof(uint32_t)));
+  ACC_HIP(hipMemsetAsync(flags.p, 0, sizeof(uint32_t), s));
+  k_check_rows<<<blocks_for((uint64_t)n + 1), kThreads, 0, s>>>(in->row_ptr, n, nnz,
+                                                                 flags.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  uint32_t h_flags = 0;
+  ACC_HIP(hipMemcpyAsync(&h_flags, flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  ACC_HIP(hipStreamSynchronize(s));
+  if (h_flags) return ACCSPMM_ERR_INVALID_ARG;
+  if (nnz) {
+    k_check_canonical<<<blocks_for(nnz), kThreads, 0, s>>>(in->row_ptr, in->col_idx, n, n, nnz,
+                                                           flags.as<uint32_t>());
+    ACC_HIP(hipGetLastError());
+    ACC_HIP(hipMemcpyAsync(&h_flags, flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    ACC_HIP(hipStreamSynchronize(s));
+    if (h_flags & kBadCol) return ACCSPMM_ERR_INVALID_ARG;
+    if (h_flags & kUnsorted) return ACCSPMM_ERR_NOT_CANONICAL;
+  }
+  ACC_TRY(perm.alloc(sizeof(uint32_t) * (size_t)n));
+  if (order == 2) {
+    ACC_TRY(length_order(in->row_ptr, n, perm, s));
+  } else {
+    // reverse Cuthill-McKee on the host over the pattern
+    std::vector<uint32_t> hrp((size_t)n + 1), hci(nnz), hp(n);
+    ACC_HIP(hipMemcpyAsync(hrp.data(), in->row_ptr, hrp.size() * 4, hipMemcpyDeviceToHost, s));
+    if (nnz)
+      ACC_HIP(hipMemcpyAsync(hci.data(), in->col_idx, (size_t)nnz * 4, hipMemcpyDeviceToHost, s));
+    ACC_HIP(hipStreamSynchronize(s));
+    ACC_TRY(accspmm_order_rcm(n, hrp.data(), hci.data(), hp.data()));
+    ACC_HIP(hipMemcpyAsync(perm.p, hp.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+    ACC_HIP(hipStreamSynchronize(s));
+  }
+  DevBuf inv, len;
+  ACC_TRY(inv.alloc(sizeof(uint32_t) * (size_t)n));
+  k_invert<<<blocks_for(n), kThreads, 0, s>>>(perm.as<uint32_t>(), n, inv.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  ACC_TRY(len.alloc(sizeof(uint32_t) * ((size_t)n + 1)));
+  ACC_TRY(rp.alloc(sizeof(uint32_t) * ((size_t)n + 1)));
+  k_perm_lengths<<<blocks_for((uint64_t)n + 1), kThreads, 0, s>>>(
+      in->row_ptr, perm.as<uint32_t>(), n, len.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  ACC_TRY(exclusive_scan_u32(len.as<uint32_t>(), rp.as<uint32_t>(), (uint64_t)n + 1, s));
+  ACC_TRY(ci.alloc(sizeof(uint32_t) * (size_t)std::max(nnz, 1u)));
+  ACC_TRY(va.alloc(sizeof(float) * (size_t)std::max(nnz, 1u)));
+  const uint64_t waves_per_block = kThreads / 64;
+  k_copy_rows_relabel<<<(unsigned)std::max<uint64_t>(1, (n + waves_per_block - 1) /
+                                                            waves_per_block),
+                        kThreads, 0, s>>>(in->row_ptr, in->col_idx, in->val,
+                                          perm.as<uint32_t>(), inv.as<uint32_t>(),
+                                          rp.as<uint32_t>(), n, ci.as<uint32_t>(),
+                                          va.as<float>());
+  ACC_HIP(hipGetLastError());
+  // sort each relabelled row (a permutation creates no duplicates, so the
+  // count stays nnz)
+  uint32_t nnz_out = 0;
+  ACC_TRY(accspmm_csr_canonicalize(n, n, nnz, rp.as<uint32_t>(), ci.as<uint32_t>(),
+                                   va.as<float>(), &nnz_out, s));
+  if (nnz_out != nnz) return ACCSPMM_ERR_INVALID_ARG;
+  return ACCSPMM_OK;
+}
+
+accspmm_status reorder_rows_by_length(const accspmm_csr_view* in, DevBuf& rp, DevBuf& ci,
+                                      DevBuf& va, DevBuf& perm, hipStream_t s) {
+  const uint32_t rows = in->rows, nnz = in->nnz;
+  // the copies below index by row_ptr: it must run from 0 to nnz, in order
+  DevBuf flags, len;
+  ACC_TRY(flags.alloc(sizeof(uint32_t)));
+  ACC_HIP(hipMemsetAsync(flags.p, 0, sizeof(uint32_t), s));
+  k_check_rows<<<blocks_for((uint64_t)rows + 1), kThreads, 0, s>>>(in->row_ptr, rows, nnz,
+                                                                    flags.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  uint32_t h_flags = 0;
+  ACC_HIP(hipMemcpyAsync(&h_flags, flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  ACC_HIP(hipStreamSynchronize(s));
+  if (h_flags) return ACCSPMM_ERR_INVALID_ARG;
+
+  ACC_TRY(perm.alloc(sizeof(uint32_t) * (size_t)rows));
+  ACC_TRY(length_order(in->row_ptr, rows, perm, s));
   ACC_TRY(len.alloc(sizeof(uint32_t) * ((size_t)rows + 1)));
   ACC_TRY(rp.alloc(sizeof(uint32_t) * ((size_t)rows + 1)));
   k_perm_lengths<<<blocks_for((uint64_t)rows + 1), kThreads, 0, s>>>(

@@ -8,6 +8,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 #include "../../include/accspmm.h"
@@ -70,6 +71,10 @@ struct DevBuf {
     return alloc(n);
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
+  void swap(DevBuf& o) {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
+  }
 };
 
 // Work unit of the SpMM kernels: either a run of consecutive whole row
@@ -88,6 +93,8 @@ struct alignas(16) WorkUnit {
   uint32_t ngroups;
   uint32_t panel, pad1;
 };
+// A unit with nwin == 0 does nothing: it pads the per-XCD unit lists of a
+// pinned schedule (Schedule::pin_len).
 
 struct Schedule {
   uint32_t split_tiles = 0xFFFFFFFFu;
@@ -97,6 +104,10 @@ struct Schedule {
   uint32_t num_split_windows = 0;
   uint32_t num_pieces = 0;  // units with slot >= 0
   bool packed = false;      // some unit spans several non-empty windows
+  // > 0: units pinned to XCDs (tile.xcd_slice 3): XCD x runs units
+  // [x * pin_len, (x + 1) * pin_len), the column panels p with p % 8 == x
+  // and a load-balanced share of the whole windows, padded with no-op units
+  uint32_t pin_len = 0;
   DevBuf units;             // WorkUnit[num_units]
   DevBuf split_info;        // per split window: window, first slot, piece count
   std::vector<WorkUnit> h_units;  // host copy (the sweep plan is derived from it)
@@ -135,19 +146,22 @@ struct accspmm_metcf {
   uint32_t max_group_values = 0;  // values of the fullest 4-tile group
   accspmm_build_opts opts{};
   accspmm::DevBuf rwo, tco, bits, a2b, data, grp, rix;
-  // rows reordered at build (opts.reorder_rows): format row i is input row
-  // row_perm[i]; empty otherwise
+  // rows reordered at build (opts.reorder_rows or opts.symmetric_order):
+  // format row i is input row row_perm[i]; empty otherwise
   accspmm::DevBuf row_perm;
+  // symmetric_order: format column j is input column (B row) col_perm[j];
+  // empty otherwise
+  accspmm::DevBuf col_perm;
   std::vector<uint32_t> h_rwo;  // host copy of rowWindowOffset
   std::string hist_key;         // degree-histogram part of the tuning key
   // lazily built state (guarded by mu; handles are otherwise immutable)
   mutable std::mutex mu;
-  // keyed by (split_tiles, pack_tiles, col_panels)
-  mutable std::map<std::tuple<uint32_t, uint32_t, uint32_t>,
+  // keyed by (split_tiles, pack_tiles, col_panels, pinned to XCDs)
+  mutable std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>,
                    std::unique_ptr<accspmm::Schedule>>
       schedules;
   // host-only schedules of accspmm_spmm_noalloc (no device buffers), same key
-  mutable std::map<std::tuple<uint32_t, uint32_t, uint32_t>,
+  mutable std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>,
                    std::unique_ptr<accspmm::Schedule>>
       host_schedules;
   // keyed by (split_tiles, pack_tiles, col_panels, waves, slots)
@@ -159,7 +173,7 @@ struct accspmm_metcf {
   mutable std::map<hipStream_t, std::unique_ptr<accspmm::StreamWorkspace>> stream_ws;
   uint64_t device_bytes() const {
     return rwo.bytes + tco.bytes + bits.bytes + a2b.bytes + data.bytes +
-           grp.bytes + rix.bytes + row_perm.bytes;
+           grp.bytes + rix.bytes + row_perm.bytes + col_perm.bytes;
   }
 };
 
@@ -169,12 +183,20 @@ namespace accspmm {
 // of row i
 accspmm_status reorder_rows_by_length(const accspmm_csr_view* in, DevBuf& rp, DevBuf& ci,
                                       DevBuf& va, DevBuf& perm, hipStream_t s);
+// canon.hip: the canonical CSR of P A P^T into rp/ci/va, perm[i] = input
+// row (and column) at position i; order 1 reverse Cuthill-McKee, 2 rows by
+// decreasing length.  The input must be square and canonical.
+accspmm_status symmetric_permute(const accspmm_csr_view* in, uint32_t order, DevBuf& rp,
+                                 DevBuf& ci, DevBuf& va, DevBuf& perm, hipStream_t s);
 // scan.hip
 accspmm_status exclusive_scan_u32(const uint32_t* in, uint32_t* out,
                                   uint64_t n, hipStream_t s);
 accspmm_status inclusive_scan_u32(const uint32_t* in, uint32_t* out,
                                   uint64_t n, hipStream_t s);
 // spmm.hip
+// cache key of a tile's work-unit schedule: (split_tiles, pack_tiles,
+// col_panels, pinned to XCDs)
+std::tuple<uint32_t, uint32_t, uint32_t, uint32_t> schedule_key(const accspmm_tile& tile);
 accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
                             hipStream_t s, const Schedule** out);
 // one SpMM call (spmm.hip).  phases: bit 0 converts B (BF16 form), bit 1

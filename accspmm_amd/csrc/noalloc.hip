@@ -103,9 +103,7 @@ extern "C" accspmm_status accspmm_spmm_noalloc(const accspmm_metcf* h, const flo
     d_units = sch->units.as<WorkUnit>();
     d_sinfo = sch->split_info.as<uint32_t>();
   } else {
-    const uint32_t split_tiles = t.split_tiles ? t.split_tiles : 0xFFFFFFFFu;
-    const uint32_t P = std::max<uint32_t>(1, std::min<uint32_t>(t.col_panels, 1024));
-    const auto key = std::make_tuple(split_tiles, t.pack_tiles, P);
+    const auto key = schedule_key(t);
     auto it = h->host_schedules.find(key);
     if (it == h->host_schedules.end()) {
       // the panel cut search writes W * (P - 1) words, before the layout

@@ -5,10 +5,13 @@
 // lets a server warm-start its resident set without rebuilding.
 //
 // File layout (little-endian):
-//   "ACCSPMTC" (8 bytes), format version (u32, = 2; version 1 is read too)
+//   "ACCSPMTC" (8 bytes), format version (u32, = 3; versions 1 and 2 are
+//     read too)
 //   header (u32 each): rows, cols, nnz, row_window, W, T, G, ibd, balanced,
 //     max_window_tiles, has_adpbme, the 8 build options, tuning-key length,
-//     and from version 2 the reorder_rows option
+//     from version 2 the reorder_rows option (1 when the file holds a row
+//     order), from version 3 the symmetric_order option (then the row order
+//     is the column order too)
 //   tuning key (bytes)
 //   rowWindowOffset[W+1] u32, tcOffset[T+1] u32, tcLocalBit[T*RW/8] u64,
 //   sparseA2B[8T] u32, data[nnz] f32, and with AdpBME groupOffset[G+1] u32,
@@ -32,9 +35,10 @@ using namespace accspmm;
 namespace {
 
 constexpr char kMagic[8] = {'A', 'C', 'C', 'S', 'P', 'M', 'T', 'C'};
-constexpr uint32_t kVersion = 2;
+constexpr uint32_t kVersion = 3;
 constexpr uint32_t kHeaderWordsV1 = 19;
-constexpr uint32_t kHeaderWords = 20;
+constexpr uint32_t kHeaderWordsV2 = 20;
+constexpr uint32_t kHeaderWords = 21;
 
 struct Fnv {
   uint64_t h = 1469598103934665603ull;
@@ -123,7 +127,8 @@ extern "C" accspmm_status accspmm_metcf_save(const accspmm_metcf* h, const char*
       h->rows, h->cols, h->nnz, h->rw, W, T, h->G, h->ibd, h->balanced,
       h->max_window_tiles, adp ? 1u : 0u, h->opts.row_window, h->opts.col_window,
       h->opts.t_load, h->opts.t_comp, h->opts.t_write, h->opts.target_num,
-      h->opts.imbalance_threshold, (uint32_t)h->hist_key.size(), reordered ? 1u : 0u};
+      h->opts.imbalance_threshold, (uint32_t)h->hist_key.size(), reordered ? 1u : 0u,
+      h->col_perm.p ? h->opts.symmetric_order : 0u};
   w.put(kMagic, sizeof(kMagic));
   w.put(&kVersion, 4);
   w.put(hdr, sizeof(hdr));
@@ -156,15 +161,16 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
   uint32_t version = 0, hdr[kHeaderWords] = {};
   r.get(magic, 8);
   r.get(&version, 4);
-  if (!r.ok || std::memcmp(magic, kMagic, 8) != 0 || (version != 1 && version != kVersion))
+  if (!r.ok || std::memcmp(magic, kMagic, 8) != 0 || version < 1 || version > kVersion)
     return ACCSPMM_ERR_FILE;
-  r.get(hdr, 4 * (version == 1 ? kHeaderWordsV1 : kHeaderWords));
+  r.get(hdr, 4 * (version == 1 ? kHeaderWordsV1 : version == 2 ? kHeaderWordsV2 : kHeaderWords));
   const uint32_t rows = hdr[0], cols = hdr[1], nnz = hdr[2], rw = hdr[3], W = hdr[4],
                  T = hdr[5], G = hdr[6], adp = hdr[10], key_len = hdr[18],
-                 reorder = hdr[19];
+                 reorder = hdr[19], sym = hdr[20];
   if (!r.ok || (rw != 8 && rw != 16) || rows == 0 || cols == 0 || nnz > kMaxNnz ||
       W != (rows + rw - 1) / rw || adp > 1 || reorder > 1 || key_len > 4096 ||
-      (uint64_t)T * kColWindow > 0xFFFFFFFFull)
+      (uint64_t)T * kColWindow > 0xFFFFFFFFull || sym > 2 ||
+      (sym && (!reorder || rows != cols)))
     return ACCSPMM_ERR_FILE;
   const uint32_t words = rw / 8;
   std::string key(key_len, '\0');
@@ -227,7 +233,7 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
   h->balanced = hdr[8];
   h->max_window_tiles = hdr[9];
   h->opts = accspmm_build_opts{hdr[11], hdr[12], hdr[13], hdr[14], hdr[15], hdr[16], hdr[17],
-                               adp, reorder};
+                               adp, sym ? 0u : reorder, sym};
   h->hist_key = key;
   h->h_rwo = rwo;
   // device arrays with the kernels' padding: tcOffset[T+1] repeats nnz, the
@@ -246,6 +252,7 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
     ACC_TRY(to_device(h->rix, rix, rix.size()));
   }
   if (reorder) ACC_TRY(to_device(h->row_perm, order, order.size()));
+  if (sym) ACC_TRY(to_device(h->col_perm, order, order.size()));
   ACC_HIP(hipDeviceSynchronize());
   ACC_TRY(measure_group_values(h.get(), nullptr));
   *out = h.release();

@@ -83,13 +83,24 @@ __global__ __launch_bounds__(NB == 8 ? 256 : 512) void spmm_kernel(SpmmArgs g) {
     const uint32_t tau = ((blockIdx.x & 7u) * per + (blockIdx.x >> 3)) * wpb + wave;
     unit = tau / g.n_kc;
     kc = tau % g.n_kc;
+  } else if (g.xcd_slice == 3) {
+    // pinned schedule: XCD x runs its own unit list [x * pin_len, ...), the
+    // column panels p = x (mod 8), so each B row is cached by one XCD's L2
+    const uint32_t i = (blockIdx.x >> 3) * wpb + wave;
+    const uint32_t slot = i / g.n_kc;
+    if (slot >= g.pin_len) return;
+    kc = i % g.n_kc;
+    unit = (blockIdx.x & 7u) * g.pin_len + slot;
   } else {
     const uint32_t tau = blockIdx.x * wpb + wave;
     unit = tau / g.n_kc;
     kc = tau % g.n_kc;
   }
   if (unit >= g.num_units) return;
-  __shared__ __align__(16) float lds_vals[8 * wave_lds<RW>()];
+  // per-wave value buffers, sized by the launch (wpb * wave_lds floats): a
+  // fixed 8-wave array would hold 16.5 KiB per 1-wave block and cap the CU
+  // at 9 resident blocks
+  extern __shared__ __align__(16) float lds_vals[];
   float* lv = lds_vals + wave * wave_lds<RW>();
   if ((threadIdx.x & 63u) == 0) lv[group_values<RW>()] = 0.f;
   const WorkUnit u = g.units[unit];
@@ -103,11 +114,12 @@ __global__ __launch_bounds__(NB == 8 ? 256 : 512) void spmm_kernel(SpmmArgs g) {
 template <int FORM>
 __global__ __launch_bounds__(256) void convert_b(const float* __restrict__ B, uint32_t N,
                                                  uint32_t K, uint32_t ldb, void* __restrict__ out,
-                                                 uint32_t ldx, uint32_t vec) {
+                                                 uint32_t ldx, uint32_t vec,
+                                                 const uint32_t* __restrict__ perm) {
   const uint64_t total = (uint64_t)(N + 1) * (ldx / 4);
   for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total;
        c += (uint64_t)gridDim.x * blockDim.x)
-    convert_chunk<FORM>(B, N, K, ldb, out, ldx, vec, c);
+    convert_chunk<FORM>(B, N, K, ldb, out, ldx, vec, c, perm);
 }
 
 // For window w and panel boundary p in [1, P): the number of the window's
@@ -166,6 +178,12 @@ constexpr uint32_t kPanelMinTiles = 8;
 uint64_t panel_bound_words(const accspmm_metcf* h, const accspmm_tile& tile) {
   const uint32_t P = std::max<uint32_t>(1, std::min<uint32_t>(tile.col_panels, 1024));
   return P > 1 && h->W > 0 && h->T > 0 ? (uint64_t)h->W * (P - 1) : 0;
+}
+
+std::tuple<uint32_t, uint32_t, uint32_t, uint32_t> schedule_key(const accspmm_tile& tile) {
+  const uint32_t split_tiles = tile.split_tiles ? tile.split_tiles : 0xFFFFFFFFu;
+  const uint32_t P = std::max<uint32_t>(1, std::min<uint32_t>(tile.col_panels, 1024));
+  return std::make_tuple(split_tiles, tile.pack_tiles, P, tile.xcd_slice == 3 && P > 1 ? 1u : 0u);
 }
 
 accspmm_status plan_schedule(const accspmm_metcf* h, const accspmm_tile& tile, hipStream_t s,
@@ -274,7 +292,34 @@ accspmm_status plan_schedule(const accspmm_metcf* h, const accspmm_tile& tile, h
     if (open.tile_end - open.tile_begin >= pack_tiles) close_open();
   }
   close_open();
-  for (auto& v : by_panel) units.insert(units.end(), v.begin(), v.end());
+  sch->pin_len = 0;
+  if (tile.xcd_slice == 3 && P > 1) {
+    // pinned: XCD x gets the pieces of the panels p = x (mod 8) in window
+    // order, then whole-window units where its tile count is lowest; the
+    // eight lists are padded with no-op units to one length
+    std::vector<std::vector<WorkUnit>> lists(8);
+    std::vector<uint64_t> load(8, 0);
+    for (uint32_t p = 0; p < P; ++p)
+      for (const WorkUnit& u : by_panel[p]) {
+        lists[p % 8].push_back(u);
+        load[p % 8] += u.tile_end - u.tile_begin;
+      }
+    for (const WorkUnit& u : units) {
+      const uint32_t x = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+      lists[x].push_back(u);
+      load[x] += (u.tile_end - u.tile_begin) + u.nwin;
+    }
+    size_t len = 1;
+    for (const auto& l : lists) len = std::max(len, l.size());
+    if (len * 8 > 0xFFFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
+    std::vector<WorkUnit> pinned(len * 8, WorkUnit{0, 0, 0, 0, -1, 0, 0, 0});
+    for (uint32_t x = 0; x < 8; ++x)
+      std::copy(lists[x].begin(), lists[x].end(), pinned.begin() + (size_t)x * len);
+    units = std::move(pinned);
+    sch->pin_len = (uint32_t)len;
+  } else {
+    for (auto& v : by_panel) units.insert(units.end(), v.begin(), v.end());
+  }
   sch->num_units = (uint32_t)units.size();
   sch->packed = false;
   for (const WorkUnit& u : units) sch->packed |= (u.slot < 0 && u.nwin > 1 && u.ngroups > 0);
@@ -310,9 +355,7 @@ accspmm_status upload_schedule(const Schedule& sch, void* units, void* sinfo, hi
 // and uploaded to the handle's own device buffers on first use.
 accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
                             hipStream_t s, const Schedule** out) {
-  const uint32_t split_tiles = tile.split_tiles ? tile.split_tiles : 0xFFFFFFFFu;
-  const uint32_t P = std::max<uint32_t>(1, std::min<uint32_t>(tile.col_panels, 1024));
-  const auto key = std::make_tuple(split_tiles, tile.pack_tiles, P);
+  const auto key = schedule_key(tile);
   auto it = h->schedules.find(key);
   if (it != h->schedules.end()) {
     *out = it->second.get();
@@ -336,9 +379,7 @@ accspmm_status get_schedule(const accspmm_metcf* h, const accspmm_tile& tile,
 
 // The handle's cached schedule for the tile, or null (no allocation).
 const Schedule* find_schedule(const accspmm_metcf* h, const accspmm_tile& tile) {
-  const uint32_t split_tiles = tile.split_tiles ? tile.split_tiles : 0xFFFFFFFFu;
-  const uint32_t P = std::max<uint32_t>(1, std::min<uint32_t>(tile.col_panels, 1024));
-  auto it = h->schedules.find(std::make_tuple(split_tiles, tile.pack_tiles, P));
+  auto it = h->schedules.find(schedule_key(tile));
   return it == h->schedules.end() ? nullptr : it->second.get();
 }
 
@@ -349,12 +390,13 @@ void launch_t(const SpmmArgs& g, uint32_t blocks, uint32_t wpb, bool packed,
               hipStream_t s) {
   const bool buf = gather_by_buffer(g, FORM);
   const dim3 grid(blocks), blk(64 * wpb);
+  const size_t lds = (size_t)wpb * wave_lds<RW>() * sizeof(float);
   if (packed) {
-    if (buf) hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, true, true>), grid, blk, 0, s, g);
-    else hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, true, false>), grid, blk, 0, s, g);
+    if (buf) hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, true, true>), grid, blk, lds, s, g);
+    else hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, true, false>), grid, blk, lds, s, g);
   } else {
-    if (buf) hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, false, true>), grid, blk, 0, s, g);
-    else hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, false, false>), grid, blk, 0, s, g);
+    if (buf) hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, false, true>), grid, blk, lds, s, g);
+    else hipLaunchKernelGGL((spmm_kernel<FORM, NB, RW, false, false>), grid, blk, lds, s, g);
   }
 }
 
@@ -494,6 +536,8 @@ accspmm_status launch_single(const SpmmArgs& g, int form, uint32_t rw, uint32_t 
   } else if (g.xcd_slice == 2) {
     blocks = ((uint64_t)num_units * n_kc + wpb - 1) / wpb;
     blocks = (blocks + 7) / 8 * 8;
+  } else if (g.xcd_slice == 3) {
+    blocks = 8 * (((uint64_t)g.pin_len * n_kc + wpb - 1) / wpb);
   } else {
     blocks = ((uint64_t)num_units * n_kc + wpb - 1) / wpb;
   }
@@ -519,7 +563,11 @@ void spmm_workspace_need(const accspmm_metcf* h, const Schedule* sch, uint32_t K
   const uint64_t kcp = (uint64_t)((K + KC - 1) / KC) * KC;
   const int kform = kernel_form(form, tile);
   const uint64_t pb = tile.sweep ? 0 : (uint64_t)sch->num_pieces * h->rw * kcp * 4;
-  const uint64_t cb = raw_b(kform) ? 0 : ((uint64_t)h->cols + 1) * kcp * (kform == kBF16 ? 2 : 4);
+  // the FP32-gathering forms read the caller's B, or a permuted copy when
+  // the graph has a column order
+  const uint64_t cb = raw_b(kform) && !h->col_perm.p
+                          ? 0
+                          : ((uint64_t)h->cols + 1) * kcp * (kform == kBF16 ? 2 : 4);
   *partial_bytes = (pb + 255) / 256 * 256;
   *bconv_bytes = (cb + 255) / 256 * 256;
 }
@@ -574,15 +622,18 @@ accspmm_status spmm_run(const accspmm_metcf* h, const Schedule* sch, const WorkU
   g.num_units = sch->num_units;
   g.n_kc = n_kc;
   g.kc_padded = n_kc * KC;
-  g.xcd_slice = tile.xcd_slice == 2 ? 2u
+  g.xcd_slice = sch->pin_len ? 3u
+                : tile.xcd_slice == 2 ? 2u
                 : (tile.xcd_slice == 1 && (8 % n_kc) == 0) ? 1u : 0u;
+  g.pin_len = sch->pin_len;
   g.zero_row = h->cols;
   g.b_rows = raw_b(kform) ? h->cols : h->cols + 1;
   g.pad_tile = h->T;
   g.wide = h->rw == 16 && h->max_group_values > dev::kSpanValues ? 1u : 0u;
   g.partials = pbytes ? w.partials : nullptr;
 
-  if (raw_b(kform)) {
+  const uint32_t* perm = h->col_perm.p ? h->col_perm.as<uint32_t>() : nullptr;
+  if (raw_b(kform) && !perm) {
     g.B = B;
     g.ldb = ldb;
   } else {
@@ -591,12 +642,15 @@ accspmm_status spmm_run(const accspmm_metcf* h, const Schedule* sch, const WorkU
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((chunks + 255) / 256, 8192);
     const uint32_t vec = (ldb % 4 == 0 && ((uintptr_t)B % 16) == 0) ? 1u : 0u;
     if (phases & 1) {
-      if (kform == kSPLIT)
+      if (raw_b(kform))
+        hipLaunchKernelGGL(convert_b<kFP32>, dim3(blocks), dim3(256), 0, s, B, h->cols, K, ldb,
+                           w.bconv, ldx, vec, perm);
+      else if (kform == kSPLIT)
         hipLaunchKernelGGL(convert_b<kSPLIT>, dim3(blocks), dim3(256), 0, s, B, h->cols, K, ldb,
-                           w.bconv, ldx, vec);
+                           w.bconv, ldx, vec, perm);
       else
         hipLaunchKernelGGL(convert_b<kBF16>, dim3(blocks), dim3(256), 0, s, B, h->cols, K, ldb,
-                           w.bconv, ldx, vec);
+                           w.bconv, ldx, vec, perm);
       ACC_HIP(hipGetLastError());
     }  // else converted by an earlier prepare call on this stream
     g.B = w.bconv;

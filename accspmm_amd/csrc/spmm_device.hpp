@@ -55,6 +55,7 @@ struct SpmmArgs {
   uint32_t n_kc;  // feature chunks
   uint32_t kc_padded;  // KC * n_kc (partials row stride)
   uint32_t xcd_slice;
+  uint32_t pin_len;    // xcd_slice 3: units per XCD (Schedule::pin_len)
   uint32_t rw, nb;     // used by the batched kernel's dispatch
   // 1: 16-row windows with groups of more than 256 values (the graph's
   // max_group_values), which load their values in two spans
@@ -118,21 +119,24 @@ __device__ inline void split3(uint32_t x, uint32_t y, uint32_t& t0, uint32_t& t1
                              __builtin_bit_cast(uint32_t, sx), 0x07060302u);
 }
 
-// B conversion for the converted forms, one chunk of four consecutive
-// features: chunk c of the (N + 1) x ldx / 4 chunks of the converted B (ldx
-// a multiple of 16, zero past K and in the all-zero row N that padded tile
-// columns read).  A 16-byte load when the segment lies inside K and B is
-// 16-byte aligned with ldb % 4 == 0 (vec); one 16-byte (split: hi | lo << 16
-// per feature, split2's rounding) or 8-byte (BF16) store.
+// B conversion, one chunk of four consecutive features: chunk c of the
+// (N + 1) x ldx / 4 chunks of the converted B (ldx a multiple of 16, zero
+// past K and in the all-zero row N that padded tile columns read).  Row r
+// takes B's row perm[r] (a graph built with a symmetric ordering), or row r
+// when perm is null.  A 16-byte load when the segment lies inside K and B
+// is 16-byte aligned with ldb % 4 == 0 (vec); one 16-byte (split: hi | lo
+// << 16 per feature, split2's rounding; FP32: the values, the permuted copy
+// the FP32-gathering forms read) or 8-byte (BF16) store.
 template <int FORM>
 __device__ inline void convert_chunk(const float* __restrict__ B, uint32_t N, uint32_t K,
                                      uint32_t ldb, void* __restrict__ out, uint32_t ldx,
-                                     uint32_t vec, uint64_t c) {
+                                     uint32_t vec, uint64_t c,
+                                     const uint32_t* __restrict__ perm) {
   const uint32_t cpr = ldx / 4;  // chunks per row
   const uint32_t r = (uint32_t)(c / cpr), k = (uint32_t)(c % cpr) * 4;
   float x[4] = {0.f, 0.f, 0.f, 0.f};
   if (r < N) {
-    const float* src = B + (uint64_t)r * ldb + k;
+    const float* src = B + (uint64_t)(perm ? perm[r] : r) * ldb + k;
     if (vec && k + 4 <= K) {
       const f4 v = *reinterpret_cast<const f4*>(src);
       x[0] = v[0]; x[1] = v[1]; x[2] = v[2]; x[3] = v[3];
@@ -141,7 +145,9 @@ __device__ inline void convert_chunk(const float* __restrict__ B, uint32_t N, ui
       for (int i = 0; i < 4; ++i) x[i] = k + i < K ? src[i] : 0.f;
     }
   }
-  if (FORM == kSPLIT) {
+  if (FORM == kFP32) {
+    reinterpret_cast<f4*>(out)[c] = f4{x[0], x[1], x[2], x[3]};
+  } else if (FORM == kSPLIT) {
     u4 o;
 #pragma unroll
     for (int i = 0; i < 4; i += 2) {
@@ -801,6 +807,7 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
                          const void* B, uint32_t ldb, float* C, uint32_t ldc,
                          uint32_t K, uint32_t rows, float* lv) {
   constexpr uint32_t KC = 16u * NB;
+  if (u.nwin == 0) return;  // padding of a pinned schedule's unit lists
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t q = lane >> 4, n = lane & 15u;
   const uint32_t k0 = kc * KC;

@@ -53,7 +53,7 @@ std::string make_key(const accspmm_metcf* h, uint32_t K, int form) {
 bool valid_tile(const accspmm_tile& t) {
   const uint32_t nb = t.feat_per_lane;
   return (nb == 1 || nb == 2 || nb == 4 || nb == 8) && t.waves_per_block <= 8 &&
-         t.xcd_slice <= 2 && t.sweep <= 1 && t.b_split <= 2 && t.col_panels <= 1024;
+         t.xcd_slice <= 3 && t.sweep <= 1 && t.b_split <= 2 && t.col_panels <= 1024;
 }
 
 void load_file(const char* path, std::map<std::string, accspmm_tile>& into) {

@@ -81,6 +81,20 @@ typedef struct accspmm_build_opts {
    * ddi.my_reorder.mtx in its tree).  2: reorder when rows average >= 128
    * nonzeros (measured faster there, slower on short-row graphs). */
   uint32_t reorder_rows;
+  /* Locality ordering of a square matrix, applied to rows and columns alike
+   * (A' = P A P^T; reorder_rows is then ignored): 0 (default) none; 1
+   * reverse Cuthill-McKee (host breadth-first search, accspmm_order_rcm):
+   * banded and mesh-like graphs stored in a scattered order get their band
+   * back, so neighbouring row windows gather neighbouring B rows; 2 rows
+   * and columns by decreasing row length (hub rows and hub columns first).
+   * B is permuted the same way inside each SpMM call (by the conversion
+   * pass, or a permuted copy for the forms that gather FP32 B) and C is
+   * written in input row order, so results keep their input layout.  The
+   * exported arrays describe P A P^T (accspmm_metcf_row_order /
+   * _col_order give P).  No reference counterpart (the reference reads an
+   * externally reordered file, ddi.my_reorder.mtx in its tree).
+   * ACCSPMM_ERR_UNSUPPORTED for a non-square matrix. */
+  uint32_t symmetric_order;
 } accspmm_build_opts;
 
 void accspmm_build_opts_default(accspmm_build_opts* o);
@@ -109,6 +123,7 @@ typedef struct accspmm_metcf_info {
    * first tile); with 16-row windows, groups of more than 256 values
    * (tiles over half full) take a slower two-span load in the kernels */
   uint32_t max_group_values;
+  uint32_t symmetric_order; /* the build's symmetric_order (0: none)      */
 } accspmm_metcf_info;
 
 /* GPU METCF builder.  Replaces METCFBit::convertFromCSR
@@ -161,6 +176,12 @@ accspmm_status accspmm_metcf_export(const accspmm_metcf* h,
  * with reorder_rows = 1).  No reference counterpart. */
 accspmm_status accspmm_metcf_row_order(const accspmm_metcf* h, uint32_t* row_order);
 
+/* Column order of the format: format column j is input column col_order[j]
+ * (HOST buffer of `cols` entries; the identity unless built with a
+ * symmetric_order, then equal to the row order).  No reference
+ * counterpart. */
+accspmm_status accspmm_metcf_col_order(const accspmm_metcf* h, uint32_t* col_order);
+
 /* Binary save / load of a built graph (no reference counterpart: the
  * reference rebuilds the format from the .mtx file on every run,
  * src/mma_tf32.cu:195-226).  The file holds the build options, the arrays
@@ -191,7 +212,10 @@ typedef struct accspmm_tile {
   uint32_t split_tiles;    /* windows above this many tiles are split (0: no)  */
   uint32_t xcd_slice;      /* XCD mapping of work: 0 round-robin, 1 pin
                               feature chunks to XCDs, 2 one contiguous range
-                              of work units per XCD                          */
+                              of work units per XCD, 3 pin column panels to
+                              XCDs (panel p on XCD p % 8, so each B row is
+                              cached in one XCD's L2; needs col_panels > 1,
+                              else 0)                                        */
   uint32_t pack_tiles;     /* consecutive small windows are packed into one
                               work unit of at least this many tiles (0: one
                               window per unit)                               */
@@ -396,6 +420,16 @@ accspmm_status accspmm_gen_stencil27(uint32_t n, uint64_t seed,
                                      accspmm_host_csr* out);
 /* Dense row-major rows x cols, U[-1,1]. */
 void accspmm_gen_dense(uint64_t seed, uint64_t count, float* out);
+
+/* Reverse Cuthill-McKee order of the n x n pattern (HOST arrays; row i's
+ * columns are its neighbours): perm[i] = the node placed at position i.
+ * Components are ordered from pseudo-peripheral start nodes, breadth
+ * first, neighbours by increasing degree (ties: smaller index), then the
+ * whole order is reversed.  Deterministic.  The builder's symmetric_order
+ * 1 uses it.  ACCSPMM_ERR_INVALID_ARG for a bad row_ptr or a column >= n.
+ * No reference counterpart. */
+accspmm_status accspmm_order_rcm(uint32_t n, const uint32_t* row_ptr,
+                                 const uint32_t* col_idx, uint32_t* perm);
 
 /* ------------------------------------------------------------------------ */
 /* Serving: resident graphs, HBM-budget admission, continuous batching.     */
