@@ -393,8 +393,9 @@ def order_rcm(row_ptr, col_idx, n: int | None = None) -> np.ndarray:
 
 
 def autotune_layout(csr: CSR, B, C_out, form: int = FORM_FP32, stream=None):
-    """Fastest of the four layouts (row window 8/16 x input/reordered rows)
-    for (csr, B, form): returns (BuildOpts, ms) (accspmm_autotune_layout)."""
+    """Fastest layout (row window 8/16 x input rows, rows by length, and
+    reverse Cuthill-McKee for square matrices) for (csr, B, form): returns
+    (BuildOpts, ms) (accspmm_autotune_layout)."""
     rp, ci, va = csr.to_device()
     view = CsrView(csr.rows, csr.cols, csr.nnz, rp.data_ptr(), ci.data_ptr(), va.data_ptr())
     o = BuildOpts()

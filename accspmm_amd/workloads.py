@@ -10,6 +10,9 @@ full-size GPU tests so that both run exactly the same matrices:
   symmetric values; B seed 2 (+ K).
 * chung_lu_2m: one of the serving pool's two 2,000,000-node graphs (mean
   degree 32, exponent 2.1, U(0,1] values, graph seed 3100).
+* stencil27_100_scrambled: the 100^3 stencil with its nodes relabelled by a
+  seeded random permutation (P A P^T), the case a locality ordering must
+  undo; B seed 14 + K.
 * the batch: graph i has 500-50,000 nodes (log-uniform), mean degree
   U[4,64], exponent U[2.0,2.5], U(0,1] values, graph seed 1000 + i, K from
   {16, 64, 128}, B seed 2000 + i (numpy Generator seeded 1000 draws the
@@ -43,6 +46,36 @@ def make_graph(spec: dict):
         return gen_chung_lu(spec["n"], spec["deg"], spec["gamma"], spec["seed"],
                             spec["values"])
     return gen_stencil27(spec["g"], spec["seed"])
+
+
+def symmetric_permuted(csr, perm):
+    """P A P^T as a canonical CSR: row i is input row perm[i], columns
+    relabelled by the inverse permutation and sorted within each row."""
+    from . import CSR
+    n = csr.rows
+    perm = np.asarray(perm, np.int64)
+    inv = np.empty(n, np.int64)
+    inv[perm] = np.arange(n)
+    rp = csr.row_ptr.astype(np.int64)
+    lens = np.diff(rp)[perm]
+    nrp = np.concatenate([[0], np.cumsum(lens)])
+    src = np.repeat(rp[perm] - nrp[:-1], lens) + np.arange(nrp[-1])
+    rows = np.repeat(np.arange(n), lens)
+    cols = inv[csr.col_idx.astype(np.int64)[src]]
+    order = np.lexsort((cols, rows))
+    return CSR(n, n, nrp.astype(np.uint32), cols[order].astype(np.uint32),
+               csr.data[src][order].astype(np.float32))
+
+
+def make_graph_by_name(name: str):
+    """(CSR, B seed) of a WORKLOADS entry or of stencil27_100_scrambled."""
+    if name == "stencil27_100_scrambled":
+        base = WORKLOADS["stencil27_100"]
+        c = make_graph(base)
+        p = np.random.default_rng(17).permutation(c.rows)
+        return symmetric_permuted(c, p), base["bseed"]
+    spec = WORKLOADS[name]
+    return make_graph(spec), spec["bseed"]
 
 
 def batch_specs(ngraphs: int = 256) -> list[dict]:

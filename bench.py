@@ -357,6 +357,55 @@ def ingest_bench(workload: str) -> dict:
     return out
 
 
+ORDERINGS = (
+    ("input", dict(reorder_rows=0)),
+    ("rows_by_length", dict(reorder_rows=1)),
+    ("degree_symmetric", dict(symmetric_order=2)),
+    ("rcm", dict(symmetric_order=1)),
+)
+
+
+def ordering_bench(workload: str, K: int, form_name: str, steps: int, warmup: int) -> dict:
+    """The locality orderings on one workload (automatic row window each):
+    tiles T, tile fill nnz / (T * RW * 8), window-gather bytes, build time
+    and the SpMM step time of `form_name` at its library tile."""
+    import torch
+
+    import accspmm_amd as acc
+    from accspmm_amd.workloads import make_graph_by_name
+
+    name_to_form = {"fp32": acc.FORM_FP32, "split_bf16": acc.FORM_SPLIT_BF16,
+                    "bf16": acc.FORM_BF16}
+    form = name_to_form[form_name]
+    csr, bseed = make_graph_by_name(workload)
+    B = torch.from_numpy(acc.gen_dense(bseed + K, csr.cols, K)).cuda()
+    Cd = torch.empty((csr.rows, K), dtype=torch.float32, device="cuda")
+    out = {"workload": workload, "K": K, "form": form_name, "rows": csr.rows, "nnz": csr.nnz}
+    for label, kw in ORDERINGS:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g = acc.DeviceGraph(csr, acc.default_build_opts(0, **kw))
+        torch.cuda.synchronize()
+        build_ms = (time.perf_counter() - t0) * 1e3
+        inf = g.info
+        diag = format_diagnostics(g, K, form)
+        tile = g.tile_select(K, form)
+        r = time_form(g, B, Cd, form, tile, steps, warmup)
+        row = {"row_window": inf.row_window, "tiles": inf.num_tiles,
+               "tile_fill": sig(inf.nnz / (inf.num_tiles * inf.row_window * 8.0)),
+               "gather_gb": sig(diag["gather_bytes"] / 1e9),
+               "build_ms": round(build_ms, 1), "ms_per_step": sig(r["ms_per_step"]),
+               "kernel_ms": sig(r["kernel_ms_mean"]),
+               "gflops": round(2.0 * csr.nnz * K / (r["ms_per_step"] * 1e-3) / 1e9, 1),
+               "tile": list(tile.as_tuple())}
+        out[label] = row
+        log(f"ordering: {workload} {label}: RW {row['row_window']}, {row['tiles']} tiles, "
+            f"fill {row['tile_fill']}, gathers {row['gather_gb']} GB, build {build_ms:.0f} ms, "
+            f"{row['ms_per_step']} ms/step ({row['gflops']} GFLOP/s)")
+        del g
+    return out
+
+
 def batch_bench(ngraphs: int, steps: int, warmup: int, tune: bool = True):
     """Batching: ngraphs Chung-Lu graphs (500-50,000 nodes log-uniform, mean
     degree U[4,64], exponent U[2.0,2.5], U(0,1] values, graph seed 1000+i),
@@ -439,6 +488,12 @@ def batch_bench(ngraphs: int, steps: int, warmup: int, tune: bool = True):
     return out
 
 
+# serving HBM budget beyond the resident graphs: in-flight workspaces (B
+# conversion and split partials) must fit in it, so at most two requests
+# on a 2M-node graph at K = 128 (1.03 GB of converted B each) run at once
+SERVE_WORKSPACE_BUDGET = 3 << 30
+
+
 def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000,
                 row_window: int = 0, reorder: int = 2):
     """Serving: Poisson arrivals over a pool of 64 resident Chung-Lu graphs
@@ -446,8 +501,12 @@ def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000,
     2M-node mean-degree-32 graphs; each request is one graph (uniform) and K
     uniform in {32, 128}, split-BF16.  Runs the continuous-batching server and
     the one-launch-per-request baseline: saturation throughput of each, then
-    p50/p99 latency of both at the same fixed QPS.  Graphs are registered
-    with the given row window and reorder_rows build options."""
+    p50/p99 latency of both at the same fixed QPS.  Both servers admit
+    against an HBM budget of the resident graphs plus SERVE_WORKSPACE_BUDGET.
+    Then the same request stream split by class (requests on the two 2M-node
+    graphs; on the 64 small ones): each class's saturation time per request in
+    both modes, the measured cap of the throughput ratio.  Graphs are
+    registered with the given row window and reorder_rows build options."""
     import numpy as np
     import torch
 
@@ -463,42 +522,57 @@ def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000,
             bufs[(gi, K)] = (B, Cs)
     gsel, ksel = serving_requests(len(pool), requests, seed)
     flops = np.array([2.0 * pool[g].nnz * k for g, k in zip(gsel, ksel)])
+    big = [gi for gi, c in enumerate(pool) if c.rows >= big_nodes]
+    is_big = np.isin(gsel, big)
 
-    def run(server, qps):
-        gaps = poisson_gaps(requests, qps)
+    def run(server, qps, sel=None):
+        idx = np.arange(requests) if sel is None else np.flatnonzero(sel)
+        gaps = poisson_gaps(len(idx), qps)
         rids = []
         t0 = time.perf_counter()
         t_next = t0
-        for i in range(requests):
-            t_next += gaps[i]
+        for j, i in enumerate(idx):
+            t_next += gaps[j]
             while qps and time.perf_counter() < t_next:
                 pass
             B, Cs = bufs[(int(gsel[i]), int(ksel[i]))]
-            rids.append(server.submit(gids[int(gsel[i])], B, Cs[i % 2]))
+            rids.append(server.submit(gids[int(gsel[i])], B, Cs[j % 2]))
         server.drain()
         t1 = time.perf_counter()
         lats = np.array([server.poll(r) for r in rids])
         return t1 - t0, lats
 
+    def server(mode):
+        s = acc.Server(hbm_budget_bytes=budget, max_batch=64, max_inflight=3,
+                       form=acc.FORM_SPLIT_BF16, one_per_launch=(mode == "one_per_launch"))
+        return s, [s.register(c, opts) for c in pool]
+
+    # the budget: what the pool holds resident (measured once) plus the
+    # in-flight workspace allowance
+    probe = acc.Server(hbm_budget_bytes=1 << 50, form=acc.FORM_SPLIT_BF16)
+    for c in pool:
+        probe.register(c, opts)
+    resident = int(probe.stats().resident_bytes)
+    probe.close()
+    budget = resident + SERVE_WORKSPACE_BUDGET
+    log(f"serve: {resident / 1e9:.2f} GB resident, budget {budget / 1e9:.2f} GB")
+
     out = {}
     for mode in ("one_per_launch", "batched"):
-        srv = acc.Server(hbm_budget_bytes=200 << 30, max_batch=64, max_inflight=3,
-                         form=acc.FORM_SPLIT_BF16, one_per_launch=(mode == "one_per_launch"))
-        gids = [srv.register(c, opts) for c in pool]
+        srv, gids = server(mode)
         run(srv, 0)  # warm-up: schedules, workspaces
         wall, lats = run(srv, 0)
         st = srv.stats()
         out[mode] = {"saturation_req_per_s": requests / wall,
                      "saturation_gflops": flops.sum() / wall / 1e9,
-                     "launches": int(st.launches)}
+                     "launches": int(st.launches),
+                     "peak_inflight_bytes": int(st.peak_inflight_bytes)}
         srv.close()
         log(f"serve: {mode}: {requests / wall:.0f} req/s, {flops.sum() / wall / 1e9:.1f} GFLOP/s "
-            f"at saturation")
+            f"at saturation, peak in-flight workspaces {st.peak_inflight_bytes / 1e9:.2f} GB")
     qps = 0.7 * out["one_per_launch"]["saturation_req_per_s"]
     for mode in ("one_per_launch", "batched"):
-        srv = acc.Server(hbm_budget_bytes=200 << 30, max_batch=64, max_inflight=3,
-                         form=acc.FORM_SPLIT_BF16, one_per_launch=(mode == "one_per_launch"))
-        gids = [srv.register(c, opts) for c in pool]
+        srv, gids = server(mode)
         run(srv, 0)
         wall, lats = run(srv, qps)
         srv.close()
@@ -509,6 +583,28 @@ def serve_bench(requests: int, seed: int = 4000, big_nodes: int = 2_000_000,
             f"p99 {out[mode]['p99_ms']:.3f} ms")
     out["throughput_ratio"] = (out["batched"]["saturation_req_per_s"] /
                                out["one_per_launch"]["saturation_req_per_s"])
+    # per class: microseconds of saturated server time per request
+    per = {}
+    for mode in ("one_per_launch", "batched"):
+        srv, gids = server(mode)
+        run(srv, 0)
+        for cls, sel in (("big", is_big), ("small", ~is_big)):
+            wall, _ = run(srv, 0, sel)
+            per.setdefault(cls, {})[mode + "_us"] = round(wall / max(1, int(sel.sum())) * 1e6, 2)
+        srv.close()
+    share = float(is_big.mean())
+    t_opl = share * per["big"]["one_per_launch_us"] + (1 - share) * per["small"]["one_per_launch_us"]
+    t_bat = share * per["big"]["batched_us"] + (1 - share) * per["small"]["batched_us"]
+    out["breakdown"] = {
+        "big_share_of_requests": round(share, 4), "per_request": per,
+        "modelled_ratio": round(t_opl / t_bat, 3),
+        # even with free small requests the ratio stays below this: the
+        # 2M-node graphs' kernel time is the same in both modes
+        "ratio_cap_small_free": round(t_opl / (share * per["big"]["batched_us"]), 3)}
+    log(f"serve: per request (us): {per}; ratio model {t_opl / t_bat:.2f}, cap with free "
+        f"small requests {out['breakdown']['ratio_cap_small_free']:.2f}")
+    out["hbm_budget_bytes"] = budget
+    out["resident_bytes"] = resident
     out["requests"] = requests
     out["row_window"] = row_window
     out["reorder_rows"] = reorder
@@ -551,6 +647,10 @@ def main():
                     help="also time the 100^3 stencil at K (1) or not (0)")
     ap.add_argument("--batch-tune", type=int, default=1,
                     help="run the batch autotuner in the batch leg (1) or use library tiles (0)")
+    ap.add_argument("--ordering", default="",
+                    help="only compare the locality orderings (input, rows by length, "
+                         "degree symmetric, reverse Cuthill-McKee) on these comma-separated "
+                         "workloads for the head form, and print their JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -578,6 +678,13 @@ def main():
 
     if args.ingest:
         print(json.dumps(ingest_bench(args.workload)), flush=True)
+        return
+
+    if args.ordering:
+        head = [f for f in args.forms.split(",") if f][0]
+        res = [ordering_bench(w, args.K, head, args.steps, args.warmup)
+               for w in args.ordering.split(",") if w]
+        print(json.dumps({"ordering": res}), flush=True)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -627,7 +734,8 @@ def main():
 
     if args.profile_leg:
         form = name_to_form[forms[0]]
-        tile = g.tile_select(K, form)
+        tile = (acc.Tile(*[int(x) for x in args.tile.split(",")]) if args.tile
+                else g.tile_select(K, form))
         time_form(g, B, Cd, form, tile, 1, 1)
         log(f"bench: profile leg {forms[0]} tile={tile.as_tuple()}")
         return

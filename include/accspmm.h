@@ -307,11 +307,13 @@ accspmm_status accspmm_autotune(const accspmm_metcf* h, const float* B,
                                 const char* cache_path, accspmm_tile* best,
                                 float* best_ms, void* stream);
 /* Layout autotuner: builds the DEVICE CSR in each layout (row window 8 and
- * 16, rows in input order and reordered by length), times the SpMM of each
- * (library tile, median of 5 launches) on (B, C, K, form) on `stream`, and
- * returns the fastest layout as build options (the other fields are the
- * defaults) with its time.  Costs four builds; no reference counterpart (the
- * reference fixes ROW_WINDOW at compile time, include/common.hpp:46). */
+ * 16; rows in input order, reordered by length, and for a square matrix
+ * the reverse Cuthill-McKee symmetric order), times the SpMM of each
+ * (library tile, B conversion included, median of 5 launches) on (B, C, K,
+ * form) on `stream`, and returns the fastest layout as build options (the
+ * other fields are the defaults) with its time.  Costs four builds (six
+ * for a square matrix); no reference counterpart (the reference fixes
+ * ROW_WINDOW at compile time, include/common.hpp:46). */
 accspmm_status accspmm_autotune_layout(const accspmm_csr_view* csr, const float* B,
                                        uint32_t ldb, float* C, uint32_t ldc, uint32_t K,
                                        accspmm_form form, accspmm_build_opts* best,

@@ -165,14 +165,19 @@ def test_automatic_choices():
 
 @pytest.mark.parametrize("form", [acc.FORM_SPLIT_BF16, acc.FORM_FP32])
 def test_layout_autotuner(form):
-    # the returned layout is one of the four, and a graph built with it
-    # computes C within the form's tolerance; bad input reports its status
+    # the returned layout is one of the candidates (row window 8/16 x input
+    # rows, rows by length, or reverse Cuthill-McKee for square matrices),
+    # and a graph built with it computes C within the form's tolerance; bad
+    # input reports its status
     torch = _torch()
     for name, csr in _graphs().items():
         B = torch.from_numpy(acc.gen_dense(8, csr.cols, 64)).cuda()
         Cs = torch.empty((csr.rows, 64), device="cuda")
         opts, ms = acc.autotune_layout(csr, B, Cs, form)
         assert opts.row_window in (8, 16) and opts.reorder_rows in (0, 1) and ms > 0
+        assert opts.symmetric_order in (0, 1)
+        assert not (opts.symmetric_order and opts.reorder_rows)
+        assert not opts.symmetric_order or csr.rows == csr.cols
         g = acc.DeviceGraph(csr, opts)
         Cg = g.spmm(B, None, form).cpu().numpy()
         Bh = B.cpu().numpy()
