@@ -29,7 +29,7 @@ __all__ = [
     "lib", "Status", "AccSpMMError", "FORM_FP32", "FORM_SPLIT_BF16", "FORM_BF16",
     "BuildOpts", "Tile", "CSR", "read_mtx", "canonicalize_device", "autotune_layout", "METCFBit", "AdpBME",
     "balanceStrategy", "DeviceGraph", "Batch", "Server", "gen_chung_lu", "gen_stencil27",
-    "gen_dense", "MetcfInfo", "order_rcm",
+    "gen_dense", "MetcfInfo", "order_rcm", "gather_ceiling",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -142,6 +142,8 @@ _SIGS = {
     "accspmm_metcf_row_order": (C.c_int, [C.c_void_p, C.c_void_p]),
     "accspmm_metcf_col_order": (C.c_int, [C.c_void_p, C.c_void_p]),
     "accspmm_order_rcm": (C.c_int, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "accspmm_gather_probe": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                       C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "accspmm_autotune_layout": (C.c_int, [C.POINTER(CsrView), C.c_void_p, C.c_uint32,
                                           C.c_void_p, C.c_uint32, C.c_uint32, C.c_int,
                                           C.POINTER(BuildOpts), C.POINTER(C.c_float),
@@ -390,6 +392,36 @@ def order_rcm(row_ptr, col_idx, n: int | None = None) -> np.ndarray:
     _check(lib().accspmm_order_rcm(n, _ptr(rp), _ptr(ci) if len(ci) else 0, _ptr(perm)),
            "order_rcm")
     return perm
+
+
+def gather_ceiling(region_rows: int, row_bytes: int = 512, waves: int = 4096,
+                   rows_per_wave: int = 2048, reps: int = 5, seed: int = 7) -> float:
+    """GB/s at which the GPU serves random row gathers from a region of
+    region_rows rows of row_bytes bytes in the SpMM kernels' load pattern
+    (accspmm_gather_probe): median of `reps` timed launches after two
+    warm-up launches, uniform random row indices."""
+    import torch
+    region = torch.zeros(region_rows * row_bytes // 4, dtype=torch.float32, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    idx = torch.randint(0, region_rows, (waves * rows_per_wave,), dtype=torch.int32,
+                        device="cuda", generator=g)
+    out = torch.empty(waves * 64, dtype=torch.int32, device="cuda")
+    s = torch.cuda.current_stream()
+    ms = []
+    for i in range(reps + 2):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        _check(lib().accspmm_gather_probe(region.data_ptr(), region_rows, row_bytes,
+                                          idx.data_ptr(), waves, rows_per_wave,
+                                          out.data_ptr(), s.cuda_stream), "gather_probe")
+        e1.record(s)
+        e1.synchronize()
+        if i >= 2:
+            ms.append(e0.elapsed_time(e1))
+    ms.sort()
+    return waves * rows_per_wave * row_bytes / (ms[len(ms) // 2] * 1e-3) / 1e9
 
 
 def autotune_layout(csr: CSR, B, C_out, form: int = FORM_FP32, stream=None):

@@ -32,7 +32,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]
 
 HIP_SOURCES = ["scan.hip", "canon.hip", "builder.hip", "spmm.hip", "sweep.hip", "batch.hip",
-               "server.hip", "noalloc.hip"]
+               "server.hip", "noalloc.hip", "probe.hip"]
 HOST_HIP_SOURCES = ["tune.cpp", "serialize.cpp"]  # host code using the HIP runtime
 HOST_SOURCES = ["host.cpp"]
 

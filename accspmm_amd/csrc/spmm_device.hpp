@@ -192,10 +192,11 @@ __device__ inline uint32_t lane_feat(uint32_t k0, uint32_t n) {
 
 // Where a lane's B row segments come from.  BUF: buffer loads through rsrc
 // (num_records = b_rows * row_bytes) at 32-bit offsets c * row_bytes +
-// feature bytes (one v_mad_u32_u24): a padded column (kPad) is clamped to
-// row b_rows, just past the end, and the hardware returns zeros for it, so
-// no mask and no zero row are needed.  Used when every offset fits 32 bits
-// (buffer_ok).  Otherwise 64-bit addresses from B with stride ldb elements:
+// feature bytes (one v_mad_u32_u24 of the column's low 24 bits); a padded
+// column (kPad, low 24 bits 0xFFFFFF) lands past the end of the buffer and
+// the hardware returns zeros for it, so no clamp, mask or zero row is
+// needed.  Used when every offset fits 32 bits and that padded product
+// does fall past the end (buffer_ok).  Otherwise 64-bit addresses from B with stride ldb elements:
 // padded columns read the converted B's zero row (BF16 form) or row 0 of
 // the caller's B (masked when consumed if that row is not finite).
 struct BSrc {
@@ -207,12 +208,17 @@ struct BSrc {
   uint32_t zero_row;   // 64-bit path, BF16 form: the all-zero row
 };
 
-// Whether every B offset of the BUF path fits its 32-bit arithmetic: rows
-// and row bytes within v_mad_u32_u24's 24-bit operands, and the clamped
-// row's segment below 2^32.
+// Whether every B offset of the BUF path fits its 32-bit arithmetic (rows
+// and row bytes within v_mad_u32_u24's 24-bit operands, every row's segment
+// below 2^32) and a padded column's offset, the low 32 bits of 0xFFFFFF *
+// row_bytes, lies past the buffer's b_rows * row_bytes bytes with its whole
+// row segment below 2^32 (so its loads return zeros without a clamp).
 __host__ __device__ inline bool buffer_ok(uint64_t b_rows, uint64_t row_bytes) {
-  return b_rows < (1u << 24) && row_bytes < (1u << 24) &&
-         (b_rows + 1) * row_bytes <= 0xFFFFFF00ull;
+  if (!(b_rows < (1u << 24) && row_bytes < (1u << 24) &&
+        (b_rows + 1) * row_bytes <= 0xFFFFFF00ull))
+    return false;
+  const uint64_t pad = (0xFFFFFFull * row_bytes) & 0xFFFFFFFFull;
+  return pad >= b_rows * row_bytes && pad + row_bytes <= 0x100000000ull;
 }
 
 template <int FORM, int NB, bool BUF>
@@ -240,7 +246,7 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const BSrc& src, uint32_t co
                                  uint32_t feat, uint32_t K) {
   constexpr uint32_t es = FORM == kBF16 ? 2u : 4u;
   if constexpr (BUF) {
-    const uint32_t off = __umul24(min(col, src.b_rows), src.row_bytes) + feat * es;
+    const uint32_t off = __umul24(col, src.row_bytes) + feat * es;
     if constexpr (FORM == kBF16) {
       if constexpr (NB == 8) {
         const u4 v = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(src.rsrc, off, 0, 0));
@@ -255,11 +261,13 @@ __device__ inline void load_brow(BRow<FORM, NB>& r, const BSrc& src, uint32_t co
       }
     } else if constexpr (FORM == kSPLIT || FULL) {
       if constexpr (NB >= 4) {
+        // the second 16 bytes of NB = 8 (64 features on) through the
+        // scalar offset: no per-row address add
 #pragma unroll
         for (int i = 0; i < NB / 4; ++i) {
           const u4 v = __builtin_bit_cast(
-              u4, __builtin_amdgcn_raw_buffer_load_b128(src.rsrc, off + 4u * word_feat<FORM, NB>(4 * i),
-                                                        0, 0));
+              u4, __builtin_amdgcn_raw_buffer_load_b128(src.rsrc, off,
+                                                        4u * word_feat<FORM, NB>(4 * i), 0));
           r.w[4 * i] = v[0]; r.w[4 * i + 1] = v[1];
           r.w[4 * i + 2] = v[2]; r.w[4 * i + 3] = v[3];
         }

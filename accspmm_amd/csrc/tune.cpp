@@ -208,6 +208,13 @@ extern "C" accspmm_status accspmm_autotune(const accspmm_metcf* h,
           c.split_tiles = sp;
           consider(c);
         }
+      // panels pinned to XCDs (each B row cached by one L2)
+      for (uint32_t p : {8u, 16u}) {
+        accspmm_tile c = base;
+        c.col_panels = p;
+        c.xcd_slice = 3;
+        consider(c);
+      }
     }
   }
   // stage 3 (split-BF16): both ways of splitting B, around the best shape

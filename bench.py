@@ -879,6 +879,20 @@ def main():
     head_form = name_to_form[head]
     ab = alg_bytes(csr.rows, csr.cols, csr.nnz, K, head_form)
     achieved = ab / (hr["kernel_ms_mean"] * 1e-3) / 1e9
+    # the measured ceiling of the kernels' B access pattern: random rows of
+    # the head form's gathered row size from an L2-sized region and from a
+    # region the size of the gathered B
+    gather = None
+    row_bytes = K * (2 if head_form == acc.FORM_BF16 else 4)
+    if row_bytes in (256, 512):
+        gather = {"row_bytes": row_bytes,
+                  "l2_region_bytes": 4096 * row_bytes,
+                  "l2_region_gbs": round(acc.gather_ceiling(4096, row_bytes), 1),
+                  "b_region_bytes": (csr.cols + 1) * row_bytes,
+                  "b_region_gbs": round(acc.gather_ceiling(csr.cols + 1, row_bytes), 1)}
+        log(f"bench: gather ceiling ({row_bytes}-byte rows): {gather['l2_region_gbs']:.0f} GB/s "
+            f"from L2, {gather['b_region_gbs']:.0f} GB/s from a {gather['b_region_bytes'] / 1e6:.0f}"
+            f" MB region")
     # the profiled traffic counts only for the kernels it was measured on:
     # the profile carries the digest of their sources, and the tile
     traffic, traffic_note = None, "no profile"
@@ -889,6 +903,16 @@ def main():
         if prof.get("source_stamp") == kernel_source_stamp():
             traffic = prof.get("hbm_bytes_per_launch")
             traffic_note = f"{tfile.relative_to(REPO)} (source stamp {prof['source_stamp']})"
+            # the SpMM kernel's reads from beyond L2 against the gather ceiling
+            ff = prof.get("calibration", {}).get("fetch_factor", 1.0)
+            fetch = sum(v["FETCH_SIZE"] * 1024.0 * ff
+                        for k, v in prof.get("counters_kib_per_dispatch", {}).items()
+                        if k.startswith("spmm_kernel") and "FETCH_SIZE" in v)
+            if gather and fetch:
+                miss_gbs = fetch / (hr["kernel_ms_mean"] * 1e-3) / 1e9
+                gather["kernel_miss_read_bytes"] = round(fetch)
+                gather["kernel_miss_read_gbs"] = round(miss_gbs, 1)
+                gather["kernel_miss_frac_of_b_region"] = round(miss_gbs / gather["b_region_gbs"], 4)
         else:
             traffic_note = (f"stale: {tfile.relative_to(REPO)} was measured at source stamp "
                             f"{prof.get('source_stamp')}, this build is {kernel_source_stamp()}")
@@ -925,7 +949,8 @@ def main():
                      "traffic_frac": (round(traffic / (hr["kernel_ms_mean"] * 1e-3) / 1e9 /
                                             HBM_PEAK_GBS, 4) if traffic else None),
                      "kernel_ms": round(hr["kernel_ms_mean"], 4),
-                     "alg_bytes_per_launch": ab},
+                     "alg_bytes_per_launch": ab,
+                     "gather_ceiling": gather},
         "diagnostics": diag,
         "reference_tf32_precision": ref_tf32,
         "cpu_baseline": cpu,

@@ -423,6 +423,19 @@ accspmm_status accspmm_gen_stencil27(uint32_t n, uint64_t seed,
 /* Dense row-major rows x cols, U[-1,1]. */
 void accspmm_gen_dense(uint64_t seed, uint64_t count, float* out);
 
+/* Gather ceiling (a measurement of the memory system, no reference
+ * counterpart): `waves` waves (a multiple of 4) each read rows_per_wave (a
+ * multiple of 64) rows idx[w * rows_per_wave + i] of the DEVICE region of
+ * region_rows rows of row_bytes (256 or 512) bytes, in the SpMM kernels'
+ * pattern (a quarter-wave per row, 16 bytes a lane, buffer loads), and
+ * write one folded word per lane to out (waves * 64 words).  An index past
+ * the region reads zeros (the buffer loads' range check).  Asynchronous on
+ * `stream`; time it with events. */
+accspmm_status accspmm_gather_probe(const void* region, uint32_t region_rows,
+                                    uint32_t row_bytes, const uint32_t* idx,
+                                    uint32_t waves, uint32_t rows_per_wave,
+                                    uint32_t* out, void* stream);
+
 /* Reverse Cuthill-McKee order of the n x n pattern (HOST arrays; row i's
  * columns are its neighbours): perm[i] = the node placed at position i.
  * Components are ordered from pseudo-peripheral start nodes, breadth
