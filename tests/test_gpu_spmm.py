@@ -96,6 +96,32 @@ def test_column_panels(form, panels, split):
 
 
 @pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+@pytest.mark.parametrize("panels,split,nb,wpb", [(8, 0, 2, 4), (16, 20, 4, 1), (3, 0, 8, 1),
+                                                 (300, 12, 2, 2)])
+def test_panels_pinned_to_xcds(form, panels, split, nb, wpb):
+    # xcd_slice 3 runs panel p on XCD p % 8 (per-XCD unit lists padded with
+    # no-op units): the same pieces and summation order as round-robin
+    # dispatch of that schedule, so C is bit-equal to it and within the
+    # form's tolerance of FP64; with one panel the mode falls back to 0
+    rng = np.random.default_rng(panels * 7 + nb)
+    rows, cols = 603, 5000
+    rp, ci, va = random_csr(rng, rows, cols, 0.05, empty_rows=0.05, heavy_rows=3)
+    csr = acc.CSR(rows, cols, rp, ci, va)
+    for rw in (8, 16):
+        g = acc.DeviceGraph(csr, acc.default_build_opts(rw))
+        for K in (32, 100, 128):
+            pinned = acc.Tile(nb, wpb, split, 3, 64, panels)
+            plain = acc.Tile(nb, wpb, split, 0, 64, panels)
+            mx, fr, Cp = run_case(csr, K, form, tile=pinned, rw=rw, seed=K, graph=g)
+            assert mx <= TOL[form], (rw, K, mx, fr)
+            _, _, Cr = run_case(csr, K, form, tile=plain, rw=rw, seed=K, graph=g)
+            assert np.array_equal(Cp, Cr), (rw, K)
+    one = acc.Tile(2, 4, 0, 3, 64, 1)
+    mx, _, _ = run_case(csr, 64, form, tile=one, seed=1)
+    assert mx <= TOL[form]
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
 @pytest.mark.parametrize("case", ["long_windows", "many_windows"])
 def test_sweep_equals_work_units(form, case):
     # The resident-window sweep runs the work-unit schedule's pieces in
