@@ -477,10 +477,11 @@ accspmm_status build_impl(const accspmm_csr_view* csr,
   // one extra all-padding tile T (columns kPad, empty bitmap, tcOffset[T] =
   // nnz): the SpMM kernels read it for group slots past a range's last tile
   ACC_TRY(h->a2b.alloc(((uint64_t)T + 1) * kColWindow * 4));
-  ACC_TRY(h->bits.alloc(((uint64_t)T + 1) * words * 8));
-  // one spare entry: the kernels read tcOffset[t..t+1] as a pair for every
-  // tile t <= T (T is the all-padding tile)
-  ACC_TRY(h->tco.alloc(((uint64_t)T + 2) * 4));
+  // a spare empty tile and three spare tcOffset entries (= nnz): the
+  // kernels read 16 bytes of bitmaps (8-row windows: tiles t and t+1) and
+  // of tcOffset (t..t+3) for every tile t <= T (T is the all-padding tile)
+  ACC_TRY(h->bits.alloc(((uint64_t)T + 2) * words * 8));
+  ACC_TRY(h->tco.alloc(((uint64_t)T + 4) * 4));
   // kValuePad zeros past the values (not part of the exported array): the
   // SpMM kernels load a 4-tile group's values as one 256-float span that
   // may run past the last value
@@ -488,7 +489,7 @@ accspmm_status build_impl(const accspmm_csr_view* csr,
   ACC_HIP(hipMemsetAsync(static_cast<char*>(h->data.p) + (uint64_t)nnz * 4, 0,
                          (uint64_t)kValuePad * 4, s));
   ACC_HIP(hipMemsetAsync(h->a2b.p, 0xFF, ((uint64_t)T + 1) * kColWindow * 4, s));
-  ACC_HIP(hipMemsetAsync(h->bits.p, 0, ((uint64_t)T + 1) * words * 8, s));
+  ACC_HIP(hipMemsetAsync(h->bits.p, 0, ((uint64_t)T + 2) * words * 8, s));
   DevBuf tile_of;
   ACC_TRY(tile_of.alloc((uint64_t)nnz * 4));
   if (nnz) {
@@ -512,8 +513,9 @@ accspmm_status build_impl(const accspmm_csr_view* csr,
   }
   ACC_TRY(exclusive_scan_u32(cnt.as<uint32_t>(), h->tco.as<uint32_t>(),
                              (uint64_t)T + 1, s));
-  ACC_HIP(hipMemcpyAsync(h->tco.as<uint32_t>() + T + 1, h->tco.as<uint32_t>() + T, 4,
-                         hipMemcpyDeviceToDevice, s));
+  for (uint32_t k = 1; k <= 3; ++k)
+    ACC_HIP(hipMemcpyAsync(h->tco.as<uint32_t>() + T + k, h->tco.as<uint32_t>() + T, 4,
+                           hipMemcpyDeviceToDevice, s));
   // 8. values
   if (nnz) {
     hipLaunchKernelGGL(k_values, dim3(blocks_for(nnz)), dim3(kThreads), 0, s,

@@ -236,15 +236,16 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
                                adp, sym ? 0u : reorder, sym};
   h->hist_key = key;
   h->h_rwo = rwo;
-  // device arrays with the kernels' padding: tcOffset[T+1] repeats nnz, the
-  // all-padding tile T (kPad columns, empty bitmap) and zeros after the values
+  // device arrays with the kernels' padding: tcOffset[T+1..T+3] repeat
+  // nnz, the all-padding tile T (kPad columns, empty bitmap), a spare empty
+  // bitmap and zeros after the values
   std::vector<uint32_t> tco_pad = tco;
-  tco_pad.push_back(nnz);
+  tco_pad.insert(tco_pad.end(), 3, nnz);
   std::vector<uint32_t> a2b_pad = a2b;
   a2b_pad.resize((size_t)(T + 1) * kColWindow, kPad);
   ACC_TRY(to_device(h->rwo, rwo, rwo.size()));
   ACC_TRY(to_device(h->tco, tco_pad, tco_pad.size()));
-  ACC_TRY(to_device(h->bits, bits, (size_t)(T + 1) * words));
+  ACC_TRY(to_device(h->bits, bits, (size_t)(T + 2) * words));
   ACC_TRY(to_device(h->a2b, a2b_pad, a2b_pad.size()));
   ACC_TRY(to_device(h->data, data, (size_t)nnz + kValuePad));
   if (adp) {

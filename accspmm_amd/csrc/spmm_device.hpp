@@ -365,17 +365,16 @@ struct Stage {
   uint32_t vbase; // RW=16: index of the group's first value (wave-uniform)
 };
 
-// Tile metadata as loaded: lane n of a quarter holds columns 4(n&1)..+3 of
-// its tile, and its even/odd lane pair one 8-byte word: the bitmap (even
-// lane) or tcOffset[t], tcOffset[t+1] (odd lane).  RW=16 tiles load both
-// bitmap words and tcOffset[t] in every lane.
+// Tile metadata as loaded: one 16-byte load per lane, the quarter's tile
+// spread over each group of four lanes (r = lane & 3): r = 0 its columns
+// 0-3, r = 1 columns 4-7 (sparseA2B), r = 2 its bitmap words (RW=8: the
+// tile's word and the next tile's; RW=16: both words), r = 3
+// tcOffset[t..t+3].  One vector-memory instruction per group for all of
+// it: the texture-address unit, which bounds this kernel, costs the same per
+// instruction whatever each lane reads.
 struct Meta {
-  u4 half;
-  uint32_t x, y;       // RW=8: bitmap lo/hi (even) or tcOffset pair (odd)
-  uint32_t w0, w1;     // RW=16: bitmap word 0 (rows 0-7), lo/hi
-  uint32_t w2, w3;     // RW=16: bitmap word 1 (rows 8-15), lo/hi
-  uint32_t off16;      // RW=16: tcOffset[t]
-  uint32_t win;        // tag of the group (wave-uniform)
+  u4 m;
+  uint32_t win;   // tag of the group (wave-uniform)
 };
 
 // Position of a 4-tile group (wave-uniform): its window w, first tile t and
@@ -400,38 +399,49 @@ __device__ inline void next_group(GroupPos& p, const uint32_t* __restrict__ rwo)
 typedef uint32_t u4a __attribute__((ext_vector_type(4), aligned(4)));
 typedef uint32_t u2a __attribute__((ext_vector_type(2), aligned(4)));
 
-// The value the even (quad permutation [0,0,2,2]) or odd ([1,1,3,3]) lane
-// of this lane's pair holds.
-__device__ inline uint32_t from_even(uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xA0, 0xF, 0xF, false);
-}
-__device__ inline uint32_t from_odd(uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xF5, 0xF, 0xF, false);
+// Where lane r = lane & 3 of a quad loads its piece of a tile's metadata:
+// tile tt's bytes base + tt * stride.  The builder pads tcOffset with
+// three spare entries and the bitmaps with a spare tile so every such
+// 16-byte read stays inside its array.
+struct MetaSrc {
+  const char* base;
+  uint32_t stride;
+};
+
+template <int RW>
+__device__ inline MetaSrc meta_src(const SpmmArgs& g, uint32_t lane) {
+  const uint32_t r = lane & 3u;
+  MetaSrc s;
+  if (r < 2) {
+    s.base = reinterpret_cast<const char*>(g.a2b) + 16u * r;
+    s.stride = 32;
+  } else if (r == 2) {
+    s.base = reinterpret_cast<const char*>(g.bits);
+    s.stride = RW == 16 ? 16 : 8;
+  } else {
+    s.base = reinterpret_cast<const char*>(g.tco);
+    s.stride = 4;
+  }
+  return s;
 }
 
 template <int RW>
 __device__ inline void load_meta(Meta& mt, const SpmmArgs& g, const GroupPos& p,
-                                 bool valid, uint32_t q, uint32_t n) {
+                                 bool valid, uint32_t q, const MetaSrc& src) {
   // a tile past the group's range (or an invalid group) reads the
   // all-padding tile: kPad columns, empty bitmap, offset nnz
   const uint32_t t = p.t + q;
   const bool tile_ok = valid && t < p.tend;
   const uint32_t tt = tile_ok ? t : g.pad_tile;
-  mt.half = *reinterpret_cast<const u4*>(g.a2b + (uint64_t)tt * 8 + 4 * (n & 1u));
-  if (RW == 8) {
-    // even lanes: bitmap word; odd lanes: tcOffset[t], tcOffset[t+1]
-    // (integer address select, one load instruction)
-    const uint64_t ab = reinterpret_cast<uint64_t>(g.bits + tt);
-    const uint64_t ao = reinterpret_cast<uint64_t>(g.tco + tt);
-    const u2a v = *reinterpret_cast<const u2a*>((n & 1u) ? ao : ab);
-    mt.x = v.x;
-    mt.y = v.y;
-  } else {
-    const u4a w = *reinterpret_cast<const u4a*>(g.bits + 2ull * tt);
-    mt.w0 = w.x; mt.w1 = w.y; mt.w2 = w.z; mt.w3 = w.w;
-    mt.off16 = g.tco[tt];
-  }
+  const u4a v = *reinterpret_cast<const u4a*>(src.base + (uint64_t)tt * src.stride);
+  mt.m = u4{v.x, v.y, v.z, v.w};
   mt.win = p.w;
+}
+
+// The value lane k (0-3) of this lane's quad holds (DPP quad broadcast).
+template <int K>
+__device__ inline uint32_t from_quad(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, K * 0x55, 0xF, 0xF, false);
 }
 
 // Gathers the B rows of the tile's 8 columns and the lane's share of the
@@ -440,24 +450,28 @@ template <int FORM, int NB, bool FULL, int RW, bool BUF>
 __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
                                  const SpmmArgs& g, uint32_t n, bool a_rows,
                                  uint32_t feat, const BSrc& src, uint32_t lane) {
-  // columns 0-3 come from the even lane of the pair, 4-7 from the odd one;
-  // the bitmap from the even lane, tcOffset[t] from the odd one
+  // columns 0-3 from quad lane 0, 4-7 from lane 1, the bitmap from lane 2,
+  // tcOffset[t] from lane 3
   uint32_t col[8];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    col[j] = from_even(mt.half[j]);
-    col[4 + j] = from_odd(mt.half[j]);
+    col[j] = from_quad<0>(mt.m[j]);
+    col[4 + j] = from_quad<1>(mt.m[j]);
   }
   unsigned long long bw;
   uint32_t off;
   if (RW == 8) {
-    bw = (unsigned long long)from_even(mt.y) << 32 | from_even(mt.x);
-    off = from_odd(mt.x);
+    bw = (unsigned long long)from_quad<2>(mt.m[1]) << 32 | from_quad<2>(mt.m[0]);
+    off = from_quad<3>(mt.m[0]);
   } else {
-    const unsigned long long w0 = (unsigned long long)mt.w1 << 32 | mt.w0;
-    const unsigned long long w1 = (unsigned long long)mt.w3 << 32 | mt.w2;
-    bw = n < 8 ? w0 : w1;
-    off = mt.off16 + (n < 8 ? 0u : popc64(w0));
+    // rows 0-7 (lanes n < 8) take word 0 and start at tcOffset[t]; rows
+    // 8-15 word 1, after word 0's values.  Every lane of a quad has the
+    // same n < 8, so the bitmap lane picks the word before the broadcast.
+    const bool low = n < 8;
+    const uint32_t lo = low ? mt.m[0] : mt.m[2], hi = low ? mt.m[1] : mt.m[3];
+    const uint32_t skip = low ? 0u : (uint32_t)__popc(mt.m[0]) + (uint32_t)__popc(mt.m[1]);
+    bw = (unsigned long long)from_quad<2>(hi) << 32 | from_quad<2>(lo);
+    off = from_quad<3>(mt.m[0]) + from_quad<2>(skip);
   }
   uint32_t bm = 0;
 #pragma unroll
@@ -470,12 +484,13 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
   const uint32_t vbase = __builtin_amdgcn_readfirstlane(off);
   const uint32_t* vsrc = reinterpret_cast<const uint32_t*>(g.data);
   // RW=8: lanes past the group's values (it ends at tcOffset[t3 + 1],
-  // which the odd lane 49 of quarter 3 holds; a padding tile there gives
-  // nnz, clamped to the span) re-read the span's first 16 bytes, so only
-  // the cache lines holding values are requested.  Every lane still issues
-  // the load: the wait counts of the pipeline stay static.
+  // which lane 51 of quarter 3, its tcOffset lane, holds; a padding tile
+  // there gives nnz, clamped to the span) re-read the span's first 16
+  // bytes, so only the cache lines holding values are requested.  Every
+  // lane still issues the load: the wait counts of the pipeline stay
+  // static.
   uint32_t nv = kSpanValues;
-  if (RW == 8) nv = min(__builtin_amdgcn_readlane(mt.y, 49) - vbase, kSpanValues);
+  if (RW == 8) nv = min(__builtin_amdgcn_readlane(mt.m[1], 51) - vbase, kSpanValues);
   const uint32_t vi = 4u * lane < nv ? 4u * lane : 0u;
   const u4a v = *reinterpret_cast<const u4a*>(vsrc + vbase + vi);
   st.vals = u4{v.x, v.y, v.z, v.w};
@@ -742,26 +757,27 @@ __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
   Meta mA, mB;
   Stage<FORM, NB> dA, dB;
   const uint32_t* vsrc = reinterpret_cast<const uint32_t*>(g.data);
+  const MetaSrc ms = meta_src<RW>(g, lane);
   auto consume = [&](Stage<FORM, NB>& d) {
     if (PACKED && out.direct) out.move_to(d.win);
     compute_stage<FORM, NB, RW, MASK>(d, out.acc, m, lv, lane, vsrc, g.wide != 0u);
   };
   // pos: position of the most recently loaded metadata
-  load_meta<RW>(mA, g, pos, true, q, m);
+  load_meta<RW>(mA, g, pos, true, q, ms);
   if (ngroups > 1) next_group<PACKED>(pos, g.rwo);
-  load_meta<RW>(mB, g, pos, ngroups > 1, q, m);
+  load_meta<RW>(mB, g, pos, ngroups > 1, q, ms);
   load_data<FORM, NB, FULL, RW, BUF>(dA, mA, g, m, a_rows, feat, src, lane);
   uint32_t gi = 0;
   for (; gi + 3 < ngroups; gi += 2) {
     next_group<PACKED>(pos, g.rwo);  // group gi + 2
-    load_meta<RW>(mA, g, pos, true, q, m);
+    load_meta<RW>(mA, g, pos, true, q, ms);
     load_data<FORM, NB, FULL, RW, BUF>(dB, mB, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
     __builtin_amdgcn_sched_barrier(0);
     const bool v3 = gi + 3 < ngroups;
     if (v3) next_group<PACKED>(pos, g.rwo);  // group gi + 3
-    load_meta<RW>(mB, g, pos, v3, q, m);
+    load_meta<RW>(mB, g, pos, v3, q, ms);
     load_data<FORM, NB, FULL, RW, BUF>(dA, mA, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dB);
@@ -771,7 +787,7 @@ __device__ inline void unit_loop(const SpmmArgs& g, Accum<FORM, NB, RW>& out,
   const uint32_t left = ngroups - gi;  // 1, 2 or 3
   if (left == 3) {
     next_group<PACKED>(pos, g.rwo);  // group gi + 2
-    load_meta<RW>(mA, g, pos, true, q, m);
+    load_meta<RW>(mA, g, pos, true, q, ms);
     load_data<FORM, NB, FULL, RW, BUF>(dB, mB, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
@@ -804,6 +820,30 @@ __device__ inline bool row0_needs_mask(const void* B, uint32_t feat, uint32_t K)
     if (feat + word_feat<FORM, NB>(i) < K)
       bad |= !__builtin_isfinite(r0[feat + word_feat<FORM, NB>(i)]);
   return __builtin_amdgcn_ballot_w64(bad) != 0;
+}
+
+// Writes a split piece's sums to its partial slot (RW rows x kc_padded
+// floats; the split form with RW=8 first adds rows r and r+8).
+template <int FORM, int NB, int RW>
+__device__ inline void store_partial(f4* acc, uint32_t slot, uint32_t q, uint32_t feat,
+                                     const SpmmArgs& g) {
+  if (is_split(FORM) && RW == 8) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[b][i] += __shfl_xor(acc[b][i], 32, 64);
+  }
+  const bool holds_rows = RW == 16 || q < 2;
+  float* part = g.partials + (uint64_t)slot * RW * g.kc_padded;
+  if (holds_rows) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float v[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) v[b] = acc[b][i];
+      store_lane<FORM, NB>(part + (uint64_t)(4 * q + i) * g.kc_padded, feat, g.kc_padded, v);
+    }
+  }
 }
 
 // PACKED: the schedule may hold multi-window units (the loop then flushes C
@@ -882,26 +922,9 @@ __device__ void run_unit(const SpmmArgs& g, const WorkUnit& u, uint32_t kc,
       zero_window<FORM, NB, RW>(e, q, feat, K, C, ldc, rows, g.row_map);
     return;
   }
-  // split piece: write partial (RW rows x kc_padded floats per slot)
-  f4* acc = out.acc;
-  if (is_split(FORM) && RW == 8) {
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc[b][i] += __shfl_xor(acc[b][i], 32, 64);
-  }
-  const bool holds_rows = RW == 16 || q < 2;
-  float* part = g.partials + (uint64_t)u.slot * RW * g.kc_padded;
-  if (holds_rows) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      float v[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) v[b] = acc[b][i];
-      store_lane<FORM, NB>(part + (uint64_t)(4 * q + i) * g.kc_padded, feat, g.kc_padded, v);
-    }
-  }
-  // the partials are combined by reduce_split_window in a second launch
+  // split piece: write partial (RW rows x kc_padded floats per slot); the
+  // partials are combined by reduce_split_window in a second launch
+  store_partial<FORM, NB, RW>(out.acc, (uint32_t)u.slot, q, feat, g);
 }
 
 // Sums the piece partials of split window `sw` in slot order and writes the

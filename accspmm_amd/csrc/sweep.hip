@@ -127,6 +127,7 @@ __device__ void sweep_loop(const SweepArgs& a, f4* acc, f4* slots, GroupPos pos,
   Meta mA, mB;
   Stage<FORM, NB> dA, dB;
   const uint32_t* vsrc = reinterpret_cast<const uint32_t*>(g.data);
+  const MetaSrc ms = meta_src<RW>(g, lane);
   auto consume = [&](Stage<FORM, NB>& d) {
     if (d.win != cur) {
       end_piece(cur);
@@ -134,21 +135,21 @@ __device__ void sweep_loop(const SweepArgs& a, f4* acc, f4* slots, GroupPos pos,
     }
     compute_stage<FORM, NB, RW, MASK>(d, acc, m, lv, lane, vsrc, g.wide != 0u);
   };
-  load_meta<RW>(mA, g, pos, true, q, m);
+  load_meta<RW>(mA, g, pos, true, q, ms);
   if (ngroups > 1) sweep_next(pos, a.pieces);
-  load_meta<RW>(mB, g, pos, ngroups > 1, q, m);
+  load_meta<RW>(mB, g, pos, ngroups > 1, q, ms);
   load_data<FORM, NB, FULL, RW, false>(dA, mA, g, m, a_rows, feat, src, lane);
   uint32_t gi = 0;
   for (; gi + 3 < ngroups; gi += 2) {
     sweep_next(pos, a.pieces);  // group gi + 2
-    load_meta<RW>(mA, g, pos, true, q, m);
+    load_meta<RW>(mA, g, pos, true, q, ms);
     load_data<FORM, NB, FULL, RW, false>(dB, mB, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
     __builtin_amdgcn_sched_barrier(0);
     const bool v3 = gi + 3 < ngroups;
     if (v3) sweep_next(pos, a.pieces);  // group gi + 3
-    load_meta<RW>(mB, g, pos, v3, q, m);
+    load_meta<RW>(mB, g, pos, v3, q, ms);
     load_data<FORM, NB, FULL, RW, false>(dA, mA, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dB);
@@ -157,7 +158,7 @@ __device__ void sweep_loop(const SweepArgs& a, f4* acc, f4* slots, GroupPos pos,
   const uint32_t left = ngroups - gi;  // 1, 2 or 3
   if (left == 3) {
     sweep_next(pos, a.pieces);
-    load_meta<RW>(mA, g, pos, true, q, m);
+    load_meta<RW>(mA, g, pos, true, q, ms);
     load_data<FORM, NB, FULL, RW, false>(dB, mB, g, m, a_rows, feat, src, lane);
     __builtin_amdgcn_sched_barrier(0);
     consume(dA);
