@@ -1018,8 +1018,9 @@ struct RcmGraph {
 };
 
 // Breadth-first level structure from r inside r's component (mark holds
-// `stamp` for visited nodes, 1 for nodes already placed): returns the number of levels and leaves the
-// last level's nodes in [last_begin, order.size()).
+// `stamp` for visited nodes, 1 for nodes already placed): returns the
+// number of levels and leaves the last level's nodes in
+// [last_begin, order.size()).
 uint32_t rcm_levels(const RcmGraph& g, uint32_t r, std::vector<uint32_t>& mark, uint32_t stamp,
                     std::vector<uint32_t>& order, size_t* last_begin) {
   order.clear();
@@ -1072,13 +1073,23 @@ extern "C" accspmm_status accspmm_order_rcm(uint32_t n, const uint32_t* row_ptr,
     std::stable_sort(by_deg.begin(), by_deg.end(),
                      [&](uint32_t a, uint32_t b) { return g.deg(a) < g.deg(b); });
     uint32_t stamp = 1;
+    // a fresh search stamp; before the counter wraps, clear every search
+    // mark (placed nodes keep 1) so an old stamp never reads as current
+    auto next_stamp = [&]() {
+      if (stamp == 0xFFFFFFFFu) {
+        for (uint32_t& m : mark)
+          if (m != 1) m = 0;
+        stamp = 1;
+      }
+      return ++stamp;
+    };
     for (uint32_t root : by_deg) {
       if (mark[root] == 1) continue;
       // pseudo-peripheral node: move to a smallest-degree node of the last
       // level while that deepens the level structure (bounded passes)
       uint32_t r = root;
       size_t last = 0;
-      uint32_t depth = rcm_levels(g, r, mark, ++stamp, order, &last);
+      uint32_t depth = rcm_levels(g, r, mark, next_stamp(), order, &last);
       for (int pass = 0; pass < 8; ++pass) {
         uint32_t best = order[last];
         for (size_t i = last; i < order.size(); ++i)
@@ -1087,7 +1098,7 @@ extern "C" accspmm_status accspmm_order_rcm(uint32_t n, const uint32_t* row_ptr,
         size_t last2 = 0;
         std::vector<uint32_t> order2;
         order2.reserve(order.size());
-        const uint32_t d2 = rcm_levels(g, best, mark, ++stamp, order2, &last2);
+        const uint32_t d2 = rcm_levels(g, best, mark, next_stamp(), order2, &last2);
         if (d2 <= depth) break;
         r = best;
         depth = d2;
