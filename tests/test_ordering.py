@@ -130,3 +130,20 @@ def test_deterministic_and_bad_input():
         acc.order_rcm(rp, np.array([0, 5], np.uint32))      # column >= n
     with pytest.raises(acc.AccSpMMError):
         acc.order_rcm(np.array([0, 2, 1], np.uint32), np.array([0, 1], np.uint32))
+
+
+def test_unsymmetric_pattern_is_a_permutation():
+    # entries (i, j) without (j, i) reach nodes of components already
+    # placed; the order must still hold every node exactly once
+    rng = np.random.default_rng(21)
+    for n, p in ((50, 0.05), (400, 0.006), (2000, 0.001)):
+        m = sp.random(n, n, density=p, random_state=rng, format="csr")
+        m = (m > 0).astype(np.float32).tocsr()
+        m.sort_indices()
+        perm = _rcm(m)
+        assert sorted(perm.tolist()) == list(range(n))
+    # a chain 0 -> 1 -> 2 with edges only one way, then an isolated node
+    rp = np.array([0, 1, 2, 2, 2], np.uint32)
+    ci = np.array([1, 2], np.uint32)
+    perm = acc.order_rcm(rp, ci, 4)
+    assert sorted(perm.tolist()) == [0, 1, 2, 3]
