@@ -474,11 +474,14 @@ void default_tile(const accspmm_metcf* h, uint32_t K, int form,
     t->xcd_slice = 0;
     t->split_tiles = h->max_window_tiles > mean ? std::max<uint32_t>(64, mean) : 0;
     if ((form == kSPLIT || form == kBF16) && nb == 8 && n_kc >= 4) {
-      // four or more 128-feature chunks in flight: 4 column panels
-      // (K = 512: split-BF16 10.83 against 11.96 ms with 2, BF16 6.26
-      // against 6.90; at K = 256, two chunks, 2 panels stay best: 5.35
-      // against 5.46 ms)
+      // four or more 128-feature chunks in flight: 4 column panels and
+      // pieces of 1.5x the mean window (K = 512: split-BF16 10.44 ms
+      // against 10.87 with pieces of the mean and 11.96 with 2 panels,
+      // BF16 6.15 against 6.23 and 6.90; at K = 256, two chunks, the
+      // one-chunk choices stay best: 5.35 against 5.46 ms with 4 panels)
       t->col_panels = bb > (32ull << 20) ? 4 : 0;
+      const uint32_t piece = std::max<uint32_t>(64, mean + mean / 2);
+      t->split_tiles = h->max_window_tiles > piece ? piece : 0;
     } else if (form == kSPLIT && nb == 8) {
       // split-BF16, one chunk: 2 column panels (0 / 2 / 4 panels: 3.28 /
       // 2.95 / 3.12 ms at K = 128)
