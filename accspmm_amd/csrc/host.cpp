@@ -640,26 +640,40 @@ static accspmm_status read_mtx_impl(const char* path, int canonicalize,
     if (!line.empty() && line[0] == '%') continue;
     break;
   }
-  if (std::sscanf(line.c_str(), "%u %u %u", &m, &n, &nz) != 3) {
-    // the reference keeps scanning tokens until three integers are read
+  {
+    // three unsigned 32-bit integers (a value that overflows, or a sign,
+    // is an error); the reference keeps scanning tokens on the following
+    // lines until three integers are read
     Cursor c0{line.c_str(), line.c_str() + line.size()};
     uint32_t got[3];
     int k = 0;
     while (k < 3 && c0.read_u32(&got[k])) ++k;
-    Cursor c1{p, end};
-    while (k < 3) {
-      if (!c1.read_u32(&got[k])) {
+    if (k < 3) {
+      c0.skip_ws();
+      if (c0.p < c0.end) {  // a token that is not an unsigned 32-bit integer
         if (mtx_code) *mtx_code = 3;
         return ACCSPMM_ERR_MTX;
       }
-      ++k;
+      Cursor c1{p, end};
+      while (k < 3) {
+        if (!c1.read_u32(&got[k])) {
+          if (mtx_code) *mtx_code = 3;
+          return ACCSPMM_ERR_MTX;
+        }
+        ++k;
+      }
+      p = c1.p;
     }
-    p = c1.p;
     m = got[0];
     n = got[1];
     nz = got[2];
   }
   bool symmetric = sym == 'S' || sym == 'H';
+  if (symmetric && m != n) {
+    // the mirrored entry (c, r) of a non-square matrix can fall outside it
+    if (mtx_code) *mtx_code = 3;
+    return ACCSPMM_ERR_MTX;
+  }
   const int threads = host_threads();
   if (threads > 1 && (size_t)(end - p) >= kParallelMinBytes) {
     accspmm_status st = ACCSPMM_OK;

@@ -219,3 +219,35 @@ def test_read_mtx_hermitian_and_skew_as_the_reference(tmp_path, sym):
     rp, ci, va = O.coo_to_csr(m, rr, cc, vv)
     assert np.array_equal(csr.row_ptr, rp) and np.array_equal(csr.col_idx, ci)
     assert np.array_equal(csr.data, va)
+
+
+@pytest.mark.parametrize("sym", ["symmetric", "Hermitian"])
+def test_read_mtx_rejects_non_square_mirrored(tmp_path, sym):
+    # a mirrored entry (c, r) of a non-square matrix falls outside it (a
+    # 4 x 2 symmetric file with entry (3, 1) would need column 3): an error,
+    # not a CSR with a column index past cols
+    field = "complex" if sym == "Hermitian" else "real"
+    vals = " 1.5 0.5" if sym == "Hermitian" else " 1.5"
+    text = f"%%MatrixMarket matrix coordinate {field} {sym}\n4 2 1\n3 1{vals}\n"
+    with pytest.raises(acc.AccSpMMError) as e:
+        acc.read_mtx(_write(tmp_path, "ns.mtx", text))
+    assert e.value.mtx_code == 3
+
+
+@pytest.mark.parametrize("size_line", ["4294967296 2 1", "2 -1 1", "2 2 99999999999999999999",
+                                       "2 2 x1"])
+def test_read_mtx_rejects_bad_size_line(tmp_path, size_line):
+    # sizes are unsigned 32-bit integers: an overflowing, signed or other
+    # token is an error (not a wrapped or saturated size)
+    text = f"%%MatrixMarket matrix coordinate real general\n{size_line}\n1 1 1.0\n"
+    with pytest.raises(acc.AccSpMMError) as e:
+        acc.read_mtx(_write(tmp_path, "sz.mtx", text))
+    assert e.value.mtx_code == 3
+
+
+def test_read_mtx_size_line_split_over_lines(tmp_path):
+    # as the reference's reader, the three sizes may continue on later lines
+    text = "%%MatrixMarket matrix coordinate real general\n3 2\n2\n1 1 1.0\n3 2 -2.0\n"
+    csr = acc.read_mtx(_write(tmp_path, "split.mtx", text))
+    assert (csr.rows, csr.cols, csr.nnz) == (3, 2, 2)
+    assert csr.row_ptr.tolist() == [0, 1, 1, 2] and csr.col_idx.tolist() == [0, 1]
