@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <new>
 #include <vector>
 
 #include "internal.hpp"
@@ -148,11 +149,9 @@ extern "C" accspmm_status accspmm_metcf_save(const accspmm_metcf* h, const char*
   return w.ok ? ACCSPMM_OK : ACCSPMM_ERR_FILE;
 }
 
-extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** out,
-                                             void* stream) {
-  if (!path || !out) return ACCSPMM_ERR_INVALID_ARG;
-  *out = nullptr;
-  (void)stream;
+namespace {
+
+accspmm_status load_impl(const char* path, accspmm_metcf** out) {
   std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "rb"), &std::fclose);
   if (!f) return ACCSPMM_ERR_FILE;
   Reader r;
@@ -173,6 +172,22 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
       (sym && (!reorder || rows != cols)))
     return ACCSPMM_ERR_FILE;
   const uint32_t words = rw / 8;
+  // the header fixes the file's length: check it before sizing any array
+  // from the header, so a corrupt count cannot allocate gigabytes
+  {
+    const uint64_t hw = version == 1 ? kHeaderWordsV1 : version == 2 ? kHeaderWordsV2
+                                                                     : kHeaderWords;
+    const uint64_t expect = 8 + 4 + 4 * hw + key_len + 4 * ((uint64_t)W + 1) +
+                            4 * ((uint64_t)T + 1) + 8 * (uint64_t)T * words +
+                            4 * (uint64_t)T * kColWindow + 4 * (uint64_t)nnz +
+                            (adp ? 4 * ((uint64_t)G + 1) + 4 * (uint64_t)T : 0) +
+                            (reorder ? 4 * (uint64_t)rows : 0) + 8;
+    const long here = std::ftell(f.get());
+    if (here < 0 || std::fseek(f.get(), 0, SEEK_END) != 0) return ACCSPMM_ERR_FILE;
+    const long size = std::ftell(f.get());
+    if (size < 0 || (uint64_t)size != expect || std::fseek(f.get(), here, SEEK_SET) != 0)
+      return ACCSPMM_ERR_FILE;
+  }
   std::string key(key_len, '\0');
   r.get(key.data(), key_len);
   std::vector<uint32_t> rwo((size_t)W + 1), tco((size_t)T + 1), a2b((size_t)T * kColWindow);
@@ -258,4 +273,18 @@ extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** o
   ACC_TRY(measure_group_values(h.get(), nullptr));
   *out = h.release();
   return ACCSPMM_OK;
+}
+
+}  // namespace
+
+extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** out,
+                                             void* stream) {
+  if (!path || !out) return ACCSPMM_ERR_INVALID_ARG;
+  *out = nullptr;
+  (void)stream;
+  try {
+    return load_impl(path, out);
+  } catch (const std::bad_alloc&) {
+    return ACCSPMM_ERR_OUT_OF_MEMORY;
+  }
 }

@@ -67,6 +67,23 @@ def test_oversized_nnz_rejected_before_device_work():
     assert not h.value
 
 
+def test_metcf_load_checks_length_before_allocating(tmp_path):
+    # a METCF file whose header claims ~4 GB of tiles but holds a few bytes
+    # is refused from its length, before any array is sized from the header
+    # or any device call (no GPU here)
+    import struct
+    L = acc.lib()
+    hdr = [10, 10, 0, 8, 2, 0x1FFFFFFF, 0, 0, 0, 0, 0, 8, 8, 1, 1, 1, 1, 1, 0, 0, 0]
+    body = b"ACCSPMTC" + struct.pack("<I", 3) + struct.pack("<21I", *hdr) + bytes(64)
+    for name, data in (("huge.metcf", body), ("short.metcf", body[:20]), ("empty.metcf", b"")):
+        path = tmp_path / name
+        path.write_bytes(data)
+        h = ctypes.c_void_p()
+        assert L.accspmm_metcf_load(str(path).encode(), ctypes.byref(h), None) == \
+            acc.Status.ERR_FILE, name
+        assert not h.value
+
+
 def test_operand_checks_refuse_host_tensors():
     # no GPU needed: host tensors and wrong shapes are refused in Python
     import pytest
