@@ -485,76 +485,84 @@ struct accspmm_batch {
 extern "C" accspmm_status accspmm_batch_create(const accspmm_problem* problems,
                                                uint32_t count, accspmm_form form,
                                                accspmm_batch** out, void* stream) {
-  if (!out || (count && !problems)) return ACCSPMM_ERR_INVALID_ARG;
-  if ((int)form < 0 || (int)form > 2) return ACCSPMM_ERR_INVALID_ARG;
-  *out = nullptr;
-  hipStream_t s = (hipStream_t)stream;
-  auto* b = new (std::nothrow) accspmm_batch();
-  if (!b) return ACCSPMM_ERR_OUT_OF_MEMORY;
-  b->form = (int)form;
-  b->probs.assign(problems, problems + count);
-  b->plans.resize(count);
-  accspmm_status st = ACCSPMM_OK;
-  for (uint32_t i = 0; i < count && st == ACCSPMM_OK; ++i)
-    st = problem_plan(problems[i], (int)form, s, &b->plans[i], b->policy);
-  if (st == ACCSPMM_OK) st = b->prepare_groups(s);
-  if (st == ACCSPMM_OK && hipStreamSynchronize(s) != hipSuccess) st = ACCSPMM_ERR_HIP;
-  if (st != ACCSPMM_OK) {
-    (void)hipStreamSynchronize(s);
-    delete b;
-    return st;
-  }
-  *out = b;
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!out || (count && !problems)) return ACCSPMM_ERR_INVALID_ARG;
+    if ((int)form < 0 || (int)form > 2) return ACCSPMM_ERR_INVALID_ARG;
+    *out = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    auto* b = new (std::nothrow) accspmm_batch();
+    if (!b) return ACCSPMM_ERR_OUT_OF_MEMORY;
+    b->form = (int)form;
+    b->probs.assign(problems, problems + count);
+    b->plans.resize(count);
+    accspmm_status st = ACCSPMM_OK;
+    for (uint32_t i = 0; i < count && st == ACCSPMM_OK; ++i)
+      st = problem_plan(problems[i], (int)form, s, &b->plans[i], b->policy);
+    if (st == ACCSPMM_OK) st = b->prepare_groups(s);
+    if (st == ACCSPMM_OK && hipStreamSynchronize(s) != hipSuccess) st = ACCSPMM_ERR_HIP;
+    if (st != ACCSPMM_OK) {
+      (void)hipStreamSynchronize(s);
+      delete b;
+      return st;
+    }
+    *out = b;
+    return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_batch_rebind(accspmm_batch* b,
                                                const accspmm_problem* problems,
                                                uint32_t count, void* stream) {
-  if (!b || (count && !problems)) return ACCSPMM_ERR_INVALID_ARG;
-  if (count != b->probs.size()) return ACCSPMM_ERR_SHAPE;
-  hipStream_t s = (hipStream_t)stream;
-  std::vector<ProblemPlan> plans(count);
-  for (uint32_t i = 0; i < count; ++i) {
-    if (problems[i].graph != b->probs[i].graph || problems[i].K != b->probs[i].K)
-      return ACCSPMM_ERR_SHAPE;
-    ACC_TRY(problem_plan(problems[i], b->form, s, &plans[i], b->policy));
-    // the new operands must keep the planned tile (captured graphs replay it)
-    if (plans[i].nb != b->plans[i].nb) return ACCSPMM_ERR_UNSUPPORTED;
-  }
-  ACC_HIP(hipStreamSynchronize(s));
-  b->probs.assign(problems, problems + count);
-  b->plans = plans;
-  // same sizes => no buffer is reallocated, so a captured graph stays valid
-  ACC_TRY(b->prepare_groups(s));
-  ACC_HIP(hipStreamSynchronize(s));
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!b || (count && !problems)) return ACCSPMM_ERR_INVALID_ARG;
+    if (count != b->probs.size()) return ACCSPMM_ERR_SHAPE;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<ProblemPlan> plans(count);
+    for (uint32_t i = 0; i < count; ++i) {
+      if (problems[i].graph != b->probs[i].graph || problems[i].K != b->probs[i].K)
+        return ACCSPMM_ERR_SHAPE;
+      ACC_TRY(problem_plan(problems[i], b->form, s, &plans[i], b->policy));
+      // the new operands must keep the planned tile (captured graphs replay it)
+      if (plans[i].nb != b->plans[i].nb) return ACCSPMM_ERR_UNSUPPORTED;
+    }
+    ACC_HIP(hipStreamSynchronize(s));
+    b->probs.assign(problems, problems + count);
+    b->plans = plans;
+    // same sizes => no buffer is reallocated, so a captured graph stays valid
+    ACC_TRY(b->prepare_groups(s));
+    ACC_HIP(hipStreamSynchronize(s));
+    return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_batch_run(accspmm_batch* b, void* stream) {
-  if (!b) return ACCSPMM_ERR_INVALID_ARG;
-  return b->launch((hipStream_t)stream);
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!b) return ACCSPMM_ERR_INVALID_ARG;
+    return b->launch((hipStream_t)stream);
+  });
 }
 
 extern "C" accspmm_status accspmm_batch_run_graph(accspmm_batch* b, void* stream) {
-  if (!b) return ACCSPMM_ERR_INVALID_ARG;
-  if (!b->exec) {
-    if (!b->cap_stream)
-      ACC_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
-    ACC_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-    accspmm_status st = b->launch(b->cap_stream);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
-    if (st != ACCSPMM_OK) {
-      if (g) (void)hipGraphDestroy(g);
-      return st;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!b) return ACCSPMM_ERR_INVALID_ARG;
+    if (!b->exec) {
+      if (!b->cap_stream)
+        ACC_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+      ACC_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+      accspmm_status st = b->launch(b->cap_stream);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+      if (st != ACCSPMM_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      if (e != hipSuccess) return ACCSPMM_ERR_HIP;
+      b->graph = g;
+      ACC_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
-    if (e != hipSuccess) return ACCSPMM_ERR_HIP;
-    b->graph = g;
-    ACC_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-  }
-  ACC_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
-  return ACCSPMM_OK;
+    ACC_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return ACCSPMM_OK;
+  });
 }
 
 namespace {
@@ -574,69 +582,75 @@ accspmm_status replan(accspmm_batch* b, const BatchPolicy& p, hipStream_t s) {
 
 extern "C" accspmm_status accspmm_batch_autotune(accspmm_batch* b, void* stream,
                                                  float* best_ms) {
-  if (!b) return ACCSPMM_ERR_INVALID_ARG;
-  hipStream_t s = (hipStream_t)stream;
-  // a captured graph may hold buffers that re-planning replaces
-  if (b->exec) {
-    (void)hipGraphExecDestroy(b->exec);
-    b->exec = nullptr;
-  }
-  if (b->graph) {
-    (void)hipGraphDestroy(b->graph);
-    b->graph = nullptr;
-  }
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  ACC_HIP(hipEventCreate(&e0));
-  if (hipEventCreate(&e1) != hipSuccess) {
-    (void)hipEventDestroy(e0);
-    return ACCSPMM_ERR_HIP;
-  }
-  BatchPolicy best = b->policy;
-  float bms = 1e30f;
-  accspmm_status st = ACCSPMM_OK;
-  for (uint32_t cap : {4u, 2u, 1u}) {
-    for (uint32_t pack : {0u, 32u, 128u, 512u}) {
-      BatchPolicy p;
-      p.nb_cap = cap;
-      p.pack = pack;
-      st = replan(b, p, s);
-      if (st != ACCSPMM_OK) break;
-      for (int i = 0; i < 2 && st == ACCSPMM_OK; ++i) st = b->launch(s);
-      if (st != ACCSPMM_OK) break;
-      if (hipEventRecord(e0, s) != hipSuccess) { st = ACCSPMM_ERR_HIP; break; }
-      for (int i = 0; i < 5 && st == ACCSPMM_OK; ++i) st = b->launch(s);
-      if (st != ACCSPMM_OK) break;
-      float ms = 0.f;
-      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        st = ACCSPMM_ERR_HIP;
-        break;
-      }
-      if (ms / 5 < bms) {
-        bms = ms / 5;
-        best = p;
-      }
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!b) return ACCSPMM_ERR_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    // a captured graph may hold buffers that re-planning replaces
+    if (b->exec) {
+      (void)hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
     }
-    if (st != ACCSPMM_OK) break;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  ACC_TRY(st);
-  ACC_TRY(replan(b, best, s));
-  if (best_ms) *best_ms = bms;
-  return ACCSPMM_OK;
+    if (b->graph) {
+      (void)hipGraphDestroy(b->graph);
+      b->graph = nullptr;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ACC_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      return ACCSPMM_ERR_HIP;
+    }
+    BatchPolicy best = b->policy;
+    float bms = 1e30f;
+    accspmm_status st = ACCSPMM_OK;
+    for (uint32_t cap : {4u, 2u, 1u}) {
+      for (uint32_t pack : {0u, 32u, 128u, 512u}) {
+        BatchPolicy p;
+        p.nb_cap = cap;
+        p.pack = pack;
+        st = replan(b, p, s);
+        if (st != ACCSPMM_OK) break;
+        for (int i = 0; i < 2 && st == ACCSPMM_OK; ++i) st = b->launch(s);
+        if (st != ACCSPMM_OK) break;
+        if (hipEventRecord(e0, s) != hipSuccess) { st = ACCSPMM_ERR_HIP; break; }
+        for (int i = 0; i < 5 && st == ACCSPMM_OK; ++i) st = b->launch(s);
+        if (st != ACCSPMM_OK) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          st = ACCSPMM_ERR_HIP;
+          break;
+        }
+        if (ms / 5 < bms) {
+          bms = ms / 5;
+          best = p;
+        }
+      }
+      if (st != ACCSPMM_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ACC_TRY(st);
+    ACC_TRY(replan(b, best, s));
+    if (best_ms) *best_ms = bms;
+    return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_batch_destroy(accspmm_batch* b) {
-  if (!b) return ACCSPMM_ERR_INVALID_ARG;
-  (void)hipDeviceSynchronize();
-  delete b;
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!b) return ACCSPMM_ERR_INVALID_ARG;
+    (void)hipDeviceSynchronize();
+    delete b;
+    return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_batch_num_tasks(const accspmm_batch* b,
                                                   uint64_t* tasks) {
-  if (!b || !tasks) return ACCSPMM_ERR_INVALID_ARG;
-  *tasks = b->L[0].total_tasks + b->L[1].total_tasks;
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!b || !tasks) return ACCSPMM_ERR_INVALID_ARG;
+    *tasks = b->L[0].total_tasks + b->L[1].total_tasks;
+    return ACCSPMM_OK;
+  });
 }

@@ -649,195 +649,209 @@ extern "C" accspmm_status accspmm_metcf_build(const accspmm_csr_view* csr,
                                               const accspmm_build_opts* opts,
                                               accspmm_metcf** out,
                                               void* stream) {
-  if (!csr || !out) return ACCSPMM_ERR_INVALID_ARG;
-  *out = nullptr;
-  accspmm_build_opts o;
-  if (opts) o = *opts; else accspmm_build_opts_default(&o);
-  if (o.row_window != 0 && o.row_window != 8 && o.row_window != 16)
-    return ACCSPMM_ERR_UNSUPPORTED;
-  if (o.col_window != 8) return ACCSPMM_ERR_UNSUPPORTED;
-  if (o.build_adpbme && o.t_load + o.t_comp == 0) return ACCSPMM_ERR_UNSUPPORTED;
-  if (o.reorder_rows > 2 || o.symmetric_order > 2) return ACCSPMM_ERR_UNSUPPORTED;
-  if (o.symmetric_order && csr->rows != csr->cols) return ACCSPMM_ERR_UNSUPPORTED;
-  if (csr->rows == 0 || csr->cols == 0) return ACCSPMM_ERR_INVALID_ARG;
-  if (csr->nnz > kMaxNnz) return ACCSPMM_ERR_UNSUPPORTED;
-  // automatic choices (row_window 0, reorder_rows 2): 16-row windows and
-  // rows by length only for long rows (mean >= kLongRows nonzeros), where
-  // they were measured faster (232,965-node power-law graph, mean 493:
-  // 3.74 -> 3.09 ms per split-BF16 step at K=128); on graphs with short
-  // rows (2M nodes, mean 33; 500-50,000 nodes, means 4-64) both were slower
-  const bool long_rows = (uint64_t)csr->nnz >= (uint64_t)kLongRows * csr->rows;
-  const bool auto_window = o.row_window == 0;
-  if (o.row_window == 0) o.row_window = long_rows ? 16 : 8;
-  if (o.reorder_rows == 2) o.reorder_rows = long_rows ? 1 : 0;
-  // a symmetric ordering fixes the rows' order itself
-  if (o.symmetric_order) o.reorder_rows = 0;
-  if (!csr->row_ptr || (csr->nnz && (!csr->col_idx || !csr->val)))
-    return ACCSPMM_ERR_INVALID_ARG;
-  auto* h = new (std::nothrow) accspmm_metcf();
-  if (!h) return ACCSPMM_ERR_OUT_OF_MEMORY;
-  // reorder_rows: build from the rows in order of decreasing length;
-  // symmetric_order: build from P A P^T.  The handle keeps the permutation
-  // (rows; columns for P A P^T): the kernels write C rows through it and
-  // each call permutes B through it.
-  DevBuf prp, pci, pva, rperm, cperm;
-  accspmm_csr_view pv;
-  const accspmm_csr_view* src = csr;
-  accspmm_status st = ACCSPMM_OK;
-  if (o.reorder_rows || o.symmetric_order) {
-    st = o.symmetric_order
-             ? symmetric_permute(csr, o.symmetric_order, prp, pci, pva, rperm,
-                                 (hipStream_t)stream)
-             : reorder_rows_by_length(csr, prp, pci, pva, rperm, (hipStream_t)stream);
-    if (st == ACCSPMM_OK && o.symmetric_order) {
-      st = cperm.alloc((size_t)csr->cols * 4);
-      if (st == ACCSPMM_OK &&
-          hipMemcpyAsync(cperm.p, rperm.p, (size_t)csr->cols * 4, hipMemcpyDeviceToDevice,
-                         (hipStream_t)stream) != hipSuccess)
-        st = ACCSPMM_ERR_HIP;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!csr || !out) return ACCSPMM_ERR_INVALID_ARG;
+    *out = nullptr;
+    accspmm_build_opts o;
+    if (opts) o = *opts; else accspmm_build_opts_default(&o);
+    if (o.row_window != 0 && o.row_window != 8 && o.row_window != 16)
+      return ACCSPMM_ERR_UNSUPPORTED;
+    if (o.col_window != 8) return ACCSPMM_ERR_UNSUPPORTED;
+    if (o.build_adpbme && o.t_load + o.t_comp == 0) return ACCSPMM_ERR_UNSUPPORTED;
+    if (o.reorder_rows > 2 || o.symmetric_order > 2) return ACCSPMM_ERR_UNSUPPORTED;
+    if (o.symmetric_order && csr->rows != csr->cols) return ACCSPMM_ERR_UNSUPPORTED;
+    if (csr->rows == 0 || csr->cols == 0) return ACCSPMM_ERR_INVALID_ARG;
+    if (csr->nnz > kMaxNnz) return ACCSPMM_ERR_UNSUPPORTED;
+    // automatic choices (row_window 0, reorder_rows 2): 16-row windows and
+    // rows by length only for long rows (mean >= kLongRows nonzeros), where
+    // they were measured faster (232,965-node power-law graph, mean 493:
+    // 3.74 -> 3.09 ms per split-BF16 step at K=128); on graphs with short
+    // rows (2M nodes, mean 33; 500-50,000 nodes, means 4-64) both were slower
+    const bool long_rows = (uint64_t)csr->nnz >= (uint64_t)kLongRows * csr->rows;
+    const bool auto_window = o.row_window == 0;
+    if (o.row_window == 0) o.row_window = long_rows ? 16 : 8;
+    if (o.reorder_rows == 2) o.reorder_rows = long_rows ? 1 : 0;
+    // a symmetric ordering fixes the rows' order itself
+    if (o.symmetric_order) o.reorder_rows = 0;
+    if (!csr->row_ptr || (csr->nnz && (!csr->col_idx || !csr->val)))
+      return ACCSPMM_ERR_INVALID_ARG;
+    auto* h = new (std::nothrow) accspmm_metcf();
+    if (!h) return ACCSPMM_ERR_OUT_OF_MEMORY;
+    // reorder_rows: build from the rows in order of decreasing length;
+    // symmetric_order: build from P A P^T.  The handle keeps the permutation
+    // (rows; columns for P A P^T): the kernels write C rows through it and
+    // each call permutes B through it.
+    DevBuf prp, pci, pva, rperm, cperm;
+    accspmm_csr_view pv;
+    const accspmm_csr_view* src = csr;
+    accspmm_status st = ACCSPMM_OK;
+    if (o.reorder_rows || o.symmetric_order) {
+      st = o.symmetric_order
+               ? symmetric_permute(csr, o.symmetric_order, prp, pci, pva, rperm,
+                                   (hipStream_t)stream)
+               : reorder_rows_by_length(csr, prp, pci, pva, rperm, (hipStream_t)stream);
+      if (st == ACCSPMM_OK && o.symmetric_order) {
+        st = cperm.alloc((size_t)csr->cols * 4);
+        if (st == ACCSPMM_OK &&
+            hipMemcpyAsync(cperm.p, rperm.p, (size_t)csr->cols * 4, hipMemcpyDeviceToDevice,
+                           (hipStream_t)stream) != hipSuccess)
+          st = ACCSPMM_ERR_HIP;
+      }
+      pv = accspmm_csr_view{csr->rows, csr->cols, csr->nnz, prp.as<uint32_t>(),
+                            pci.as<uint32_t>(), pva.as<float>()};
+      src = &pv;
     }
-    pv = accspmm_csr_view{csr->rows, csr->cols, csr->nnz, prp.as<uint32_t>(),
-                          pci.as<uint32_t>(), pva.as<float>()};
-    src = &pv;
-  }
-  if (st == ACCSPMM_OK) st = build_impl(src, o, h, (hipStream_t)stream);
-  // automatic choice on short rows of a large graph (>= 64 * 8192 8-row
-  // tiles): 16-row windows when they hold at most 92% of the 8-row tiles,
-  // i.e. rows of a window share columns as in banded matrices (27-point
-  // stencils: 0.88-0.90; large random and power-law graphs: 0.95-0.99).
-  // 27-point stencil 100^3, split-BF16 at K=128: 0.591 ms against 0.674 ms;
-  // FP32 0.712 against 0.822 ms.  Small graphs keep 8-row windows (many
-  // small dense ones share columns too, unmeasured there).
-  // The 8-row build validated the CSR, which the tile count needs.  The
-  // count is optional: if it fails the 8-row graph stands.  The 8-row
-  // arrays are released before the 16-row build (peak memory: one graph
-  // and the count's scratch, never two graphs); should that build fail,
-  // the 8-row graph is built again.
-  if (st == ACCSPMM_OK && auto_window && o.row_window == 8 && !o.reorder_rows &&
-      (uint64_t)h->T >= 64ull * 8192) {
-    uint64_t t16 = 0;
-    const uint64_t t8 = h->T;
-    if (count_tiles(src, 16, (hipStream_t)stream, &t16) == ACCSPMM_OK &&
-        t16 * 100 <= t8 * 92) {
-      (void)hipStreamSynchronize((hipStream_t)stream);
-      delete h;
-      h = new (std::nothrow) accspmm_metcf();
-      if (!h) return ACCSPMM_ERR_OUT_OF_MEMORY;
-      accspmm_build_opts o16 = o;
-      o16.row_window = 16;
-      st = build_impl(src, o16, h, (hipStream_t)stream);
-      if (st != ACCSPMM_OK) {
+    if (st == ACCSPMM_OK) st = build_impl(src, o, h, (hipStream_t)stream);
+    // automatic choice on short rows of a large graph (>= 64 * 8192 8-row
+    // tiles): 16-row windows when they hold at most 92% of the 8-row tiles,
+    // i.e. rows of a window share columns as in banded matrices (27-point
+    // stencils: 0.88-0.90; large random and power-law graphs: 0.95-0.99).
+    // 27-point stencil 100^3, split-BF16 at K=128: 0.591 ms against 0.674 ms;
+    // FP32 0.712 against 0.822 ms.  Small graphs keep 8-row windows (many
+    // small dense ones share columns too, unmeasured there).
+    // The 8-row build validated the CSR, which the tile count needs.  The
+    // count is optional: if it fails the 8-row graph stands.  The 8-row
+    // arrays are released before the 16-row build (peak memory: one graph
+    // and the count's scratch, never two graphs); should that build fail,
+    // the 8-row graph is built again.
+    if (st == ACCSPMM_OK && auto_window && o.row_window == 8 && !o.reorder_rows &&
+        (uint64_t)h->T >= 64ull * 8192) {
+      uint64_t t16 = 0;
+      const uint64_t t8 = h->T;
+      if (count_tiles(src, 16, (hipStream_t)stream, &t16) == ACCSPMM_OK &&
+          t16 * 100 <= t8 * 92) {
         (void)hipStreamSynchronize((hipStream_t)stream);
         delete h;
         h = new (std::nothrow) accspmm_metcf();
         if (!h) return ACCSPMM_ERR_OUT_OF_MEMORY;
-        st = build_impl(src, o, h, (hipStream_t)stream);
+        accspmm_build_opts o16 = o;
+        o16.row_window = 16;
+        st = build_impl(src, o16, h, (hipStream_t)stream);
+        if (st != ACCSPMM_OK) {
+          (void)hipStreamSynchronize((hipStream_t)stream);
+          delete h;
+          h = new (std::nothrow) accspmm_metcf();
+          if (!h) return ACCSPMM_ERR_OUT_OF_MEMORY;
+          st = build_impl(src, o, h, (hipStream_t)stream);
+        }
+      } else {
+        (void)hipGetLastError();
       }
-    } else {
-      (void)hipGetLastError();
     }
-  }
-  if (st != ACCSPMM_OK) {
-    (void)hipStreamSynchronize((hipStream_t)stream);
-    delete h;
-    return st;
-  }
-  h->row_perm.swap(rperm);
-  h->col_perm.swap(cperm);
-  h->opts.symmetric_order = o.symmetric_order;
-  *out = h;
-  return ACCSPMM_OK;
+    if (st != ACCSPMM_OK) {
+      (void)hipStreamSynchronize((hipStream_t)stream);
+      delete h;
+      return st;
+    }
+    h->row_perm.swap(rperm);
+    h->col_perm.swap(cperm);
+    h->opts.symmetric_order = o.symmetric_order;
+    *out = h;
+    return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_metcf_build_host(const accspmm_csr_view* hc,
                                                    const accspmm_build_opts* opts,
                                                    accspmm_metcf** out) {
-  if (!hc || !out || !hc->row_ptr || (hc->nnz && (!hc->col_idx || !hc->val)))
-    return ACCSPMM_ERR_INVALID_ARG;
-  DevBuf rp, ci, va;
-  ACC_TRY(rp.alloc(((uint64_t)hc->rows + 1) * 4));
-  ACC_TRY(ci.alloc((uint64_t)hc->nnz * 4));
-  ACC_TRY(va.alloc((uint64_t)hc->nnz * 4));
-  ACC_HIP(hipMemcpy(rp.p, hc->row_ptr, ((uint64_t)hc->rows + 1) * 4, hipMemcpyHostToDevice));
-  if (hc->nnz) {
-    ACC_HIP(hipMemcpy(ci.p, hc->col_idx, (uint64_t)hc->nnz * 4, hipMemcpyHostToDevice));
-    ACC_HIP(hipMemcpy(va.p, hc->val, (uint64_t)hc->nnz * 4, hipMemcpyHostToDevice));
-  }
-  accspmm_csr_view dv{hc->rows, hc->cols, hc->nnz, rp.as<uint32_t>(), ci.as<uint32_t>(),
-                      va.as<float>()};
-  return accspmm_metcf_build(&dv, opts, out, nullptr);
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!hc || !out || !hc->row_ptr || (hc->nnz && (!hc->col_idx || !hc->val)))
+      return ACCSPMM_ERR_INVALID_ARG;
+    DevBuf rp, ci, va;
+    ACC_TRY(rp.alloc(((uint64_t)hc->rows + 1) * 4));
+    ACC_TRY(ci.alloc((uint64_t)hc->nnz * 4));
+    ACC_TRY(va.alloc((uint64_t)hc->nnz * 4));
+    ACC_HIP(hipMemcpy(rp.p, hc->row_ptr, ((uint64_t)hc->rows + 1) * 4, hipMemcpyHostToDevice));
+    if (hc->nnz) {
+      ACC_HIP(hipMemcpy(ci.p, hc->col_idx, (uint64_t)hc->nnz * 4, hipMemcpyHostToDevice));
+      ACC_HIP(hipMemcpy(va.p, hc->val, (uint64_t)hc->nnz * 4, hipMemcpyHostToDevice));
+    }
+    accspmm_csr_view dv{hc->rows, hc->cols, hc->nnz, rp.as<uint32_t>(), ci.as<uint32_t>(),
+                        va.as<float>()};
+    return accspmm_metcf_build(&dv, opts, out, nullptr);
+  });
 }
 
 extern "C" accspmm_status accspmm_metcf_destroy(accspmm_metcf* h) {
-  if (!h) return ACCSPMM_ERR_INVALID_ARG;
-  (void)hipDeviceSynchronize();
-  delete h;
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!h) return ACCSPMM_ERR_INVALID_ARG;
+    (void)hipDeviceSynchronize();
+    delete h;
+    return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_metcf_get_info(const accspmm_metcf* h,
                                                  accspmm_metcf_info* info) {
-  if (!h || !info) return ACCSPMM_ERR_INVALID_ARG;
-  info->rows = h->rows;
-  info->cols = h->cols;
-  info->nnz = h->nnz;
-  info->row_window = h->rw;
-  info->num_windows = h->W;
-  info->num_tiles = h->T;
-  info->num_groups = h->G;
-  info->ibd = h->ibd;
-  info->balanced = h->balanced;
-  info->max_window_tiles = h->max_window_tiles;
-  info->device_bytes = h->device_bytes();
-  info->reordered = h->row_perm.p ? 1u : 0u;
-  info->max_group_values = h->max_group_values;
-  info->symmetric_order = h->col_perm.p ? h->opts.symmetric_order : 0u;
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!h || !info) return ACCSPMM_ERR_INVALID_ARG;
+    info->rows = h->rows;
+    info->cols = h->cols;
+    info->nnz = h->nnz;
+    info->row_window = h->rw;
+    info->num_windows = h->W;
+    info->num_tiles = h->T;
+    info->num_groups = h->G;
+    info->ibd = h->ibd;
+    info->balanced = h->balanced;
+    info->max_window_tiles = h->max_window_tiles;
+    info->device_bytes = h->device_bytes();
+    info->reordered = h->row_perm.p ? 1u : 0u;
+    info->max_group_values = h->max_group_values;
+    info->symmetric_order = h->col_perm.p ? h->opts.symmetric_order : 0u;
+    return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_metcf_row_order(const accspmm_metcf* h,
                                                   uint32_t* row_order) {
-  if (!h || !row_order) return ACCSPMM_ERR_INVALID_ARG;
-  if (!h->row_perm.p) {
-    for (uint32_t i = 0; i < h->rows; ++i) row_order[i] = i;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!h || !row_order) return ACCSPMM_ERR_INVALID_ARG;
+    if (!h->row_perm.p) {
+      for (uint32_t i = 0; i < h->rows; ++i) row_order[i] = i;
+      return ACCSPMM_OK;
+    }
+    ACC_HIP(hipSetDevice(h->device));
+    ACC_HIP(hipMemcpy(row_order, h->row_perm.p, (size_t)h->rows * 4, hipMemcpyDeviceToHost));
     return ACCSPMM_OK;
-  }
-  ACC_HIP(hipSetDevice(h->device));
-  ACC_HIP(hipMemcpy(row_order, h->row_perm.p, (size_t)h->rows * 4, hipMemcpyDeviceToHost));
-  return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_metcf_col_order(const accspmm_metcf* h,
                                                   uint32_t* col_order) {
-  if (!h || !col_order) return ACCSPMM_ERR_INVALID_ARG;
-  if (!h->col_perm.p) {
-    for (uint32_t j = 0; j < h->cols; ++j) col_order[j] = j;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!h || !col_order) return ACCSPMM_ERR_INVALID_ARG;
+    if (!h->col_perm.p) {
+      for (uint32_t j = 0; j < h->cols; ++j) col_order[j] = j;
+      return ACCSPMM_OK;
+    }
+    ACC_HIP(hipSetDevice(h->device));
+    ACC_HIP(hipMemcpy(col_order, h->col_perm.p, (size_t)h->cols * 4, hipMemcpyDeviceToHost));
     return ACCSPMM_OK;
-  }
-  ACC_HIP(hipSetDevice(h->device));
-  ACC_HIP(hipMemcpy(col_order, h->col_perm.p, (size_t)h->cols * 4, hipMemcpyDeviceToHost));
-  return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_metcf_export(
     const accspmm_metcf* h, uint32_t* rowWindowOffset, uint32_t* tcOffset,
     uint64_t* tcLocalBit, uint32_t* sparseA2B, float* data,
     uint32_t* groupOffset, uint32_t* rowIndices) {
-  if (!h) return ACCSPMM_ERR_INVALID_ARG;
-  if ((groupOffset || rowIndices) && !h->opts.build_adpbme)
-    return ACCSPMM_ERR_INVALID_ARG;
-  ACC_HIP(hipSetDevice(h->device));
-  ACC_HIP(hipDeviceSynchronize());
-  auto cp = [](void* dst, const void* src, uint64_t bytes) -> hipError_t {
-    if (!dst || bytes == 0) return hipSuccess;
-    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
-  };
-  ACC_HIP(cp(rowWindowOffset, h->rwo.p, ((uint64_t)h->W + 1) * 4));
-  ACC_HIP(cp(tcOffset, h->tco.p, ((uint64_t)h->T + 1) * 4));
-  ACC_HIP(cp(tcLocalBit, h->bits.p, (uint64_t)h->T * h->words * 8));
-  ACC_HIP(cp(sparseA2B, h->a2b.p, (uint64_t)h->T * kColWindow * 4));
-  ACC_HIP(cp(data, h->data.p, (uint64_t)h->nnz * 4));
-  ACC_HIP(cp(groupOffset, h->grp.p, ((uint64_t)h->G + 1) * 4));
-  ACC_HIP(cp(rowIndices, h->rix.p, (uint64_t)h->T * 4));
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!h) return ACCSPMM_ERR_INVALID_ARG;
+    if ((groupOffset || rowIndices) && !h->opts.build_adpbme)
+      return ACCSPMM_ERR_INVALID_ARG;
+    ACC_HIP(hipSetDevice(h->device));
+    ACC_HIP(hipDeviceSynchronize());
+    auto cp = [](void* dst, const void* src, uint64_t bytes) -> hipError_t {
+      if (!dst || bytes == 0) return hipSuccess;
+      return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    };
+    ACC_HIP(cp(rowWindowOffset, h->rwo.p, ((uint64_t)h->W + 1) * 4));
+    ACC_HIP(cp(tcOffset, h->tco.p, ((uint64_t)h->T + 1) * 4));
+    ACC_HIP(cp(tcLocalBit, h->bits.p, (uint64_t)h->T * h->words * 8));
+    ACC_HIP(cp(sparseA2B, h->a2b.p, (uint64_t)h->T * kColWindow * 4));
+    ACC_HIP(cp(data, h->data.p, (uint64_t)h->nnz * 4));
+    ACC_HIP(cp(groupOffset, h->grp.p, ((uint64_t)h->G + 1) * 4));
+    ACC_HIP(cp(rowIndices, h->rix.p, (uint64_t)h->T * 4));
+    return ACCSPMM_OK;
+  });
 }

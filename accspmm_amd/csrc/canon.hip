@@ -387,10 +387,12 @@ extern "C" accspmm_status accspmm_csr_canonicalize(uint32_t rows, uint32_t cols,
                                                    uint32_t* row_ptr, uint32_t* col_idx,
                                                    float* val, uint32_t* nnz_out,
                                                    void* stream) {
-  if (!row_ptr || !nnz_out || rows == 0 || cols == 0 || (nnz && (!col_idx || !val)))
-    return ACCSPMM_ERR_INVALID_ARG;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const uint32_t bits = bit_width(cols - 1u) + bit_width(rows - 1u);
-  if (bits <= 32u) return canonicalize<uint32_t>(rows, cols, nnz, row_ptr, col_idx, val, nnz_out, s);
-  return canonicalize<uint64_t>(rows, cols, nnz, row_ptr, col_idx, val, nnz_out, s);
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!row_ptr || !nnz_out || rows == 0 || cols == 0 || (nnz && (!col_idx || !val)))
+      return ACCSPMM_ERR_INVALID_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t bits = bit_width(cols - 1u) + bit_width(rows - 1u);
+    if (bits <= 32u) return canonicalize<uint32_t>(rows, cols, nnz, row_ptr, col_idx, val, nnz_out, s);
+    return canonicalize<uint64_t>(rows, cols, nnz, row_ptr, col_idx, val, nnz_out, s);
+  });
 }

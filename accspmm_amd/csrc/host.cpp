@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/accspmm.h"
+#include "guard.hpp"
 #include "rng.hpp"
 
 extern "C" void accspmm_host_csr_free(accspmm_host_csr* c) {
@@ -734,18 +735,20 @@ static accspmm_status read_mtx_impl(const char* path, int canonicalize,
 
 extern "C" accspmm_status accspmm_read_mtx(const char* path, int canonicalize,
                                            accspmm_host_csr* out, int* mtx_code) {
-  if (!path || !out) return ACCSPMM_ERR_INVALID_ARG;
-  std::memset(out, 0, sizeof(*out));
-  if (mtx_code) *mtx_code = 0;
-  g_threads_used = 0;
-  try {
-    const accspmm_status st = read_mtx_impl(path, canonicalize, out, mtx_code);
-    if (st != ACCSPMM_OK) accspmm_host_csr_free(out);
-    return st;
-  } catch (const std::exception&) {  // allocation or thread creation failed
-    accspmm_host_csr_free(out);
-    return ACCSPMM_ERR_OUT_OF_MEMORY;
-  }
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!path || !out) return ACCSPMM_ERR_INVALID_ARG;
+    std::memset(out, 0, sizeof(*out));
+    if (mtx_code) *mtx_code = 0;
+    g_threads_used = 0;
+    try {
+      const accspmm_status st = read_mtx_impl(path, canonicalize, out, mtx_code);
+      if (st != ACCSPMM_OK) accspmm_host_csr_free(out);
+      return st;
+    } catch (const std::exception&) {  // allocation or thread creation failed
+      accspmm_host_csr_free(out);
+      return ACCSPMM_ERR_OUT_OF_MEMORY;
+    }
+  });
 }
 
 extern "C" int accspmm_read_mtx_threads_used(void) { return g_threads_used; }
@@ -795,46 +798,48 @@ inline void put_entry(std::string& s, uint32_t r, uint32_t c, float v) {
 // sorted rows.  Rows are formatted on several threads and written in order.
 extern "C" accspmm_status accspmm_write_mtx(const accspmm_host_csr* c, const char* path,
                                             int symmetric) {
-  if (!c || !path || !c->row_ptr || (c->nnz && (!c->col_idx || !c->val)))
-    return ACCSPMM_ERR_INVALID_ARG;
-  if (symmetric && c->rows != c->cols) return ACCSPMM_ERR_SHAPE;
-  try {
-    const int parts = c->nnz >= (1u << 16) ? host_threads() : 1;
-    const std::vector<uint32_t> blk = row_blocks(c->row_ptr, c->rows, parts);
-    std::vector<std::string> text(parts);
-    std::vector<uint64_t> count(parts, 0);
-    run_parallel(parts, [&](int t) {
-      std::string& s = text[t];
-      s.reserve((size_t)(c->row_ptr[blk[t + 1]] - c->row_ptr[blk[t]]) * 24);
-      for (uint32_t r = blk[t]; r < blk[t + 1]; ++r) {
-        for (uint32_t e = c->row_ptr[r]; e < c->row_ptr[r + 1]; ++e) {
-          const uint32_t col = c->col_idx[e];
-          if (symmetric) {
-            if (col < r) continue;
-            put_entry(s, col, r, c->val[e]);  // (col, r) below the diagonal
-          } else {
-            put_entry(s, r, col, c->val[e]);
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!c || !path || !c->row_ptr || (c->nnz && (!c->col_idx || !c->val)))
+      return ACCSPMM_ERR_INVALID_ARG;
+    if (symmetric && c->rows != c->cols) return ACCSPMM_ERR_SHAPE;
+    try {
+      const int parts = c->nnz >= (1u << 16) ? host_threads() : 1;
+      const std::vector<uint32_t> blk = row_blocks(c->row_ptr, c->rows, parts);
+      std::vector<std::string> text(parts);
+      std::vector<uint64_t> count(parts, 0);
+      run_parallel(parts, [&](int t) {
+        std::string& s = text[t];
+        s.reserve((size_t)(c->row_ptr[blk[t + 1]] - c->row_ptr[blk[t]]) * 24);
+        for (uint32_t r = blk[t]; r < blk[t + 1]; ++r) {
+          for (uint32_t e = c->row_ptr[r]; e < c->row_ptr[r + 1]; ++e) {
+            const uint32_t col = c->col_idx[e];
+            if (symmetric) {
+              if (col < r) continue;
+              put_entry(s, col, r, c->val[e]);  // (col, r) below the diagonal
+            } else {
+              put_entry(s, r, col, c->val[e]);
+            }
+            ++count[t];
           }
-          ++count[t];
         }
-      }
-    });
-    uint64_t total = 0;
-    for (uint64_t x : count) total += x;
-    std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "wb"), &std::fclose);
-    if (!f) return ACCSPMM_ERR_FILE;
-    std::string head = std::string("%%MatrixMarket matrix coordinate real ") +
-                       (symmetric ? "symmetric\n" : "general\n");
-    head += std::to_string(c->rows) + " " + std::to_string(c->cols) + " " +
-            std::to_string(total) + "\n";
-    bool ok = std::fwrite(head.data(), 1, head.size(), f.get()) == head.size();
-    for (const std::string& s : text)
-      ok = ok && std::fwrite(s.data(), 1, s.size(), f.get()) == s.size();
-    ok = (std::fclose(f.release()) == 0) && ok;
-    return ok ? ACCSPMM_OK : ACCSPMM_ERR_FILE;
-  } catch (const std::exception&) {
-    return ACCSPMM_ERR_OUT_OF_MEMORY;
-  }
+      });
+      uint64_t total = 0;
+      for (uint64_t x : count) total += x;
+      std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "wb"), &std::fclose);
+      if (!f) return ACCSPMM_ERR_FILE;
+      std::string head = std::string("%%MatrixMarket matrix coordinate real ") +
+                         (symmetric ? "symmetric\n" : "general\n");
+      head += std::to_string(c->rows) + " " + std::to_string(c->cols) + " " +
+              std::to_string(total) + "\n";
+      bool ok = std::fwrite(head.data(), 1, head.size(), f.get()) == head.size();
+      for (const std::string& s : text)
+        ok = ok && std::fwrite(s.data(), 1, s.size(), f.get()) == s.size();
+      ok = (std::fclose(f.release()) == 0) && ok;
+      return ok ? ACCSPMM_OK : ACCSPMM_ERR_FILE;
+    } catch (const std::exception&) {
+      return ACCSPMM_ERR_OUT_OF_MEMORY;
+    }
+  });
 }
 
 // ------------------------------------------------------------------------
@@ -857,160 +862,164 @@ extern "C" accspmm_status accspmm_gen_chung_lu(uint32_t n, double dbar,
                                                double gamma, uint64_t seed,
                                                int value_mode,
                                                accspmm_host_csr* out) {
-  if (!out || n == 0 || !(gamma > 1.0) || !(dbar >= 0.0))
-    return ACCSPMM_ERR_INVALID_ARG;
-  std::memset(out, 0, sizeof(*out));
-  std::vector<double> w(n);
-  const double ex = -1.0 / (gamma - 1.0);
-  for (uint32_t i = 0; i < n; ++i) w[i] = std::pow((double)(i + 1), ex);
-  for (int it = 0; it < 30; ++it) {
-    double s = 0.0;
-    for (double x : w) s += x;
-    double scale = dbar * n / s;
-    for (double& x : w) x *= scale;
-    s = dbar * n;
-    double cap = std::sqrt(s);
-    for (double& x : w) x = std::min(x, cap);
-  }
-  double S = 0.0;
-  for (double x : w) S += x;
-  accspmm::Xoshiro256 rng(seed);
-  std::vector<uint32_t> eu, ev;
-  eu.reserve((size_t)(S / 2 * 1.02) + 16);
-  ev.reserve(eu.capacity());
-  if (S > 0) {
-    for (uint32_t u = 0; u + 1 < n; ++u) {
-      uint64_t v = u + 1;
-      double p = std::min(w[u] * w[v] / S, 1.0);
-      while (v < n && p > 0) {
-        if (p != 1.0) {
-          double r = rng.open_unit();
-          double skip = std::floor(std::log(r) / std::log1p(-p));
-          if (skip > (double)n) break;
-          v += (uint64_t)skip;
-        }
-        if (v < n) {
-          double q = std::min(w[u] * w[v] / S, 1.0);
-          double r = rng.open_unit();
-          if (r < q / p) {
-            eu.push_back(u);
-            ev.push_back((uint32_t)v);
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!out || n == 0 || !(gamma > 1.0) || !(dbar >= 0.0))
+      return ACCSPMM_ERR_INVALID_ARG;
+    std::memset(out, 0, sizeof(*out));
+    std::vector<double> w(n);
+    const double ex = -1.0 / (gamma - 1.0);
+    for (uint32_t i = 0; i < n; ++i) w[i] = std::pow((double)(i + 1), ex);
+    for (int it = 0; it < 30; ++it) {
+      double s = 0.0;
+      for (double x : w) s += x;
+      double scale = dbar * n / s;
+      for (double& x : w) x *= scale;
+      s = dbar * n;
+      double cap = std::sqrt(s);
+      for (double& x : w) x = std::min(x, cap);
+    }
+    double S = 0.0;
+    for (double x : w) S += x;
+    accspmm::Xoshiro256 rng(seed);
+    std::vector<uint32_t> eu, ev;
+    eu.reserve((size_t)(S / 2 * 1.02) + 16);
+    ev.reserve(eu.capacity());
+    if (S > 0) {
+      for (uint32_t u = 0; u + 1 < n; ++u) {
+        uint64_t v = u + 1;
+        double p = std::min(w[u] * w[v] / S, 1.0);
+        while (v < n && p > 0) {
+          if (p != 1.0) {
+            double r = rng.open_unit();
+            double skip = std::floor(std::log(r) / std::log1p(-p));
+            if (skip > (double)n) break;
+            v += (uint64_t)skip;
           }
-          p = q;
-          ++v;
+          if (v < n) {
+            double q = std::min(w[u] * w[v] / S, 1.0);
+            double r = rng.open_unit();
+            if (r < q / p) {
+              eu.push_back(u);
+              ev.push_back((uint32_t)v);
+            }
+            p = q;
+            ++v;
+          }
         }
       }
     }
-  }
-  // relabel with a Fisher-Yates permutation
-  std::vector<uint32_t> perm(n);
-  std::iota(perm.begin(), perm.end(), 0u);
-  for (uint32_t i = n - 1; i > 0; --i) {
-    uint32_t j = (uint32_t)(rng.next() % (uint64_t)(i + 1));
-    std::swap(perm[i], perm[j]);
-  }
-  const uint64_t E = eu.size();
-  const uint64_t nnz = 2 * E + n;
-  if (nnz > 0xFFFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
-  if (!alloc_csr(out, n, n, nnz)) return ACCSPMM_ERR_OUT_OF_MEMORY;
-  std::vector<float> evals;
-  std::vector<float> svals;
-  if (value_mode == 0) {
-    accspmm::Xoshiro256 vr(seed ^ 0x5bd1e995ull);
-    evals.resize(E);
-    for (uint64_t i = 0; i < E; ++i) evals[i] = vr.pos_unit();
-    svals.resize(n);
-    for (uint32_t i = 0; i < n; ++i) svals[i] = vr.pos_unit();
-  }
-  uint32_t* rp = out->row_ptr;
-  std::fill(rp, rp + n + 1, 0u);
-  for (uint64_t i = 0; i < E; ++i) {
-    rp[perm[eu[i]] + 1]++;
-    rp[perm[ev[i]] + 1]++;
-  }
-  for (uint32_t i = 0; i < n; ++i) rp[i + 1] += 1;  // self loops
-  for (uint32_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
-  std::vector<uint32_t> fill(rp, rp + n);
-  for (uint32_t i = 0; i < n; ++i) {
-    uint32_t r = perm[i];
-    uint32_t d = fill[r]++;
-    out->col_idx[d] = r;
-    out->val[d] = value_mode == 0 ? svals[i] : 0.0f;
-  }
-  for (uint64_t i = 0; i < E; ++i) {
-    uint32_t a = perm[eu[i]], b = perm[ev[i]];
-    float v = value_mode == 0 ? evals[i] : 0.0f;
-    uint32_t d = fill[a]++;
-    out->col_idx[d] = b;
-    out->val[d] = v;
-    d = fill[b]++;
-    out->col_idx[d] = a;
-    out->val[d] = v;
-  }
-  eu.clear(); eu.shrink_to_fit();
-  ev.clear(); ev.shrink_to_fit();
-  // per-row sort (columns are unique: no duplicate pairs, one self loop);
-  // rows are independent, so the result does not depend on the split
-  {
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> pool;
-    for (unsigned ti = 0; ti < nt; ++ti)
-      pool.emplace_back([&, ti]() {
-        std::vector<std::pair<uint32_t, float>> tmp;
-        for (uint64_t r = ti; r < n; r += nt) {
-          uint32_t b = rp[r], e = rp[r + 1];
-          tmp.resize(e - b);
-          for (uint32_t i = b; i < e; ++i) tmp[i - b] = {out->col_idx[i], out->val[i]};
-          std::sort(tmp.begin(), tmp.end(),
-                    [](const std::pair<uint32_t, float>& x,
-                       const std::pair<uint32_t, float>& y) { return x.first < y.first; });
-          for (uint32_t i = b; i < e; ++i) {
-            out->col_idx[i] = tmp[i - b].first;
-            out->val[i] = tmp[i - b].second;
-          }
-        }
-      });
-    for (auto& th : pool) th.join();
-  }
-  if (value_mode == 1) {
-    for (uint32_t r = 0; r < n; ++r) {
-      float v = (float)(1.0 / (double)(rp[r + 1] - rp[r]));
-      for (uint32_t e = rp[r]; e < rp[r + 1]; ++e) out->val[e] = v;
+    // relabel with a Fisher-Yates permutation
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    for (uint32_t i = n - 1; i > 0; --i) {
+      uint32_t j = (uint32_t)(rng.next() % (uint64_t)(i + 1));
+      std::swap(perm[i], perm[j]);
     }
-  } else if (value_mode == 2) {
-    accspmm::Xoshiro256 vr(seed ^ 0x5bd1e995ull);
-    for (uint64_t e = 0; e < nnz; ++e) out->val[e] = vr.sym_unit();
-  }
-  return ACCSPMM_OK;
+    const uint64_t E = eu.size();
+    const uint64_t nnz = 2 * E + n;
+    if (nnz > 0xFFFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
+    if (!alloc_csr(out, n, n, nnz)) return ACCSPMM_ERR_OUT_OF_MEMORY;
+    std::vector<float> evals;
+    std::vector<float> svals;
+    if (value_mode == 0) {
+      accspmm::Xoshiro256 vr(seed ^ 0x5bd1e995ull);
+      evals.resize(E);
+      for (uint64_t i = 0; i < E; ++i) evals[i] = vr.pos_unit();
+      svals.resize(n);
+      for (uint32_t i = 0; i < n; ++i) svals[i] = vr.pos_unit();
+    }
+    uint32_t* rp = out->row_ptr;
+    std::fill(rp, rp + n + 1, 0u);
+    for (uint64_t i = 0; i < E; ++i) {
+      rp[perm[eu[i]] + 1]++;
+      rp[perm[ev[i]] + 1]++;
+    }
+    for (uint32_t i = 0; i < n; ++i) rp[i + 1] += 1;  // self loops
+    for (uint32_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
+    std::vector<uint32_t> fill(rp, rp + n);
+    for (uint32_t i = 0; i < n; ++i) {
+      uint32_t r = perm[i];
+      uint32_t d = fill[r]++;
+      out->col_idx[d] = r;
+      out->val[d] = value_mode == 0 ? svals[i] : 0.0f;
+    }
+    for (uint64_t i = 0; i < E; ++i) {
+      uint32_t a = perm[eu[i]], b = perm[ev[i]];
+      float v = value_mode == 0 ? evals[i] : 0.0f;
+      uint32_t d = fill[a]++;
+      out->col_idx[d] = b;
+      out->val[d] = v;
+      d = fill[b]++;
+      out->col_idx[d] = a;
+      out->val[d] = v;
+    }
+    eu.clear(); eu.shrink_to_fit();
+    ev.clear(); ev.shrink_to_fit();
+    // per-row sort (columns are unique: no duplicate pairs, one self loop);
+    // rows are independent, so the result does not depend on the split
+    {
+      unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+      std::vector<std::thread> pool;
+      for (unsigned ti = 0; ti < nt; ++ti)
+        pool.emplace_back([&, ti]() {
+          std::vector<std::pair<uint32_t, float>> tmp;
+          for (uint64_t r = ti; r < n; r += nt) {
+            uint32_t b = rp[r], e = rp[r + 1];
+            tmp.resize(e - b);
+            for (uint32_t i = b; i < e; ++i) tmp[i - b] = {out->col_idx[i], out->val[i]};
+            std::sort(tmp.begin(), tmp.end(),
+                      [](const std::pair<uint32_t, float>& x,
+                         const std::pair<uint32_t, float>& y) { return x.first < y.first; });
+            for (uint32_t i = b; i < e; ++i) {
+              out->col_idx[i] = tmp[i - b].first;
+              out->val[i] = tmp[i - b].second;
+            }
+          }
+        });
+      for (auto& th : pool) th.join();
+    }
+    if (value_mode == 1) {
+      for (uint32_t r = 0; r < n; ++r) {
+        float v = (float)(1.0 / (double)(rp[r + 1] - rp[r]));
+        for (uint32_t e = rp[r]; e < rp[r + 1]; ++e) out->val[e] = v;
+      }
+    } else if (value_mode == 2) {
+      accspmm::Xoshiro256 vr(seed ^ 0x5bd1e995ull);
+      for (uint64_t e = 0; e < nnz; ++e) out->val[e] = vr.sym_unit();
+    }
+    return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_gen_stencil27(uint32_t g, uint64_t seed,
                                                 accspmm_host_csr* out) {
-  if (!out || g == 0) return ACCSPMM_ERR_INVALID_ARG;
-  std::memset(out, 0, sizeof(*out));
-  uint64_t n = (uint64_t)g * g * g;
-  uint64_t per = 3ull * g - 2;
-  uint64_t nnz = per * per * per;
-  if (n > 0xFFFFFFFFull || nnz > 0xFFFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
-  if (!alloc_csr(out, (uint32_t)n, (uint32_t)n, nnz))
-    return ACCSPMM_ERR_OUT_OF_MEMORY;
-  accspmm::Xoshiro256 vr(seed);
-  uint64_t k = 0;
-  for (uint64_t id = 0; id < n; ++id) {
-    out->row_ptr[id] = (uint32_t)k;
-    int64_t x = id % g, y = (id / g) % g, z = id / ((uint64_t)g * g);
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          int64_t X = x + dx, Y = y + dy, Z = z + dz;
-          if (X < 0 || Y < 0 || Z < 0 || X >= g || Y >= g || Z >= g) continue;
-          out->col_idx[k] = (uint32_t)(X + (int64_t)g * Y + (int64_t)g * g * Z);
-          out->val[k] = vr.sym_unit();
-          ++k;
-        }
-  }
-  out->row_ptr[n] = (uint32_t)k;
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!out || g == 0) return ACCSPMM_ERR_INVALID_ARG;
+    std::memset(out, 0, sizeof(*out));
+    uint64_t n = (uint64_t)g * g * g;
+    uint64_t per = 3ull * g - 2;
+    uint64_t nnz = per * per * per;
+    if (n > 0xFFFFFFFFull || nnz > 0xFFFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
+    if (!alloc_csr(out, (uint32_t)n, (uint32_t)n, nnz))
+      return ACCSPMM_ERR_OUT_OF_MEMORY;
+    accspmm::Xoshiro256 vr(seed);
+    uint64_t k = 0;
+    for (uint64_t id = 0; id < n; ++id) {
+      out->row_ptr[id] = (uint32_t)k;
+      int64_t x = id % g, y = (id / g) % g, z = id / ((uint64_t)g * g);
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            int64_t X = x + dx, Y = y + dy, Z = z + dz;
+            if (X < 0 || Y < 0 || Z < 0 || X >= g || Y >= g || Z >= g) continue;
+            out->col_idx[k] = (uint32_t)(X + (int64_t)g * Y + (int64_t)g * g * Z);
+            out->val[k] = vr.sym_unit();
+            ++k;
+          }
+    }
+    out->row_ptr[n] = (uint32_t)k;
+    return ACCSPMM_OK;
+  });
 }
 
 // Reverse Cuthill-McKee ordering of a square pattern (Cuthill and McKee,
@@ -1067,82 +1076,84 @@ uint32_t rcm_levels(const RcmGraph& g, uint32_t r, std::vector<uint32_t>& mark, 
 
 extern "C" accspmm_status accspmm_order_rcm(uint32_t n, const uint32_t* row_ptr,
                                             const uint32_t* col_idx, uint32_t* perm) {
-  if (!row_ptr || !perm || n == 0) return ACCSPMM_ERR_INVALID_ARG;
-  if (row_ptr[0] != 0) return ACCSPMM_ERR_INVALID_ARG;
-  for (uint32_t v = 0; v < n; ++v)
-    if (row_ptr[v + 1] < row_ptr[v]) return ACCSPMM_ERR_INVALID_ARG;
-  if (row_ptr[n] && !col_idx) return ACCSPMM_ERR_INVALID_ARG;
-  for (uint64_t e = 0; e < row_ptr[n]; ++e)
-    if (col_idx[e] >= n) return ACCSPMM_ERR_INVALID_ARG;
-  try {
-    const RcmGraph g{n, row_ptr, col_idx};
-    // stamps: 1 = placed in the final order; >= 2 the search passes
-    std::vector<uint32_t> mark(n, 0), order, nbr;
-    order.reserve(n);
-    std::vector<uint32_t> out;
-    out.reserve(n);
-    // component roots are taken in increasing degree (ties: index)
-    std::vector<uint32_t> by_deg(n);
-    std::iota(by_deg.begin(), by_deg.end(), 0u);
-    std::stable_sort(by_deg.begin(), by_deg.end(),
-                     [&](uint32_t a, uint32_t b) { return g.deg(a) < g.deg(b); });
-    uint32_t stamp = 1;
-    // a fresh search stamp; before the counter wraps, clear every search
-    // mark (placed nodes keep 1) so an old stamp never reads as current
-    auto next_stamp = [&]() {
-      if (stamp == 0xFFFFFFFFu) {
-        for (uint32_t& m : mark)
-          if (m != 1) m = 0;
-        stamp = 1;
-      }
-      return ++stamp;
-    };
-    for (uint32_t root : by_deg) {
-      if (mark[root] == 1) continue;
-      // pseudo-peripheral node: move to a smallest-degree node of the last
-      // level while that deepens the level structure (bounded passes)
-      uint32_t r = root;
-      size_t last = 0;
-      uint32_t depth = rcm_levels(g, r, mark, next_stamp(), order, &last);
-      for (int pass = 0; pass < 8; ++pass) {
-        uint32_t best = order[last];
-        for (size_t i = last; i < order.size(); ++i)
-          if (g.deg(order[i]) < g.deg(best) || (g.deg(order[i]) == g.deg(best) && order[i] < best))
-            best = order[i];
-        size_t last2 = 0;
-        std::vector<uint32_t> order2;
-        order2.reserve(order.size());
-        const uint32_t d2 = rcm_levels(g, best, mark, next_stamp(), order2, &last2);
-        if (d2 <= depth) break;
-        r = best;
-        depth = d2;
-        order.swap(order2);
-        last = last2;
-      }
-      // Cuthill-McKee from r over the component
-      const size_t first = out.size();
-      out.push_back(r);
-      mark[r] = 1;
-      for (size_t i = first; i < out.size(); ++i) {
-        const uint32_t v = out[i];
-        nbr.clear();
-        for (uint32_t e = g.rp[v]; e < g.rp[v + 1]; ++e) {
-          const uint32_t w = g.ci[e];
-          if (mark[w] != 1) {
-            mark[w] = 1;
-            nbr.push_back(w);
-          }
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!row_ptr || !perm || n == 0) return ACCSPMM_ERR_INVALID_ARG;
+    if (row_ptr[0] != 0) return ACCSPMM_ERR_INVALID_ARG;
+    for (uint32_t v = 0; v < n; ++v)
+      if (row_ptr[v + 1] < row_ptr[v]) return ACCSPMM_ERR_INVALID_ARG;
+    if (row_ptr[n] && !col_idx) return ACCSPMM_ERR_INVALID_ARG;
+    for (uint64_t e = 0; e < row_ptr[n]; ++e)
+      if (col_idx[e] >= n) return ACCSPMM_ERR_INVALID_ARG;
+    try {
+      const RcmGraph g{n, row_ptr, col_idx};
+      // stamps: 1 = placed in the final order; >= 2 the search passes
+      std::vector<uint32_t> mark(n, 0), order, nbr;
+      order.reserve(n);
+      std::vector<uint32_t> out;
+      out.reserve(n);
+      // component roots are taken in increasing degree (ties: index)
+      std::vector<uint32_t> by_deg(n);
+      std::iota(by_deg.begin(), by_deg.end(), 0u);
+      std::stable_sort(by_deg.begin(), by_deg.end(),
+                       [&](uint32_t a, uint32_t b) { return g.deg(a) < g.deg(b); });
+      uint32_t stamp = 1;
+      // a fresh search stamp; before the counter wraps, clear every search
+      // mark (placed nodes keep 1) so an old stamp never reads as current
+      auto next_stamp = [&]() {
+        if (stamp == 0xFFFFFFFFu) {
+          for (uint32_t& m : mark)
+            if (m != 1) m = 0;
+          stamp = 1;
         }
-        std::sort(nbr.begin(), nbr.end(), [&](uint32_t a, uint32_t b) {
-          return g.deg(a) != g.deg(b) ? g.deg(a) < g.deg(b) : a < b;
-        });
-        out.insert(out.end(), nbr.begin(), nbr.end());
+        return ++stamp;
+      };
+      for (uint32_t root : by_deg) {
+        if (mark[root] == 1) continue;
+        // pseudo-peripheral node: move to a smallest-degree node of the last
+        // level while that deepens the level structure (bounded passes)
+        uint32_t r = root;
+        size_t last = 0;
+        uint32_t depth = rcm_levels(g, r, mark, next_stamp(), order, &last);
+        for (int pass = 0; pass < 8; ++pass) {
+          uint32_t best = order[last];
+          for (size_t i = last; i < order.size(); ++i)
+            if (g.deg(order[i]) < g.deg(best) || (g.deg(order[i]) == g.deg(best) && order[i] < best))
+              best = order[i];
+          size_t last2 = 0;
+          std::vector<uint32_t> order2;
+          order2.reserve(order.size());
+          const uint32_t d2 = rcm_levels(g, best, mark, next_stamp(), order2, &last2);
+          if (d2 <= depth) break;
+          r = best;
+          depth = d2;
+          order.swap(order2);
+          last = last2;
+        }
+        // Cuthill-McKee from r over the component
+        const size_t first = out.size();
+        out.push_back(r);
+        mark[r] = 1;
+        for (size_t i = first; i < out.size(); ++i) {
+          const uint32_t v = out[i];
+          nbr.clear();
+          for (uint32_t e = g.rp[v]; e < g.rp[v + 1]; ++e) {
+            const uint32_t w = g.ci[e];
+            if (mark[w] != 1) {
+              mark[w] = 1;
+              nbr.push_back(w);
+            }
+          }
+          std::sort(nbr.begin(), nbr.end(), [&](uint32_t a, uint32_t b) {
+            return g.deg(a) != g.deg(b) ? g.deg(a) < g.deg(b) : a < b;
+          });
+          out.insert(out.end(), nbr.begin(), nbr.end());
+        }
       }
+      std::reverse(out.begin(), out.end());
+      std::copy(out.begin(), out.end(), perm);
+    } catch (const std::bad_alloc&) {
+      return ACCSPMM_ERR_OUT_OF_MEMORY;
     }
-    std::reverse(out.begin(), out.end());
-    std::copy(out.begin(), out.end(), perm);
-  } catch (const std::bad_alloc&) {
-    return ACCSPMM_ERR_OUT_OF_MEMORY;
-  }
-  return ACCSPMM_OK;
+    return ACCSPMM_OK;
+  });
 }

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/accspmm.h"
+#include "guard.hpp"
 
 #define ACC_HIP(expr)                                        \
   do {                                                       \

@@ -77,58 +77,60 @@ extern "C" accspmm_status accspmm_spmm_noalloc(const accspmm_metcf* h, const flo
                                                uint32_t ldb, float* C, uint32_t ldc,
                                                uint32_t K, accspmm_form form,
                                                const accspmm_tile* tile, void* stream) {
-  if (!h || !B || !C || K == 0) return ACCSPMM_ERR_INVALID_ARG;
-  if (ldb < K || ldc < K) return ACCSPMM_ERR_SHAPE;
-  if ((int)form < 0 || (int)form > 2) return ACCSPMM_ERR_INVALID_ARG;
-  // the resident-window sweep keeps a device plan per handle: not offered
-  if (tile && tile->feat_per_lane && tile->sweep) return ACCSPMM_ERR_UNSUPPORTED;
-  hipStream_t s = (hipStream_t)stream;
-  accspmm_tile t{};
-  ACC_TRY(spmm_fit_tile(h, B, ldb, C, ldc, K, (int)form, tile, &t));
-  t.sweep = 0;
-  ACC_TRY(spmm_check_operands(B, ldb, C, ldc, (int)form, t, 3));
-  void* base = nullptr;
-  ACC_TRY(scratch_address(&base));
-  char* sp = static_cast<char*>(base);
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!h || !B || !C || K == 0) return ACCSPMM_ERR_INVALID_ARG;
+    if (ldb < K || ldc < K) return ACCSPMM_ERR_SHAPE;
+    if ((int)form < 0 || (int)form > 2) return ACCSPMM_ERR_INVALID_ARG;
+    // the resident-window sweep keeps a device plan per handle: not offered
+    if (tile && tile->feat_per_lane && tile->sweep) return ACCSPMM_ERR_UNSUPPORTED;
+    hipStream_t s = (hipStream_t)stream;
+    accspmm_tile t{};
+    ACC_TRY(spmm_fit_tile(h, B, ldb, C, ldc, K, (int)form, tile, &t));
+    t.sweep = 0;
+    ACC_TRY(spmm_check_operands(B, ldb, C, ldc, (int)form, t, 3));
+    void* base = nullptr;
+    ACC_TRY(scratch_address(&base));
+    char* sp = static_cast<char*>(base);
 
-  std::lock_guard<std::mutex> lock(h->mu);
-  // the handle's device schedule when an earlier call built it; otherwise
-  // one planned on the host (kept in the handle's host memory for the next
-  // calls) and uploaded into the scratch array
-  const Schedule* sch = find_schedule(h, t);
-  const WorkUnit* d_units = nullptr;
-  const uint32_t* d_sinfo = nullptr;
-  uint64_t off = 0;
-  if (sch) {
-    d_units = sch->units.as<WorkUnit>();
-    d_sinfo = sch->split_info.as<uint32_t>();
-  } else {
-    const auto key = schedule_key(t);
-    auto it = h->host_schedules.find(key);
-    if (it == h->host_schedules.end()) {
-      // the panel cut search writes W * (P - 1) words, before the layout
-      // below is in use
-      if (panel_bound_words(h, t) * 4 > kScratchBytes) return ACCSPMM_ERR_SCRATCH;
-      auto plan = std::make_unique<Schedule>();
-      ACC_TRY(plan_schedule(h, t, s, reinterpret_cast<uint32_t*>(sp), plan.get()));
-      it = h->host_schedules.emplace(key, std::move(plan)).first;
+    std::lock_guard<std::mutex> lock(h->mu);
+    // the handle's device schedule when an earlier call built it; otherwise
+    // one planned on the host (kept in the handle's host memory for the next
+    // calls) and uploaded into the scratch array
+    const Schedule* sch = find_schedule(h, t);
+    const WorkUnit* d_units = nullptr;
+    const uint32_t* d_sinfo = nullptr;
+    uint64_t off = 0;
+    if (sch) {
+      d_units = sch->units.as<WorkUnit>();
+      d_sinfo = sch->split_info.as<uint32_t>();
+    } else {
+      const auto key = schedule_key(t);
+      auto it = h->host_schedules.find(key);
+      if (it == h->host_schedules.end()) {
+        // the panel cut search writes W * (P - 1) words, before the layout
+        // below is in use
+        if (panel_bound_words(h, t) * 4 > kScratchBytes) return ACCSPMM_ERR_SCRATCH;
+        auto plan = std::make_unique<Schedule>();
+        ACC_TRY(plan_schedule(h, t, s, reinterpret_cast<uint32_t*>(sp), plan.get()));
+        it = h->host_schedules.emplace(key, std::move(plan)).first;
+      }
+      sch = it->second.get();
+      const uint64_t ub = align256(schedule_unit_bytes(*sch));
+      const uint64_t sb = align256(schedule_sinfo_bytes(*sch));
+      if (ub + sb > kScratchBytes) return ACCSPMM_ERR_SCRATCH;
+      d_units = reinterpret_cast<const WorkUnit*>(sp);
+      d_sinfo = reinterpret_cast<const uint32_t*>(sp + ub);
+      off = ub + sb;
     }
-    sch = it->second.get();
-    const uint64_t ub = align256(schedule_unit_bytes(*sch));
-    const uint64_t sb = align256(schedule_sinfo_bytes(*sch));
-    if (ub + sb > kScratchBytes) return ACCSPMM_ERR_SCRATCH;
-    d_units = reinterpret_cast<const WorkUnit*>(sp);
-    d_sinfo = reinterpret_cast<const uint32_t*>(sp + ub);
-    off = ub + sb;
-  }
-  uint64_t pbytes = 0, cbytes = 0;
-  spmm_workspace_need(h, sch, K, (int)form, t, &pbytes, &cbytes);
-  if (off + pbytes + cbytes > kScratchBytes) return ACCSPMM_ERR_SCRATCH;
-  if (off)
-    ACC_TRY(upload_schedule(*sch, const_cast<WorkUnit*>(d_units),
-                            const_cast<uint32_t*>(d_sinfo), s));
-  SpmmWorkspace w;
-  w.partials = reinterpret_cast<float*>(sp + off);
-  w.bconv = sp + off + pbytes;
-  return spmm_run(h, sch, d_units, d_sinfo, B, ldb, C, ldc, K, (int)form, t, s, 3, w);
+    uint64_t pbytes = 0, cbytes = 0;
+    spmm_workspace_need(h, sch, K, (int)form, t, &pbytes, &cbytes);
+    if (off + pbytes + cbytes > kScratchBytes) return ACCSPMM_ERR_SCRATCH;
+    if (off)
+      ACC_TRY(upload_schedule(*sch, const_cast<WorkUnit*>(d_units),
+                              const_cast<uint32_t*>(d_sinfo), s));
+    SpmmWorkspace w;
+    w.partials = reinterpret_cast<float*>(sp + off);
+    w.bconv = sp + off + pbytes;
+    return spmm_run(h, sch, d_units, d_sinfo, B, ldb, C, ldc, K, (int)form, t, s, 3, w);
+  });
 }

@@ -55,18 +55,20 @@ extern "C" accspmm_status accspmm_gather_probe(const void* region, uint32_t regi
                                                uint32_t row_bytes, const uint32_t* idx,
                                                uint32_t waves, uint32_t rows_per_wave,
                                                uint32_t* out, void* stream) {
-  if (!region || !idx || !out || region_rows == 0 || waves == 0 || waves % 4 != 0 ||
-      rows_per_wave == 0 || rows_per_wave % 64 != 0)
-    return ACCSPMM_ERR_INVALID_ARG;
-  if (row_bytes != 256 && row_bytes != 512) return ACCSPMM_ERR_UNSUPPORTED;
-  if ((uint64_t)region_rows * row_bytes > 0x7FFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
-  const uint32_t bytes = region_rows * row_bytes;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const dim3 grid(waves / 4), blk(256);
-  if (row_bytes == 512)
-    hipLaunchKernelGGL(k_gather_probe<2>, grid, blk, 0, s, region, bytes, idx, rows_per_wave, out);
-  else
-    hipLaunchKernelGGL(k_gather_probe<1>, grid, blk, 0, s, region, bytes, idx, rows_per_wave, out);
-  ACC_HIP(hipGetLastError());
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!region || !idx || !out || region_rows == 0 || waves == 0 || waves % 4 != 0 ||
+        rows_per_wave == 0 || rows_per_wave % 64 != 0)
+      return ACCSPMM_ERR_INVALID_ARG;
+    if (row_bytes != 256 && row_bytes != 512) return ACCSPMM_ERR_UNSUPPORTED;
+    if ((uint64_t)region_rows * row_bytes > 0x7FFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
+    const uint32_t bytes = region_rows * row_bytes;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid(waves / 4), blk(256);
+    if (row_bytes == 512)
+      hipLaunchKernelGGL(k_gather_probe<2>, grid, blk, 0, s, region, bytes, idx, rows_per_wave, out);
+    else
+      hipLaunchKernelGGL(k_gather_probe<1>, grid, blk, 0, s, region, bytes, idx, rows_per_wave, out);
+    ACC_HIP(hipGetLastError());
+    return ACCSPMM_OK;
+  });
 }

@@ -100,53 +100,55 @@ bool monotone(const std::vector<uint32_t>& v, uint32_t last) {
 }  // namespace
 
 extern "C" accspmm_status accspmm_metcf_save(const accspmm_metcf* h, const char* path) {
-  if (!h || !path) return ACCSPMM_ERR_INVALID_ARG;
-  const uint32_t T = h->T, W = h->W, words = h->words;
-  // the loader's bound on sparseA2B (8T entries counted in 32 bits)
-  if ((uint64_t)T * kColWindow > 0xFFFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
-  const bool adp = h->grp.p != nullptr;
-  std::vector<uint32_t> rwo, tco, a2b, grp, rix, order;
-  std::vector<unsigned long long> bits;
-  std::vector<float> data;
-  const bool reordered = h->row_perm.p != nullptr;
-  ACC_HIP(hipSetDevice(h->device));
-  ACC_TRY(to_host(rwo, h->rwo, (size_t)W + 1));
-  ACC_TRY(to_host(tco, h->tco, (size_t)T + 1));
-  ACC_TRY(to_host(bits, h->bits, (size_t)T * words));
-  ACC_TRY(to_host(a2b, h->a2b, (size_t)T * kColWindow));
-  ACC_TRY(to_host(data, h->data, (size_t)h->nnz));
-  if (adp) {
-    ACC_TRY(to_host(grp, h->grp, (size_t)h->G + 1));
-    ACC_TRY(to_host(rix, h->rix, (size_t)T));
-  }
-  if (reordered) ACC_TRY(to_host(order, h->row_perm, (size_t)h->rows));
-  std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "wb"), &std::fclose);
-  if (!f) return ACCSPMM_ERR_FILE;
-  Writer w;
-  w.f = f.get();
-  const uint32_t hdr[kHeaderWords] = {
-      h->rows, h->cols, h->nnz, h->rw, W, T, h->G, h->ibd, h->balanced,
-      h->max_window_tiles, adp ? 1u : 0u, h->opts.row_window, h->opts.col_window,
-      h->opts.t_load, h->opts.t_comp, h->opts.t_write, h->opts.target_num,
-      h->opts.imbalance_threshold, (uint32_t)h->hist_key.size(), reordered ? 1u : 0u,
-      h->col_perm.p ? h->opts.symmetric_order : 0u};
-  w.put(kMagic, sizeof(kMagic));
-  w.put(&kVersion, 4);
-  w.put(hdr, sizeof(hdr));
-  w.put(h->hist_key.data(), h->hist_key.size());
-  w.put(rwo.data(), rwo.size() * 4);
-  w.put(tco.data(), tco.size() * 4);
-  w.put(bits.data(), bits.size() * 8);
-  w.put(a2b.data(), a2b.size() * 4);
-  w.put(data.data(), data.size() * 4);
-  if (adp) {
-    w.put(grp.data(), grp.size() * 4);
-    w.put(rix.data(), rix.size() * 4);
-  }
-  if (reordered) w.put(order.data(), order.size() * 4);
-  const uint64_t sum = w.sum.h;
-  if (w.ok) w.ok = std::fwrite(&sum, 1, 8, f.get()) == 8;
-  return w.ok ? ACCSPMM_OK : ACCSPMM_ERR_FILE;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!h || !path) return ACCSPMM_ERR_INVALID_ARG;
+    const uint32_t T = h->T, W = h->W, words = h->words;
+    // the loader's bound on sparseA2B (8T entries counted in 32 bits)
+    if ((uint64_t)T * kColWindow > 0xFFFFFFFFull) return ACCSPMM_ERR_UNSUPPORTED;
+    const bool adp = h->grp.p != nullptr;
+    std::vector<uint32_t> rwo, tco, a2b, grp, rix, order;
+    std::vector<unsigned long long> bits;
+    std::vector<float> data;
+    const bool reordered = h->row_perm.p != nullptr;
+    ACC_HIP(hipSetDevice(h->device));
+    ACC_TRY(to_host(rwo, h->rwo, (size_t)W + 1));
+    ACC_TRY(to_host(tco, h->tco, (size_t)T + 1));
+    ACC_TRY(to_host(bits, h->bits, (size_t)T * words));
+    ACC_TRY(to_host(a2b, h->a2b, (size_t)T * kColWindow));
+    ACC_TRY(to_host(data, h->data, (size_t)h->nnz));
+    if (adp) {
+      ACC_TRY(to_host(grp, h->grp, (size_t)h->G + 1));
+      ACC_TRY(to_host(rix, h->rix, (size_t)T));
+    }
+    if (reordered) ACC_TRY(to_host(order, h->row_perm, (size_t)h->rows));
+    std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "wb"), &std::fclose);
+    if (!f) return ACCSPMM_ERR_FILE;
+    Writer w;
+    w.f = f.get();
+    const uint32_t hdr[kHeaderWords] = {
+        h->rows, h->cols, h->nnz, h->rw, W, T, h->G, h->ibd, h->balanced,
+        h->max_window_tiles, adp ? 1u : 0u, h->opts.row_window, h->opts.col_window,
+        h->opts.t_load, h->opts.t_comp, h->opts.t_write, h->opts.target_num,
+        h->opts.imbalance_threshold, (uint32_t)h->hist_key.size(), reordered ? 1u : 0u,
+        h->col_perm.p ? h->opts.symmetric_order : 0u};
+    w.put(kMagic, sizeof(kMagic));
+    w.put(&kVersion, 4);
+    w.put(hdr, sizeof(hdr));
+    w.put(h->hist_key.data(), h->hist_key.size());
+    w.put(rwo.data(), rwo.size() * 4);
+    w.put(tco.data(), tco.size() * 4);
+    w.put(bits.data(), bits.size() * 8);
+    w.put(a2b.data(), a2b.size() * 4);
+    w.put(data.data(), data.size() * 4);
+    if (adp) {
+      w.put(grp.data(), grp.size() * 4);
+      w.put(rix.data(), rix.size() * 4);
+    }
+    if (reordered) w.put(order.data(), order.size() * 4);
+    const uint64_t sum = w.sum.h;
+    if (w.ok) w.ok = std::fwrite(&sum, 1, 8, f.get()) == 8;
+    return w.ok ? ACCSPMM_OK : ACCSPMM_ERR_FILE;
+  });
 }
 
 namespace {
@@ -279,12 +281,14 @@ accspmm_status load_impl(const char* path, accspmm_metcf** out) {
 
 extern "C" accspmm_status accspmm_metcf_load(const char* path, accspmm_metcf** out,
                                              void* stream) {
-  if (!path || !out) return ACCSPMM_ERR_INVALID_ARG;
-  *out = nullptr;
-  (void)stream;
-  try {
-    return load_impl(path, out);
-  } catch (const std::bad_alloc&) {
-    return ACCSPMM_ERR_OUT_OF_MEMORY;
-  }
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!path || !out) return ACCSPMM_ERR_INVALID_ARG;
+    *out = nullptr;
+    (void)stream;
+    try {
+      return load_impl(path, out);
+    } catch (const std::bad_alloc&) {
+      return ACCSPMM_ERR_OUT_OF_MEMORY;
+    }
+  });
 }

@@ -246,27 +246,29 @@ extern "C" void accspmm_server_opts_default(accspmm_server_opts* o) {
 
 extern "C" accspmm_status accspmm_server_create(const accspmm_server_opts* opts,
                                                 accspmm_server** out) {
-  if (!out) return ACCSPMM_ERR_INVALID_ARG;
-  *out = nullptr;
-  accspmm_server_opts o;
-  if (opts) o = *opts; else accspmm_server_opts_default(&o);
-  if (o.form > 2 || o.max_inflight == 0 || o.max_inflight > 8 || o.max_queue == 0)
-    return ACCSPMM_ERR_INVALID_ARG;
-  if (o.one_per_launch) o.max_inflight = 1;
-  auto s = std::make_unique<accspmm_server>();
-  s->o = o;
-  ACC_HIP(hipGetDevice(&s->device));
-  ACC_HIP(hipStreamCreateWithFlags(&s->setup, hipStreamNonBlocking));
-  for (uint32_t i = 0; i < o.max_inflight; ++i) {
-    auto sl = std::make_unique<Slot>();
-    ACC_HIP(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
-    ACC_HIP(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
-    s->slots.push_back(std::move(sl));
-  }
-  accspmm_server* raw = s.release();
-  raw->worker = std::thread([raw] { raw->run(); });
-  *out = raw;
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!out) return ACCSPMM_ERR_INVALID_ARG;
+    *out = nullptr;
+    accspmm_server_opts o;
+    if (opts) o = *opts; else accspmm_server_opts_default(&o);
+    if (o.form > 2 || o.max_inflight == 0 || o.max_inflight > 8 || o.max_queue == 0)
+      return ACCSPMM_ERR_INVALID_ARG;
+    if (o.one_per_launch) o.max_inflight = 1;
+    auto s = std::make_unique<accspmm_server>();
+    s->o = o;
+    ACC_HIP(hipGetDevice(&s->device));
+    ACC_HIP(hipStreamCreateWithFlags(&s->setup, hipStreamNonBlocking));
+    for (uint32_t i = 0; i < o.max_inflight; ++i) {
+      auto sl = std::make_unique<Slot>();
+      ACC_HIP(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+      ACC_HIP(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
+      s->slots.push_back(std::move(sl));
+    }
+    accspmm_server* raw = s.release();
+    raw->worker = std::thread([raw] { raw->run(); });
+    *out = raw;
+    return ACCSPMM_OK;
+  });
 }
 
 namespace {
@@ -304,166 +306,186 @@ extern "C" accspmm_status accspmm_server_register_opts(accspmm_server* s,
                                                        const accspmm_csr_view* csr,
                                                        const accspmm_build_opts* opts,
                                                        uint32_t* graph_id) {
-  if (!s || !csr || !graph_id) return ACCSPMM_ERR_INVALID_ARG;
-  accspmm_build_opts bo;
-  if (opts) {
-    bo = *opts;
-  } else {
-    // served graphs are never exported: the library's automatic layout
-    accspmm_build_opts_default(&bo);
-    bo.row_window = 0;
-    bo.reorder_rows = 2;
-  }
-  bo.build_adpbme = 0;  // the served kernels use work units, not AdpBME groups
-  accspmm_metcf* h = nullptr;
-  ACC_TRY(accspmm_metcf_build(csr, &bo, &h, s->setup));
-  return register_handle(s, h, graph_id);
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!s || !csr || !graph_id) return ACCSPMM_ERR_INVALID_ARG;
+    accspmm_build_opts bo;
+    if (opts) {
+      bo = *opts;
+    } else {
+      // served graphs are never exported: the library's automatic layout
+      accspmm_build_opts_default(&bo);
+      bo.row_window = 0;
+      bo.reorder_rows = 2;
+    }
+    bo.build_adpbme = 0;  // the served kernels use work units, not AdpBME groups
+    accspmm_metcf* h = nullptr;
+    ACC_TRY(accspmm_metcf_build(csr, &bo, &h, s->setup));
+    return register_handle(s, h, graph_id);
+  });
 }
 
 extern "C" accspmm_status accspmm_server_register(accspmm_server* s,
                                                   const accspmm_csr_view* csr,
                                                   uint32_t* graph_id) {
-  return accspmm_server_register_opts(s, csr, nullptr, graph_id);
+  return accspmm::guard([&]() -> accspmm_status {
+    return accspmm_server_register_opts(s, csr, nullptr, graph_id);
+  });
 }
 
 extern "C" accspmm_status accspmm_server_register_file(accspmm_server* s, const char* path,
                                                        uint32_t* graph_id) {
-  if (!s || !path || !graph_id) return ACCSPMM_ERR_INVALID_ARG;
-  accspmm_metcf* h = nullptr;
-  ACC_TRY(accspmm_metcf_load(path, &h, s->setup));
-  return register_handle(s, h, graph_id);
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!s || !path || !graph_id) return ACCSPMM_ERR_INVALID_ARG;
+    accspmm_metcf* h = nullptr;
+    ACC_TRY(accspmm_metcf_load(path, &h, s->setup));
+    return register_handle(s, h, graph_id);
+  });
 }
 
 extern "C" accspmm_status accspmm_server_unregister(accspmm_server* s,
                                                     uint32_t graph_id) {
-  if (!s) return ACCSPMM_ERR_INVALID_ARG;
-  accspmm_metcf* h = nullptr;
-  {
-    std::lock_guard<std::mutex> lock(s->mu);
-    auto it = s->graphs.find(graph_id);
-    if (it == s->graphs.end()) return ACCSPMM_ERR_NOT_FOUND;
-    if (it->second.users) return ACCSPMM_ERR_NOT_READY;
-    h = it->second.h;
-    s->resident -= it->second.bytes;
-    s->graphs.erase(it);
-  }
-  return accspmm_metcf_destroy(h);
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!s) return ACCSPMM_ERR_INVALID_ARG;
+    accspmm_metcf* h = nullptr;
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      auto it = s->graphs.find(graph_id);
+      if (it == s->graphs.end()) return ACCSPMM_ERR_NOT_FOUND;
+      if (it->second.users) return ACCSPMM_ERR_NOT_READY;
+      h = it->second.h;
+      s->resident -= it->second.bytes;
+      s->graphs.erase(it);
+    }
+    return accspmm_metcf_destroy(h);
+  });
 }
 
 extern "C" accspmm_status accspmm_server_submit(accspmm_server* s, uint32_t graph_id,
                                                 const float* B, uint32_t ldb, float* C,
                                                 uint32_t ldc, uint32_t K,
                                                 uint64_t* request_id) {
-  if (!s || !request_id || !B || !C || K == 0) return ACCSPMM_ERR_INVALID_ARG;
-  const auto t0 = Clock::now();
-  accspmm_metcf* h = nullptr;
-  {
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!s || !request_id || !B || !C || K == 0) return ACCSPMM_ERR_INVALID_ARG;
+    const auto t0 = Clock::now();
+    accspmm_metcf* h = nullptr;
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      auto it = s->graphs.find(graph_id);
+      if (it == s->graphs.end()) return ACCSPMM_ERR_NOT_FOUND;
+      if (s->pending.size() + s->pending_large.size() >= s->o.max_queue)
+        return ACCSPMM_ERR_QUEUE_FULL;
+      h = it->second.h;
+      it->second.users++;  // keeps the graph registered until completion
+    }
+    Req r;
+    r.gid = graph_id;
+    r.prob = accspmm_problem{h, B, ldb, C, ldc, K};
+    r.t0 = t0;
+    accspmm_status st = problem_plan(r.prob, (int)s->o.form, s->setup, &r.plan);
     std::lock_guard<std::mutex> lock(s->mu);
-    auto it = s->graphs.find(graph_id);
-    if (it == s->graphs.end()) return ACCSPMM_ERR_NOT_FOUND;
-    if (s->pending.size() + s->pending_large.size() >= s->o.max_queue)
-      return ACCSPMM_ERR_QUEUE_FULL;
-    h = it->second.h;
-    it->second.users++;  // keeps the graph registered until completion
-  }
-  Req r;
-  r.gid = graph_id;
-  r.prob = accspmm_problem{h, B, ldb, C, ldc, K};
-  r.t0 = t0;
-  accspmm_status st = problem_plan(r.prob, (int)s->o.form, s->setup, &r.plan);
-  std::lock_guard<std::mutex> lock(s->mu);
-  auto g = s->graphs.find(graph_id);
-  if (st != ACCSPMM_OK) {
-    if (g != s->graphs.end()) g->second.users--;
-    return st;
-  }
-  r.ws = s->o.one_per_launch ? 0 : r.plan.workspace_bytes();
-  if (s->o.one_per_launch) {
-    // the standalone path converts into the graph's own workspace
-    r.ws = r.plan.bconv_bytes + r.plan.partial_bytes;
-  }
-  if (s->resident + r.ws > s->o.hbm_budget_bytes) {
-    g->second.users--;
-    s->rejected_budget++;
-    return ACCSPMM_ERR_BUDGET;  // can never fit
-  }
-  const uint64_t id = s->next_rid++;
-  const uint64_t large_work = s->o.large_work ? s->o.large_work : (1ull << 30);
-  r.large = !s->o.one_per_launch && (uint64_t)h->nnz * K >= large_work;
-  s->reqs.emplace(id, r);
-  (r.large ? s->pending_large : s->pending).push_back(id);
-  s->submitted++;
-  *request_id = id;
-  s->cv_work.notify_one();
-  return ACCSPMM_OK;
+    auto g = s->graphs.find(graph_id);
+    if (st != ACCSPMM_OK) {
+      if (g != s->graphs.end()) g->second.users--;
+      return st;
+    }
+    r.ws = s->o.one_per_launch ? 0 : r.plan.workspace_bytes();
+    if (s->o.one_per_launch) {
+      // the standalone path converts into the graph's own workspace
+      r.ws = r.plan.bconv_bytes + r.plan.partial_bytes;
+    }
+    if (s->resident + r.ws > s->o.hbm_budget_bytes) {
+      g->second.users--;
+      s->rejected_budget++;
+      return ACCSPMM_ERR_BUDGET;  // can never fit
+    }
+    const uint64_t id = s->next_rid++;
+    const uint64_t large_work = s->o.large_work ? s->o.large_work : (1ull << 30);
+    r.large = !s->o.one_per_launch && (uint64_t)h->nnz * K >= large_work;
+    s->reqs.emplace(id, r);
+    (r.large ? s->pending_large : s->pending).push_back(id);
+    s->submitted++;
+    *request_id = id;
+    s->cv_work.notify_one();
+    return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_server_poll(accspmm_server* s, uint64_t request_id,
                                               double* latency_s) {
-  if (!s) return ACCSPMM_ERR_INVALID_ARG;
-  std::lock_guard<std::mutex> lock(s->mu);
-  auto it = s->reqs.find(request_id);
-  if (it == s->reqs.end()) return ACCSPMM_ERR_NOT_FOUND;
-  if (it->second.state < 2) return ACCSPMM_ERR_NOT_READY;
-  const bool ok = it->second.state == 2;
-  if (latency_s) *latency_s = it->second.latency;
-  s->reqs.erase(it);
-  return ok ? ACCSPMM_OK : ACCSPMM_ERR_HIP;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!s) return ACCSPMM_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->reqs.find(request_id);
+    if (it == s->reqs.end()) return ACCSPMM_ERR_NOT_FOUND;
+    if (it->second.state < 2) return ACCSPMM_ERR_NOT_READY;
+    const bool ok = it->second.state == 2;
+    if (latency_s) *latency_s = it->second.latency;
+    s->reqs.erase(it);
+    return ok ? ACCSPMM_OK : ACCSPMM_ERR_HIP;
+  });
 }
 
 extern "C" accspmm_status accspmm_server_drain(accspmm_server* s) {
-  if (!s) return ACCSPMM_ERR_INVALID_ARG;
-  std::unique_lock<std::mutex> lock(s->mu);
-  s->cv_work.notify_one();
-  s->cv_idle.wait(lock, [s] {
-    if ((!s->pending.empty() || !s->pending_large.empty()) && !s->paused) return false;
-    for (auto& sp : s->slots)
-      if (sp->busy) return false;
-    return true;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!s) return ACCSPMM_ERR_INVALID_ARG;
+    std::unique_lock<std::mutex> lock(s->mu);
+    s->cv_work.notify_one();
+    s->cv_idle.wait(lock, [s] {
+      if ((!s->pending.empty() || !s->pending_large.empty()) && !s->paused) return false;
+      for (auto& sp : s->slots)
+        if (sp->busy) return false;
+      return true;
+    });
+    return ACCSPMM_OK;
   });
-  return ACCSPMM_OK;
 }
 
 extern "C" accspmm_status accspmm_server_pause(accspmm_server* s, int paused) {
-  if (!s) return ACCSPMM_ERR_INVALID_ARG;
-  std::lock_guard<std::mutex> lock(s->mu);
-  s->paused = paused != 0;
-  s->cv_work.notify_one();
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!s) return ACCSPMM_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->paused = paused != 0;
+    s->cv_work.notify_one();
+    return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_server_get_stats(accspmm_server* s,
                                                    accspmm_server_stats* st) {
-  if (!s || !st) return ACCSPMM_ERR_INVALID_ARG;
-  std::lock_guard<std::mutex> lock(s->mu);
-  st->submitted = s->submitted;
-  st->completed = s->completed;
-  st->launches = s->launches;
-  st->rejected_budget = s->rejected_budget;
-  st->resident_bytes = s->resident;
-  st->peak_inflight_bytes = s->peak_inflight;
-  st->busy_s = s->busy_s;
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!s || !st) return ACCSPMM_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(s->mu);
+    st->submitted = s->submitted;
+    st->completed = s->completed;
+    st->launches = s->launches;
+    st->rejected_budget = s->rejected_budget;
+    st->resident_bytes = s->resident;
+    st->peak_inflight_bytes = s->peak_inflight;
+    st->busy_s = s->busy_s;
+    return ACCSPMM_OK;
+  });
 }
 
 extern "C" accspmm_status accspmm_server_destroy(accspmm_server* s) {
-  if (!s) return ACCSPMM_ERR_INVALID_ARG;
-  {
-    std::lock_guard<std::mutex> lock(s->mu);
-    s->stop = true;
-    s->paused = false;
-    s->cv_work.notify_one();
-  }
-  if (s->worker.joinable()) s->worker.join();
-  for (auto& sp : s->slots) {
-    (void)hipStreamSynchronize(sp->stream);
-    (void)hipEventDestroy(sp->done);
-    if (sp->L.upload_event) (void)hipEventDestroy(sp->L.upload_event);
-    (void)hipStreamDestroy(sp->stream);
-  }
-  s->slots.clear();
-  for (auto& kv : s->graphs) accspmm_metcf_destroy(kv.second.h);
-  if (s->setup) (void)hipStreamDestroy(s->setup);
-  delete s;
-  return ACCSPMM_OK;
+  return accspmm::guard([&]() -> accspmm_status {
+    if (!s) return ACCSPMM_ERR_INVALID_ARG;
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      s->stop = true;
+      s->paused = false;
+      s->cv_work.notify_one();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    for (auto& sp : s->slots) {
+      (void)hipStreamSynchronize(sp->stream);
+      (void)hipEventDestroy(sp->done);
+      if (sp->L.upload_event) (void)hipEventDestroy(sp->L.upload_event);
+      (void)hipStreamDestroy(sp->stream);
+    }
+    s->slots.clear();
+    for (auto& kv : s->graphs) accspmm_metcf_destroy(kv.second.h);
+    if (s->setup) (void)hipStreamDestroy(s->setup);
+    delete s;
+    return ACCSPMM_OK;
+  });
 }

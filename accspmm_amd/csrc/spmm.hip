@@ -774,7 +774,9 @@ extern "C" accspmm_status accspmm_spmm(const accspmm_metcf* h, const float* B,
                                        uint32_t K, accspmm_form form,
                                        const accspmm_tile* tile,
                                        void* stream) {
-  return spmm_entry(h, B, ldb, C, ldc, K, form, tile, stream, 3);
+  return accspmm::guard([&]() -> accspmm_status {
+    return spmm_entry(h, B, ldb, C, ldc, K, form, tile, stream, 3);
+  });
 }
 
 extern "C" accspmm_status accspmm_spmm_prepare_b(const accspmm_metcf* h,
@@ -782,7 +784,9 @@ extern "C" accspmm_status accspmm_spmm_prepare_b(const accspmm_metcf* h,
                                                  uint32_t K, accspmm_form form,
                                                  const accspmm_tile* tile,
                                                  void* stream) {
-  return spmm_entry(h, B, ldb, nullptr, K, K, form, tile, stream, 1);
+  return accspmm::guard([&]() -> accspmm_status {
+    return spmm_entry(h, B, ldb, nullptr, K, K, form, tile, stream, 1);
+  });
 }
 
 extern "C" accspmm_status accspmm_spmm_prepared(const accspmm_metcf* h,
@@ -791,5 +795,7 @@ extern "C" accspmm_status accspmm_spmm_prepared(const accspmm_metcf* h,
                                                 uint32_t K, accspmm_form form,
                                                 const accspmm_tile* tile,
                                                 void* stream) {
-  return spmm_entry(h, B, ldb, C, ldc, K, form, tile, stream, 2);
+  return accspmm::guard([&]() -> accspmm_status {
+    return spmm_entry(h, B, ldb, C, ldc, K, form, tile, stream, 2);
+  });
 }
