@@ -28,7 +28,9 @@ typedef enum accspmm_status {
   ACCSPMM_ERR_INVALID_ARG = 1,    /* null pointer, K == 0, bad option value   */
   ACCSPMM_ERR_NOT_CANONICAL = 2,  /* CSR row not strictly increasing in col   */
   ACCSPMM_ERR_HIP = 3,            /* a HIP runtime call failed                */
-  ACCSPMM_ERR_OUT_OF_MEMORY = 4,  /* device allocation failed                 */
+  ACCSPMM_ERR_OUT_OF_MEMORY = 4,  /* device or host allocation failed (or a
+                                     host thread could not start); no C++
+                                     exception leaves an entry point         */
   ACCSPMM_ERR_UNSUPPORTED = 5,    /* option outside the supported set         */
   ACCSPMM_ERR_BUDGET = 6,         /* serving: HBM budget exhausted            */
   ACCSPMM_ERR_NOT_FOUND = 7,      /* serving: unknown graph or request id     */
