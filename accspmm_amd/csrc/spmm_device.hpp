@@ -181,20 +181,25 @@ struct BRow {
 // half in each load).  The output follows the same map.
 template <int FORM, int NB>
 constexpr bool split_halves() { return NB == 8 && FORM != kBF16; }
-// NB = 2 with 4-byte elements (a 32-feature chunk, 128 bytes of a row):
-// lane n of a quarter covers features k0 + 4 (n & 7) + 2 (n >> 3) and the
-// next, so lanes n and n ^ 8 hold the four features of one 16-byte segment
-// (the buffer-load gather loads two rows per instruction and exchanges
-// halves, pair_gather).
+// Chunks of 128-byte rows, 8 bytes a lane (NB = 2 with 4-byte elements, NB
+// = 4 with bf16): lane n of a quarter covers the P = 16 / (2 x element
+// size) features from k0 + 2P (n & 7) + P (n >> 3), so lanes n and n ^ 8
+// hold the 2P features of one 16-byte segment (the buffer-load gather loads
+// two rows per instruction and exchanges halves, pair_gather).
 template <int FORM, int NB>
-constexpr bool pair_rows() { return NB == 2 && FORM != kBF16; }
+constexpr bool pair_rows() {
+  return FORM == kBF16 ? NB == 4 : NB == 2;
+}
 template <int FORM, int NB>
 constexpr uint32_t word_feat(int i) {
   return split_halves<FORM, NB>() && i >= 4 ? 60u + (uint32_t)i : (uint32_t)i;
 }
 template <int FORM, int NB>
 __device__ inline uint32_t lane_feat(uint32_t k0, uint32_t n) {
-  if constexpr (pair_rows<FORM, NB>()) return k0 + 4u * (n & 7u) + 2u * (n >> 3);
+  if constexpr (pair_rows<FORM, NB>()) {
+    constexpr uint32_t P = FORM == kBF16 ? 4u : 2u;
+    return k0 + 2u * P * (n & 7u) + P * (n >> 3);
+  }
   return k0 + (split_halves<FORM, NB>() ? 4u : (uint32_t)NB) * n;
 }
 
@@ -454,12 +459,12 @@ __device__ inline uint32_t from_quad(uint32_t x) {
 
 // Gathers the B rows of the tile's 8 columns and the lane's share of the
 // group's value span; records the lane's A row (bitmap byte, first value).
-// Two-row gathers (pair_rows forms, buffer loads, full chunks): lane n of
-// a quarter loads the 16 bytes at feature 4 (n & 7) of row col[j] (n < 8)
-// or col[j + 1] (n >= 8), j = 0, 2, 4, 6: four loads for the tile's eight
-// rows instead of eight 8-byte ones (232,965-node power-law graph, K = 32:
-// split-BF16 1.107 -> 1.044 ms, FP32 unchanged at 1.12 ms).  The stage
-// keeps the raw words (row j's slot: the first two, row j + 1's: the last
+// Two-row gathers (pair_rows chunks, buffer loads, full chunks): lane n of
+// a quarter loads the 16-byte segment holding its features of row col[j]
+// (n < 8) or col[j + 1] (n >= 8), j = 0, 2, 4, 6: four loads for the tile's
+// eight rows instead of eight 8-byte ones (232,965-node power-law graph:
+// split-BF16 at K = 32 1.107 -> 1.044 ms, BF16 at K = 64 1.137 -> 1.052 ms,
+// FP32 at K = 32 unchanged at 1.12 ms).  The stage keeps the raw words (row j's slot: the first two, row j + 1's: the last
 // two) until exchange_pairs sorts them when it is consumed.
 template <int FORM, int NB, bool FULL, bool BUF>
 constexpr bool pair_gather() {
@@ -512,7 +517,8 @@ __device__ inline void load_data(Stage<FORM, NB>& st, const Meta& mt,
     off = from_quad<3>(mt.m[0]) + from_quad<2>(skip);
   }
   if constexpr (pair_gather<FORM, NB, FULL, BUF>()) {
-    const uint32_t seg = (feat & ~3u) * 4u;
+    constexpr uint32_t es = FORM == kBF16 ? 2u : 4u, seg_feats = 16u / es;
+    const uint32_t seg = (feat & ~(seg_feats - 1u)) * es;
     // all ones in lanes 8-15 of each quarter: a mask select, which the
     // compiler would otherwise turn into a lane-indexed read of col[]
     // (scratch memory)
