@@ -463,8 +463,8 @@ typedef struct accspmm_server_opts {
   uint32_t form;              /* accspmm_form                               */
   uint32_t one_per_launch;    /* 1: baseline mode, one launch per request   */
   uint64_t large_work;        /* requests with nnz * K at or above this run
-                                 in launches of their own, on all slots but
-                                 one (0: 2^30); smaller ones are batched     */
+                                 in launches of their own, one at a time
+                                 (0: 2^30); smaller ones are batched         */
 } accspmm_server_opts;
 
 void accspmm_server_opts_default(accspmm_server_opts* o);
