@@ -411,6 +411,35 @@ def test_edge_shapes(form):
 
 
 @pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
+@pytest.mark.parametrize("K", [24, 32, 40, 64, 96, 100])
+def test_two_row_gathers_match_row_loads(form, K):
+    # 128-byte chunk rows (NB 2 with 4-byte B, NB 4 with BF16): the work-unit
+    # kernel loads two rows per 16-byte buffer load and lanes n, n ^ 8 swap
+    # halves; the sweep kernel loads each lane's features of each row on
+    # their own.  Same C bits (chunk tails included), and within the form's
+    # tolerance of the FP64 oracle
+    torch = _torch()
+    rng = np.random.default_rng(1000 + K)
+    rows, cols = 700, 1500
+    rp, ci, va = random_csr(rng, rows, cols, 0.02, empty_rows=0.1, heavy_rows=2)
+    csr = acc.CSR(rows, cols, rp, ci, va)
+    nb = 4 if form == acc.FORM_BF16 else 2
+    bs = (1, 2) if form == acc.FORM_SPLIT_BF16 else (0,)
+    Bh = acc.gen_dense(K, cols, K)
+    B = torch.from_numpy(Bh).cuda()
+    for rw in (8, 16):
+        g = acc.DeviceGraph(csr, acc.default_build_opts(rw))
+        for b in bs:
+            outs = [g.spmm(B, None, form, acc.Tile(nb, 2, 16, 0, 64, 2, sw, b)).clone()
+                    for sw in (0, 1)]
+            torch.cuda.synchronize()
+            assert torch.equal(outs[0], outs[1]), (rw, b)
+            mx, fr, _ = run_case(csr, K, form, tile=acc.Tile(nb, 2, 16, 0, 64, 2, 0, b), rw=rw,
+                                 graph=g, B=Bh)
+            assert mx <= TOL[form], (rw, b, mx)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=acc.FORM_NAMES.get)
 def test_dense_groups(form):
     # 16-row tiles about 90% full: 4-tile groups of up to 512 values, more
     # than the 256 one span load brings (and full 8-row groups of 256).
