@@ -241,6 +241,34 @@ __global__ void k_group_values(const uint32_t* __restrict__ rwo,
   if (lane == 0) atomicMax(max_values, mx);
 }
 
+// Windows whose condensed columns span less than a 1/8 of B's rows, and
+// non-empty windows (counts[0], counts[1]).  A window's tiles are sorted by
+// column and padding sits at the end of its last tile, so the span is the
+// last tile's last real column minus the first tile's first.
+__global__ void k_window_spans(const uint32_t* __restrict__ rwo,
+                               const uint32_t* __restrict__ a2b, uint32_t W, uint32_t cols,
+                               uint32_t* __restrict__ counts) {
+  uint32_t narrow = 0, used = 0;
+  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < W;
+       w += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t tb = rwo[w], te = rwo[w + 1];
+    if (tb == te) continue;
+    const uint32_t lo = a2b[(uint64_t)tb * kColWindow];
+    uint32_t hi = lo;
+    for (int j = kColWindow - 1; j >= 0; --j) {
+      const uint32_t c = a2b[(uint64_t)(te - 1) * kColWindow + j];
+      if (c != kPad) {
+        hi = c;
+        break;
+      }
+    }
+    ++used;
+    narrow += (uint64_t)(hi - lo) * 8 < cols ? 1u : 0u;
+  }
+  if (used) atomicAdd(&counts[1], used);
+  if (narrow) atomicAdd(&counts[0], narrow);
+}
+
 // ---- AdpBME grouping (include/format.hpp:346,366-382,418-423) ----------
 // Tile t costs cf = t_load+t_comp+t_write when it is the first tile of its
 // window, else co = t_load+t_comp.  A group starting at tile i always takes
@@ -337,6 +365,7 @@ __global__ void k_group_write(const uint32_t* __restrict__ next, uint32_t T,
 // h->max_group_values from the device rowWindowOffset and tcOffset
 accspmm_status measure_group_values(accspmm_metcf* h, hipStream_t s) {
   h->max_group_values = 0;
+  h->banded = false;
   if (h->W == 0 || h->T == 0) return ACCSPMM_OK;
   DevBuf mx;
   ACC_TRY(mx.alloc(4));
@@ -347,8 +376,20 @@ accspmm_status measure_group_values(accspmm_metcf* h, hipStream_t s) {
   ACC_HIP(hipGetLastError());
   uint32_t v = 0;
   ACC_HIP(hipMemcpyAsync(&v, mx.p, 4, hipMemcpyDeviceToHost, s));
+  // banded: most non-empty windows narrow (the work-unit mapping keeps
+  // neighbouring windows on one XCD for such graphs)
+  DevBuf cnt;
+  ACC_TRY(cnt.alloc(8));
+  ACC_HIP(hipMemsetAsync(cnt.p, 0, 8, s));
+  hipLaunchKernelGGL(k_window_spans, dim3(std::min<uint32_t>((h->W + 255) / 256, 1024)),
+                     dim3(256), 0, s, h->rwo.as<uint32_t>(), h->a2b.as<uint32_t>(), h->W,
+                     h->cols, cnt.as<uint32_t>());
+  ACC_HIP(hipGetLastError());
+  uint32_t counts[2] = {0, 0};
+  ACC_HIP(hipMemcpyAsync(counts, cnt.p, 8, hipMemcpyDeviceToHost, s));
   ACC_HIP(hipStreamSynchronize(s));
   h->max_group_values = v;
+  h->banded = counts[1] && 2ull * counts[0] > counts[1];
   return ACCSPMM_OK;
 }
 }  // namespace accspmm

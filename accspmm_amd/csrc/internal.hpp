@@ -145,6 +145,9 @@ struct accspmm_metcf {
   uint32_t W = 0, T = 0, G = 0;
   uint32_t ibd = 0, balanced = 0, max_window_tiles = 0;
   uint32_t max_group_values = 0;  // values of the fullest 4-tile group
+  // most non-empty windows gather B rows within 1/8 of B (a banded
+  // matrix: neighbouring windows share B rows)
+  bool banded = false;
   accspmm_build_opts opts{};
   accspmm::DevBuf rwo, tco, bits, a2b, data, grp, rix;
   // rows reordered at build (opts.reorder_rows or opts.symmetric_order):
@@ -212,6 +215,7 @@ accspmm_status spmm_launch(const accspmm_metcf* h, const float* B, uint32_t ldb,
 void spmm_workspace_need(const accspmm_metcf* h, const Schedule* sch, uint32_t K, int form,
                          const accspmm_tile& tile, uint64_t* partial_bytes,
                          uint64_t* bconv_bytes);
-// sets h->max_group_values (builder.hip); after a build or a load
+// sets h->max_group_values and h->banded (builder.hip); after a build or a
+// load
 accspmm_status measure_group_values(accspmm_metcf* h, hipStream_t s);
 }  // namespace accspmm

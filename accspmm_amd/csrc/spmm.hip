@@ -510,7 +510,12 @@ void default_tile(const accspmm_metcf* h, uint32_t K, int form,
       t->col_panels *= 2;
     }
   } else {
-    t->xcd_slice = 2;
+    // short windows: a banded graph runs one contiguous range of work units
+    // per XCD, so neighbouring windows share their B rows in its L2 (27-point
+    // stencil 100^3, split-BF16 K = 128: RW 16 0.558 against 0.614 ms
+    // round-robin, RW 8 0.652 against 0.751); otherwise round-robin
+    // (2M-node power-law graph, RW 8: 4.121 against 4.271 ms)
+    t->xcd_slice = h->banded ? 2 : 0;
     t->col_panels = 0;
     // packing short windows into units of >= 64 tiles shortens the per-unit
     // pipeline start-up, which pays only while enough units remain to fill
